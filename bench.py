@@ -1,0 +1,136 @@
+"""Throughput benchmark of the FV residual + SSP-RK3 loop on one MI355X.
+
+Workload (the large shock problem): 2-D four-quadrant Riemann problem on
+[0,1]^2 split at (0.8,0.8), 1025 x 1025 jittered-triangle mesh (seed 1,
+1,050,625 vertices), transmissive (outlet) sides, KEPES-TeCNO Rusanov flux,
+minmod reconstruction, SSP-RK3, CFL 0.5, FP64.  A step is one SSP-RK3 time
+step (dt pass + 3 stages of gradient, edge flux, gather + update).  The metric
+is FP64 vertex-stage updates per second = n_vertex * 3 * steps / wall time of
+the timed steps, inputs resident in HBM.
+
+Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(nv, nf, nc):
+    """Algorithmic HBM bytes per launch of each kernel (FP64 8 B, int32 4 B;
+    every array counted once per launch)."""
+    return {
+        # U_old write, W read, dcarea, face/bface CSR, face normals
+        "dt": 76 * nv + 24 * nf,
+        # entropy vars, coords, 1/(6 mcarea), cell ids, vertex->cell CSR, dE write
+        "gradient": 124 * nv + 24 * nc,
+        # face ids + normal, W/dE/coords of the vertices, flux write
+        "flux": 112 * nv + 56 * nf,
+        # CSR, fluxes, W, U_old, dcarea, weight, W + entropy write
+        "update": 148 * nv + 40 * nf,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=334)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1025)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import tensum_amd as T
+    from cases import riemann, riemann_mesh
+
+    n = args.n
+    mesh = riemann_mesh(n, seed=1)
+    prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8,
+                   time_scheme="ssprk3")
+    w0 = riemann(mesh.x, mesh.y)
+    print(f"bench: mesh {n}x{n} ({mesh.n_vertex} vertices), building context", file=sys.stderr,
+          flush=True)
+    s = T.Solver(mesh, prm)
+    s.set_state(w0)
+    geo = s.geometry()
+    nv, nf, nc = mesh.n_vertex, geo["face_v"].shape[0], mesh.cell_v.size // 3
+    s.run(args.warmup)
+    # timed region: K whole steps, synchronised on both sides (tsm_run
+    # synchronises the stream before returning)
+    t_before, it_before = s.time()
+    t0 = time.perf_counter()
+    done = s.run(args.steps)
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    assert int(done[0]) == args.steps, f"only {int(done[0])} steps ran (final time reached?)"
+    value = nv * 3 * args.steps / wall
+    ms_per_step = 1e3 * wall / args.steps
+    print(f"bench: {args.steps} steps in {wall:.3f} s, {ms_per_step:.3f} ms/step", file=sys.stderr,
+          flush=True)
+
+    # per-kernel durations (HIP events on the library's stream)
+    ku = s.bench(20)
+    kb = kernel_bytes(nv, nf, nc)
+    per_step = {"dt": ku["dt_us"], "gradient": 3 * ku["gradient_us"], "flux": 3 * ku["flux_us"],
+                "update": 3 * ku["update_us"]}
+    dom = max(per_step, key=per_step.get)
+    dur_us = ku[{"dt": "dt_us", "gradient": "gradient_us", "flux": "flux_us",
+                 "update": "update_us"}[dom]]
+    achieved = kb[dom] / (dur_us * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None,
+                "kernel_us": {k: round(v, 2) for k, v in ku.items()}}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle_binding as O
+        print("bench: timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
+        o = O.Oracle(mesh, prm)
+        o.set_state(w0)
+        c0 = time.perf_counter()
+        k = o.run(args.cpu_steps)
+        c1 = time.perf_counter()
+        cpu = {"value": nv * 3 * k / (c1 - c0), "unit": "vertex-stage updates/s", "cores": 1,
+               "kind": "port",
+               "sample": f"{k} SSP-RK3 steps of the same {n}x{n} Riemann workload, oracle (C++ "
+                         f"restatement of the reference), single thread"}
+
+    out = {
+        "metric": "FP64 vertex-stage updates/s at 1M verts",
+        "value": value,
+        "unit": "vertex-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered-triangle mesh, four-quadrant Riemann initial state)",
+        "config": {"workload": "2D Riemann, KEPES-TeCNO Rusanov, minmod, SSP-RK3, CFL 0.5",
+                   "mesh": f"{n}x{n} jittered triangles, seed 1", "n_vertex": nv, "n_face": nf,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+/*
+ * tsm.h -- C ABI of libtsm, the MI355X-native vertex-centred finite-volume
+ * residual + Runge-Kutta library (drop-in for the time loop of deepray/TENSUM).
+ *
+ * The reference exposes no FFI: its hot path is the private member functions
+ * of FiniteVolume (src/fv.h:351-403) called in-process on std::vector state.
+ * Each entry point below names the reference function it replaces.  All
+ * signatures use plain C types and host pointers; device memory, streams and
+ * the vertex renumbering stay inside the opaque context.
+ *
+ * Conventions: every call returns TSM_OK (0) or a negative TSM_E* status and
+ * never aborts or exits (the reference calls MPI_Abort / exit(0) instead,
+ * src/mpi_utils.h:23-41).  The message of the last failure on the calling
+ * thread is returned by tsm_last_error().  One host thread per context.
+ * Primitive state is (T, u, v, p) per vertex, vertex order = part-file order.
+ */
+#ifndef TSM_H
+#define TSM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSM_OK 0
+#define TSM_E_ARG -1       /* bad argument / unsupported option          */
+#define TSM_E_MESH -2      /* mesh invalid (non-positive area, tags ...) */
+#define TSM_E_HIP -3       /* HIP runtime failure or no device           */
+#define TSM_E_IO -4        /* file could not be read / written           */
+#define TSM_E_STATE -5     /* non-finite or non-positive T/p detected    */
+
+/* Flux codes follow Material::FluxScheme (src/material.h:70-82). */
+enum { TSM_FLUX_KEPES_TECNO_ROE = 0, TSM_FLUX_KEPES_TECNO_RUSANOV = 1,
+       TSM_FLUX_KEPEC = 2, TSM_FLUX_ROE_EC = 3, TSM_FLUX_KEP = 4,
+       TSM_FLUX_AVG = 5 };
+/* Reconstruction codes follow Material::ReconstructScheme (src/material.h:89-98). */
+enum { TSM_REC_FIRST = 0, TSM_REC_SECOND = 1, TSM_REC_SECOND_MUSCL = 2,
+       TSM_REC_VAN_ALBADA = 3, TSM_REC_MINMOD = 4, TSM_REC_TVB_MINMOD = 5,
+       TSM_REC_ENO2 = 6 };
+/* Boundary types follow BC::BCType (src/bc.h:14-23). */
+enum { TSM_BC_NONE = 0, TSM_BC_SLIP = 1, TSM_BC_NOSLIP = 2,
+       TSM_BC_FARFIELD = 3, TSM_BC_INLET = 4, TSM_BC_OUTLET = 5,
+       TSM_BC_PRESSURE = 6, TSM_BC_PERIODIC = 7 };
+
+/* Solver parameters: the numeric/material/output fields of Parameter
+   (src/parameter.h), already parsed. */
+typedef struct tsm_params {
+  double gamma, gas_const, prandtl;
+  double mu_ref, T_ref, T_0, omega;
+  int model;          /* 0 euler, 1 ns */
+  int mu_model;       /* 0 constant, 1 sutherland, 2 power */
+  int flux;           /* TSM_FLUX_* */
+  int reconstruct;    /* TSM_REC_* */
+  double tvb_M;       /* M_value of tvb_minmod */
+  double ev_alpha, ev_beta;  /* flux_par alpha / beta */
+  int nd_active, visc_active, heat_active;
+  int time_scheme;    /* 0 ssprk3, 1 rk4 */
+  int unsteady;       /* 1 unsteady, 0 steady */
+  double cfl, time_step, final_time, min_residue;
+  long max_iter;
+  int n_time_stamps;
+  int cell_type;      /* 0 median, 1 voronoi */
+  int liou_fix;
+} tsm_params;
+
+/* NPART = 1 part file content (readme/README.md:155-201), 0-based gids. */
+typedef struct tsm_mesh {
+  int n_vertex;
+  const double* x; const double* y;
+  int n_bface;
+  const int* bface_tag;      /* n_bface */
+  const int* bface_v;        /* 2*n_bface */
+  int n_cell;
+  const int* cell_v;         /* 3*n_cell, counter-clockwise */
+  int n_periodic;
+  const int* periodic_ptr;   /* n_periodic+1 */
+  const int* periodic_v;     /* set members */
+  int n_bc;                  /* boundary table: tag -> type + constant state */
+  const int* bc_tag; const int* bc_type; const double* bc_state; /* 4*n_bc (T,u,v,p) */
+} tsm_mesh;
+
+typedef struct tsm_ctx tsm_ctx;
+
+const char* tsm_last_error(void);
+
+/* Builds geometry (replaces Grid::preproc, src/grid_preproc.cc:1147-1194),
+   renumbers, uploads, allocates state for n_samples independent samples. */
+int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_samples,
+               tsm_ctx** out);
+void tsm_destroy(tsm_ctx* ctx);
+
+/* Per-sample gamma / gas constant (batched Monte Carlo over material constants;
+   the reference can vary only IC/BC expressions per sample, src/perturb.cc). */
+int tsm_set_sample_material(tsm_ctx* ctx, int sample, double gamma, double gas_const);
+
+/* Initial state at t = 0 (replaces the IC / restart branch of
+   FiniteVolume::initialize_sample_run, src/fv.cc:320-402). prim: 4*n_vertex. */
+int tsm_set_state(tsm_ctx* ctx, int sample, const double* prim);
+int tsm_get_state(tsm_ctx* ctx, int sample, double* prim);
+
+/* Residual of the current state (FiniteVolume::compute_residual, src/fv.cc:1467). R: 4*n_vertex. */
+int tsm_residual(tsm_ctx* ctx, int sample, double* R);
+/* Vertex entropy-variable gradients (FiniteVolume::compute_gradients, src/fv.cc:699).
+   dE: 8*n_vertex ordered (f,vx,vy,l) x (x,y). */
+int tsm_gradients(tsm_ctx* ctx, int sample, double* dE);
+/* Global time step of the current state (FiniteVolume::compute_dt, src/fv.cc:1499). */
+int tsm_compute_dt(tsm_ctx* ctx, int sample, double* dt_global);
+
+/* Advances every sample by up to nsteps time steps of the loop body of
+   FiniteVolume::solve (src/fv.cc:2273-2368); a sample stops by itself when it
+   reaches final_time.  done[s] receives the steps sample s performed; log
+   (optional, 4*nsteps for sample 0) receives (iter, dt_global, t, res_total). */
+int tsm_run(tsm_ctx* ctx, long nsteps, long* done, double* log);
+int tsm_time(tsm_ctx* ctx, int sample, double* t, long* iter);
+
+/* FiniteVolume::compute_error_norm (src/fv.cc:1712-1795): exact primitive
+   field in, out[24] = (L1, L2, Linf) for (T, p, u, v, w, rho, Mach, entropy). */
+int tsm_error_norms(tsm_ctx* ctx, int sample, const double* exact_prim, double* out);
+/* FiniteVolume::compute_global (src/fv.cc:2175-2225): kinetic energy and entropy. */
+int tsm_global(tsm_ctx* ctx, int sample, double* ke, double* entropy);
+
+/* Geometry export in reference order, for parity checks. */
+int tsm_geometry_sizes(tsm_ctx* ctx, int* n_face, int* n_bface);
+int tsm_geometry(tsm_ctx* ctx, int* face_v, double* face_n, double* dcarea, double* mcarea);
+
+/* Host-only preprocessing (no device needed): runs the same geometry build as
+   tsm_create and returns it in reference order.  Arrays may be NULL (sizes
+   only): face_v/face_n 2*n_face, bface_v/bface_n 2*n_bface, areas n_vertex. */
+int tsm_host_geometry(const tsm_mesh* mesh, int cell_type, int* face_v, double* face_n,
+                      int* bface_v, double* bface_n, double* dcarea, double* mcarea,
+                      int* n_face, int* n_bface);
+
+/* Times nsteps full steps on the context stream with HIP events and reports
+   the per-kernel average durations in microseconds:
+   us[0] dt pass, us[1] gradient, us[2] edge flux, us[3] gather+update,
+   us[4] whole step.  No host synchronisation inside the timed region. */
+int tsm_bench(tsm_ctx* ctx, int nsteps, double* us);
+
+/* Seeded jittered-triangle lattice on [x0,x1]x[y0,y1] with n x n vertices
+   (the build's own mesh generator; the reference ships none). Sizes first: */
+int tsm_mesh_jittered_sizes(int n, int periodic, int* n_vertex, int* n_bface,
+                            int* n_cell, int* n_periodic, int* n_periodic_v);
+int tsm_mesh_jittered(int n, double x0, double x1, double y0, double y1,
+                      double jitter, unsigned long long seed, int periodic,
+                      double* x, double* y, int* bface_tag, int* bface_v,
+                      int* cell_v, int* periodic_ptr, int* periodic_v);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

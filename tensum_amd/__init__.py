@@ -1,0 +1,316 @@
+"""tensum_amd -- Python host binding of libtsm, the MI355X vertex-centred
+finite-volume residual + Runge-Kutta library.
+
+This module is a thin ctypes layer over the C ABI declared in include/tsm.h.
+It mirrors the reference's configuration vocabulary (flux, reconstruct and
+boundary names of src/parameter.cc:114-143, :354-372 and src/bc.h:14-23) so a
+caller can describe a run the way a TENSUM param.in does.  All numerics run in
+the HIP library; if libtsm.so is missing every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtsm.so")
+
+FLUX = {"kepes_tecno_roe": 0, "kepes_tecno_rusanov": 1, "kepec": 2, "roe_ec": 3,
+        "kep": 4, "simple_avg": 5}
+RECONSTRUCT = {"first": 0, "second": 1, "second_muscl": 2, "van_albada": 3,
+               "minmod": 4, "tvb_minmod": 5, "eno2": 6}
+BC = {"none": 0, "slip": 1, "noslip": 2, "farfield": 3, "inlet": 4, "outlet": 5,
+      "pressure": 6, "periodic": 7}
+
+
+class TsmParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("gas_const", C.c_double), ("prandtl", C.c_double),
+                ("mu_ref", C.c_double), ("T_ref", C.c_double), ("T_0", C.c_double),
+                ("omega", C.c_double),
+                ("model", C.c_int), ("mu_model", C.c_int), ("flux", C.c_int),
+                ("reconstruct", C.c_int), ("tvb_M", C.c_double),
+                ("ev_alpha", C.c_double), ("ev_beta", C.c_double),
+                ("nd_active", C.c_int), ("visc_active", C.c_int), ("heat_active", C.c_int),
+                ("time_scheme", C.c_int), ("unsteady", C.c_int),
+                ("cfl", C.c_double), ("time_step", C.c_double), ("final_time", C.c_double),
+                ("min_residue", C.c_double), ("max_iter", C.c_long),
+                ("n_time_stamps", C.c_int), ("cell_type", C.c_int), ("liou_fix", C.c_int)]
+
+
+class TsmMesh(C.Structure):
+    _fields_ = [("n_vertex", C.c_int), ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)),
+                ("n_bface", C.c_int), ("bface_tag", C.POINTER(C.c_int)),
+                ("bface_v", C.POINTER(C.c_int)),
+                ("n_cell", C.c_int), ("cell_v", C.POINTER(C.c_int)),
+                ("n_periodic", C.c_int), ("periodic_ptr", C.POINTER(C.c_int)),
+                ("periodic_v", C.POINTER(C.c_int)),
+                ("n_bc", C.c_int), ("bc_tag", C.POINTER(C.c_int)),
+                ("bc_type", C.POINTER(C.c_int)), ("bc_state", C.POINTER(C.c_double))]
+
+
+@dataclass
+class Params:
+    """Parsed numeric/material settings (names as in param.in)."""
+    gamma: float = 1.4
+    gas_const: float = 1.0
+    prandtl: float = 0.72
+    mu_ref: float = 0.0
+    T_ref: float = 1.0
+    T_0: float = 0.0
+    omega: float = 1.0
+    model: str = "euler"
+    viscosity: str = "constant"
+    flux: str = "kepes_tecno_rusanov"
+    reconstruct: str = "minmod"
+    tvb_M: float = 0.0
+    ev_alpha: float = 0.0
+    ev_beta: float = 0.0
+    nd_active: bool = True
+    visc_active: bool = True
+    heat_active: bool = True
+    time_scheme: str = "ssprk3"
+    time_mode: str = "unsteady"
+    cfl: float = 0.5
+    time_step: float = 0.0
+    final_time: float = 1.0
+    min_residue: float = 0.0
+    max_iter: int = 1000000
+    time_stamps: int = 1
+    cell: str = "median"
+    liou_fix: bool = False
+
+    def to_c(self):
+        p = TsmParams()
+        p.gamma, p.gas_const, p.prandtl = self.gamma, self.gas_const, self.prandtl
+        p.mu_ref, p.T_ref, p.T_0, p.omega = self.mu_ref, self.T_ref, self.T_0, self.omega
+        p.model = {"euler": 0, "ns": 1}[self.model]
+        p.mu_model = {"constant": 0, "sutherland": 1, "power": 2}[self.viscosity]
+        p.flux = FLUX[self.flux]
+        p.reconstruct = RECONSTRUCT[self.reconstruct]
+        p.tvb_M, p.ev_alpha, p.ev_beta = self.tvb_M, self.ev_alpha, self.ev_beta
+        p.nd_active, p.visc_active, p.heat_active = int(self.nd_active), int(self.visc_active), int(self.heat_active)
+        p.time_scheme = {"ssprk3": 0, "rk4": 1}[self.time_scheme]
+        p.unsteady = 1 if self.time_mode == "unsteady" else 0
+        p.cfl, p.time_step, p.final_time = self.cfl, self.time_step, self.final_time
+        p.min_residue, p.max_iter, p.n_time_stamps = self.min_residue, self.max_iter, self.time_stamps
+        p.cell_type = {"median": 0, "voronoi": 1}[self.cell]
+        p.liou_fix = int(self.liou_fix)
+        return p
+
+
+@dataclass
+class Mesh:
+    """NPART = 1 part-file content (0-based ids) plus the boundary table."""
+    x: np.ndarray
+    y: np.ndarray
+    bface_tag: np.ndarray
+    bface_v: np.ndarray
+    cell_v: np.ndarray
+    periodic_ptr: np.ndarray
+    periodic_v: np.ndarray
+    bc: dict = field(default_factory=dict)   # tag -> (type name, (T,u,v,p))
+
+    @property
+    def n_vertex(self):
+        return int(self.x.size)
+
+    def to_c(self):
+        keep = []
+
+        def ptr(a, t):
+            a = np.ascontiguousarray(a, dtype=np.float64 if t is C.c_double else np.int32)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(t))
+
+        m = TsmMesh()
+        m.n_vertex = self.n_vertex
+        m.x, m.y = ptr(self.x, C.c_double), ptr(self.y, C.c_double)
+        m.n_bface = int(self.bface_tag.size)
+        m.bface_tag, m.bface_v = ptr(self.bface_tag, C.c_int), ptr(self.bface_v, C.c_int)
+        m.n_cell = int(self.cell_v.size // 3)
+        m.cell_v = ptr(self.cell_v, C.c_int)
+        m.n_periodic = int(self.periodic_ptr.size - 1) if self.periodic_ptr.size else 0
+        m.periodic_ptr = ptr(self.periodic_ptr if self.periodic_ptr.size else np.zeros(1), C.c_int)
+        m.periodic_v = ptr(self.periodic_v if self.periodic_v.size else np.zeros(1), C.c_int)
+        tags = sorted(self.bc)
+        m.n_bc = len(tags)
+        m.bc_tag = ptr(np.array(tags, dtype=np.int32), C.c_int)
+        m.bc_type = ptr(np.array([BC[self.bc[t][0]] for t in tags], dtype=np.int32), C.c_int)
+        st = np.array([self.bc[t][1] if len(self.bc[t]) > 1 else (0, 0, 0, 0) for t in tags],
+                      dtype=np.float64).reshape(-1)
+        m.bc_state = ptr(st if st.size else np.zeros(4), C.c_double)
+        return m, keep
+
+
+_lib = None
+
+
+def lib():
+    """Loads libtsm.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libtsm.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.tsm_last_error.restype = C.c_char_p
+        L.tsm_create.argtypes = [C.POINTER(TsmMesh), C.POINTER(TsmParams), C.c_int,
+                                 C.POINTER(C.c_void_p)]
+        L.tsm_destroy.argtypes = [C.c_void_p]
+        L.tsm_destroy.restype = None
+        for name in ("tsm_set_state", "tsm_get_state", "tsm_residual", "tsm_gradients"):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.tsm_set_sample_material.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+        L.tsm_compute_dt.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        L.tsm_run.argtypes = [C.c_void_p, C.c_long, C.c_void_p, C.c_void_p]
+        L.tsm_time.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
+        L.tsm_error_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.tsm_global.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.tsm_geometry_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.tsm_geometry.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+        L.tsm_bench.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.tsm_mesh_jittered_sizes.argtypes = [C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 5
+        L.tsm_mesh_jittered.argtypes = ([C.c_int] + [C.c_double] * 5 + [C.c_ulonglong, C.c_int]
+                                        + [C.c_void_p] * 7)
+        L.tsm_host_geometry.argtypes = [C.POINTER(TsmMesh), C.c_int] + [C.c_void_p] * 6 + [C.POINTER(C.c_int)] * 2
+        _lib = L
+    return _lib
+
+
+class TsmError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise TsmError(f"libtsm error {rc}: {lib().tsm_last_error().decode()}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def jittered_mesh(n, box=(0.0, 1.0, 0.0, 1.0), jitter=0.2, seed=0, periodic=False, bc=None):
+    """Seeded jittered-triangle n x n lattice from libtsm's generator."""
+    L = lib()
+    s = [C.c_int() for _ in range(5)]
+    _check(L.tsm_mesh_jittered_sizes(n, int(periodic), *[C.byref(v) for v in s]))
+    nv, nb, nc, npr, npv = [v.value for v in s]
+    x, y = np.empty(nv), np.empty(nv)
+    bt, bv = np.empty(nb, np.int32), np.empty(2 * nb, np.int32)
+    cv = np.empty(3 * nc, np.int32)
+    pp, pv = np.empty(npr + 1 if periodic else 0, np.int32), np.empty(npv, np.int32)
+    _check(L.tsm_mesh_jittered(n, box[0], box[1], box[2], box[3], jitter, seed, int(periodic),
+                               _dptr(x), _dptr(y), _dptr(bt), _dptr(bv), _dptr(cv),
+                               _dptr(pp) if periodic else None, _dptr(pv) if periodic else None))
+    if bc is None:
+        bc = {t: ("periodic",) if periodic else ("outlet",) for t in (1, 2, 3, 4)}
+    return Mesh(x, y, bt, bv, cv, pp, pv, bc)
+
+
+def host_geometry(mesh: Mesh, cell="median"):
+    """Geometry of libtsm's host preprocessing (no device needed)."""
+    L = lib()
+    m, keep = mesh.to_c()
+    nf, nb = C.c_int(), C.c_int()
+    _check(L.tsm_host_geometry(C.byref(m), {"median": 0, "voronoi": 1}[cell], None, None, None,
+                               None, None, None, C.byref(nf), C.byref(nb)))
+    fv = np.empty(2 * nf.value, np.int32)
+    fn = np.empty(2 * nf.value)
+    bvv = np.empty(2 * nb.value, np.int32)
+    bn = np.empty(2 * nb.value)
+    dc, mc = np.empty(mesh.n_vertex), np.empty(mesh.n_vertex)
+    _check(L.tsm_host_geometry(C.byref(m), {"median": 0, "voronoi": 1}[cell], _dptr(fv), _dptr(fn),
+                               _dptr(bvv), _dptr(bn), _dptr(dc), _dptr(mc), C.byref(nf), C.byref(nb)))
+    return dict(face_v=fv.reshape(-1, 2), face_n=fn.reshape(-1, 2), bface_v=bvv.reshape(-1, 2),
+                bface_n=bn.reshape(-1, 2), dcarea=dc, mcarea=mc)
+
+
+class Solver:
+    """One libtsm context: the whole mesh resident on the GPU, n_samples states."""
+
+    def __init__(self, mesh: Mesh, params: Params, n_samples: int = 1):
+        L = lib()
+        self.mesh, self.params, self.n_samples = mesh, params, n_samples
+        m, self._keep = mesh.to_c()
+        self._p = params.to_c()
+        h = C.c_void_p()
+        _check(L.tsm_create(C.byref(m), C.byref(self._p), n_samples, C.byref(h)))
+        self._h = h
+        self.nv = mesh.n_vertex
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().tsm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_material(self, sample, gamma, gas_const):
+        _check(lib().tsm_set_sample_material(self._h, sample, gamma, gas_const))
+
+    def set_state(self, prim, sample=0):
+        a = np.ascontiguousarray(prim, dtype=np.float64).reshape(self.nv, 4)
+        _check(lib().tsm_set_state(self._h, sample, _dptr(a)))
+
+    def get_state(self, sample=0):
+        a = np.empty((self.nv, 4))
+        _check(lib().tsm_get_state(self._h, sample, _dptr(a)))
+        return a
+
+    def residual(self, sample=0):
+        a = np.empty((self.nv, 4))
+        _check(lib().tsm_residual(self._h, sample, _dptr(a)))
+        return a
+
+    def gradients(self, sample=0):
+        a = np.empty((self.nv, 8))
+        _check(lib().tsm_gradients(self._h, sample, _dptr(a)))
+        return a
+
+    def compute_dt(self, sample=0):
+        d = C.c_double()
+        _check(lib().tsm_compute_dt(self._h, sample, C.byref(d)))
+        return d.value
+
+    def run(self, nsteps, want_log=False):
+        done = np.zeros(self.n_samples, dtype=np.int64)
+        log = np.zeros((nsteps, 4)) if want_log else None
+        _check(lib().tsm_run(self._h, nsteps, _dptr(done), _dptr(log) if want_log else None))
+        return (done, log) if want_log else done
+
+    def time(self, sample=0):
+        t, it = C.c_double(), C.c_long()
+        _check(lib().tsm_time(self._h, sample, C.byref(t), C.byref(it)))
+        return t.value, it.value
+
+    def error_norms(self, exact_prim, sample=0):
+        a = np.ascontiguousarray(exact_prim, dtype=np.float64).reshape(self.nv, 4)
+        out = np.empty(24)
+        _check(lib().tsm_error_norms(self._h, sample, _dptr(a), _dptr(out)))
+        return out.reshape(8, 3)
+
+    def global_integrals(self, sample=0):
+        k, e = C.c_double(), C.c_double()
+        _check(lib().tsm_global(self._h, sample, C.byref(k), C.byref(e)))
+        return k.value, e.value
+
+    def geometry(self):
+        nf, nb = C.c_int(), C.c_int()
+        _check(lib().tsm_geometry_sizes(self._h, C.byref(nf), C.byref(nb)))
+        fv = np.empty(2 * nf.value, np.int32)
+        fn = np.empty(2 * nf.value)
+        dc, mc = np.empty(self.nv), np.empty(self.nv)
+        _check(lib().tsm_geometry(self._h, _dptr(fv), _dptr(fn), _dptr(dc), _dptr(mc)))
+        return dict(face_v=fv.reshape(-1, 2), face_n=fn.reshape(-1, 2), dcarea=dc, mcarea=mc)
+
+    def bench(self, nsteps):
+        us = np.zeros(5)
+        _check(lib().tsm_bench(self._h, nsteps, _dptr(us)))
+        return dict(dt_us=us[0], gradient_us=us[1], flux_us=us[2], update_us=us[3], step_us=us[4])

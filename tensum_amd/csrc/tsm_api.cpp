@@ -1,0 +1,534 @@
+// tsm_api.cpp -- host context and C ABI of libtsm (include/tsm.h).
+//
+// Owns the device copies of the geometry and of the per-sample state, one HIP
+// stream, and the stage loop that replaces the body of FiniteVolume::solve
+// (src/fv.cc:2273-2368).  No PyTorch, no MPI: the whole mesh is resident on
+// one GPU and every reduction of the reference (dt, residual norm, periodic
+// sums) is a device kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tsm_device.h"
+#include "tsm_host.h"
+
+namespace {
+thread_local std::string g_last_error;
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return fail(TSM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));  \
+  } while (0)
+}  // namespace
+
+struct tsm_ctx {
+  tsm_params P{};
+  int S = 1;
+  tsm::Geometry geo;
+  hipStream_t st = nullptr;
+  tsm::DevGeo G{};
+  tsm::StageArgs A{};
+  tsm::StepParams SP{};
+  std::vector<void*> allocs;
+  double* W[2] = {nullptr, nullptr};
+  double* E[2] = {nullptr, nullptr};
+  int cur = 0;
+  double* red = nullptr;   // reduction partials for global / error norms
+  size_t red_cap = 0;
+  std::vector<double> h_gamma, h_R;
+  int nblk_v = 0;
+
+  ~tsm_ctx() {
+    for (void* p : allocs) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  template <class T>
+  int alloc(T** p, size_t n) {
+    void* q = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess) return fail(TSM_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = (T*)q;
+    return TSM_OK;
+  }
+  template <class T>
+  int upload(const T** dst, const std::vector<T>& src) {
+    T* d = nullptr;
+    int rc = alloc(&d, src.size());
+    if (rc) return rc;
+    if (!src.empty()) HIP_TRY(hipMemcpy(d, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    *dst = d;
+    return TSM_OK;
+  }
+  void bind_state() {
+    A.W_in = W[cur]; A.W_out = W[cur ^ 1];
+    A.E_in = E[cur]; A.E_out = E[cur ^ 1];
+  }
+};
+
+extern "C" const char* tsm_last_error(void) { return g_last_error.c_str(); }
+
+namespace {
+
+int validate_params(const tsm_params& p) {
+  if (!(p.gamma > 1.0)) return fail(TSM_E_ARG, "gamma must be > 1");
+  if (!(p.gas_const > 0.0)) return fail(TSM_E_ARG, "gas_const must be > 0");
+  if (p.flux < 0 || p.flux > 5) return fail(TSM_E_ARG, "unknown flux scheme");
+  if (p.reconstruct < 0 || p.reconstruct > 6) return fail(TSM_E_ARG, "unknown reconstruction");
+  if (p.time_scheme != 0 && p.time_scheme != 1)
+    return fail(TSM_E_ARG, "time_scheme must be ssprk3 (0) or rk4 (1)");
+  if (!p.unsteady) return fail(TSM_E_ARG, "steady time_mode (local time stepping) is not supported");
+  if (p.time_step == 0.0 && p.cfl == 0.0) return fail(TSM_E_ARG, "Either time_step or cfl must be non-zero");
+  if (p.time_step > 0.0 && p.cfl > 0.0) return fail(TSM_E_ARG, "both time_step and cfl specified");
+  if (!(p.final_time > 0.0)) return fail(TSM_E_ARG, "final_time must be positive for unsteady flow");
+  if (p.n_time_stamps <= 0) return fail(TSM_E_ARG, "time_stamps must be > 0");
+  if (p.model == 0 && p.mu_ref > 0.0) return fail(TSM_E_ARG, "mu_ref must be 0 for the model=euler");
+  return TSM_OK;
+}
+
+}  // namespace
+
+extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_samples,
+                          tsm_ctx** out) {
+  if (!mesh || !params || !out || n_samples < 1) return fail(TSM_E_ARG, "null argument");
+  *out = nullptr;
+  int rc = validate_params(*params);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(TSM_E_HIP, "no HIP device visible");
+  tsm_ctx* c = new tsm_ctx();
+  c->P = *params;
+  c->S = n_samples;
+  std::string msg;
+  if (!tsm::build_geometry(*mesh, params->cell_type, c->geo, msg)) {
+    delete c;
+    return fail(TSM_E_MESH, msg);
+  }
+  const tsm::Geometry& g = c->geo;
+  const int nv = g.nv, nc = g.nc, nf = (int)g.face.size(), nb = (int)g.bface.size();
+  if (params->model == 1) {
+    for (int k = 0; k < nc; ++k)
+      if (g.cell_has_nonper_bface[k]) {
+        delete c;
+        return fail(TSM_E_ARG, "Navier-Stokes with non-periodic boundary cells is not supported");
+      }
+  }
+  if ((size_t)nf >= (1u << 30) || (size_t)nb >= (1u << 30)) {
+    delete c;
+    return fail(TSM_E_MESH, "mesh too large for 30-bit face ids");
+  }
+  // ---- host layout --------------------------------------------------------
+  std::vector<int> fv(2 * (size_t)nf);
+  std::vector<double> fn(2 * (size_t)nf);
+  for (int f = 0; f < nf; ++f) {
+    fv[2 * f] = g.face[f].v[0]; fv[2 * f + 1] = g.face[f].v[1];
+    fn[2 * f] = g.face[f].nx; fn[2 * f + 1] = g.face[f].ny;
+  }
+  // vertex -> face/bface CSR in reference order
+  std::vector<int> vfe_ptr(nv + 1, 0), vfe;
+  for (int f = 0; f < nf; ++f) { vfe_ptr[g.face[f].v[0] + 1]++; vfe_ptr[g.face[f].v[1] + 1]++; }
+  for (int b = 0; b < nb; ++b)
+    if (!g.bface[b].periodic) { vfe_ptr[g.bface[b].v[0] + 1]++; vfe_ptr[g.bface[b].v[1] + 1]++; }
+  for (int i = 0; i < nv; ++i) vfe_ptr[i + 1] += vfe_ptr[i];
+  vfe.resize(vfe_ptr[nv]);
+  {
+    std::vector<int> pos(vfe_ptr.begin(), vfe_ptr.end() - 1);
+    for (int f = 0; f < nf; ++f) {
+      vfe[pos[g.face[f].v[0]]++] = (int)((0u << 30) | (unsigned)f);
+      vfe[pos[g.face[f].v[1]]++] = (int)((1u << 30) | (unsigned)f);
+    }
+    for (int b = 0; b < nb; ++b) {
+      if (g.bface[b].periodic) continue;
+      vfe[pos[g.bface[b].v[0]]++] = (int)((2u << 30) | (unsigned)b);
+      vfe[pos[g.bface[b].v[1]]++] = (int)((2u << 30) | (unsigned)b);
+    }
+  }
+  // cell -> non-periodic bfaces (bface order)
+  std::vector<int> cbf(2 * (size_t)nc, -1);
+  for (int b = 0; b < nb; ++b) {
+    if (g.bface[b].periodic) continue;
+    int cl = g.bface[b].lcell;
+    if (cbf[2 * cl] < 0) cbf[2 * cl] = b;
+    else if (cbf[2 * cl + 1] < 0) cbf[2 * cl + 1] = b;
+    else { delete c; return fail(TSM_E_MESH, "cell with three boundary faces"); }
+  }
+  std::vector<int> vc_ptr(nv + 1, 0), vc;
+  for (int k = 0; k < nc; ++k)
+    for (int j = 0; j < 3; ++j) vc_ptr[g.cell[3 * k + j] + 1]++;
+  for (int i = 0; i < nv; ++i) vc_ptr[i + 1] += vc_ptr[i];
+  vc.resize(vc_ptr[nv]);
+  {
+    std::vector<int> pos(vc_ptr.begin(), vc_ptr.end() - 1);
+    for (int k = 0; k < nc; ++k)
+      for (int j = 0; j < 3; ++j)
+        vc[pos[g.cell[3 * k + j]]++] = (cbf[2 * k] >= 0) ? (int)(0x80000000u | (unsigned)k) : k;
+  }
+  std::vector<int> cnb(nc);
+  for (int k = 0; k < nc; ++k) cnb[k] = (cbf[2 * k] >= 0) ? 1 : 0;
+  std::vector<int> bvv(2 * (size_t)nb), btype(nb);
+  std::vector<double> bn(2 * (size_t)nb), bstate(4 * (size_t)nb, 0.0);
+  for (int b = 0; b < nb; ++b) {
+    bvv[2 * b] = g.bface[b].v[0]; bvv[2 * b + 1] = g.bface[b].v[1];
+    bn[2 * b] = g.bface[b].nx; bn[2 * b + 1] = g.bface[b].ny;
+    btype[b] = g.bface[b].bc;
+    for (int i = 0; i < mesh->n_bc; ++i)
+      if (mesh->bc_tag[i] == g.bface[b].tag)
+        for (int k = 0; k < 4; ++k) bstate[4 * b + k] = mesh->bc_state ? mesh->bc_state[4 * i + k] : 0.0;
+  }
+  std::vector<double> gfact(nv);
+  for (int i = 0; i < nv; ++i) gfact[i] = 1.0 / (6.0 * g.mcarea[i]);
+  std::vector<int> pset(g.periodic_of), pptr(1, 0), pv;
+  for (auto& P : g.periodic) { for (int v : P) pv.push_back(v); pptr.push_back((int)pv.size()); }
+
+  // ---- device ------------------------------------------------------------
+  hipError_t he = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (he != hipSuccess) { delete c; return fail(TSM_E_HIP, "hipStreamCreate failed"); }
+  tsm::DevGeo& G = c->G;
+  G.nv = nv; G.nf = nf; G.nb = nb; G.nc = nc;
+#define UP(field, vec) do { if ((rc = c->upload(&G.field, vec))) { delete c; return rc; } } while (0)
+  UP(x, g.x); UP(y, g.y); UP(dcarea, g.dcarea); UP(gfact, gfact); UP(weight, g.weight);
+  UP(carea, g.cell_area);
+  UP(fv, fv); UP(fn, fn); UP(vfe_ptr, vfe_ptr); UP(vfe, vfe); UP(vc_ptr, vc_ptr); UP(vc, vc);
+  UP(cv, g.cell); UP(cbf, cbf); UP(cnb, cnb); UP(bv, bvv); UP(bn, bn); UP(btype, btype);
+  UP(bstate, bstate); UP(pset, pset); UP(pptr, pptr); UP(pv, pv);
+#undef UP
+  tsm::StageArgs& A = c->A;
+  const int S = n_samples;
+  A.cs = (size_t)S * nv;
+  A.fs = (size_t)S * nf;
+  A.off = 0;
+  c->nblk_v = (nv + 255) / 256;
+  const size_t cs = A.cs;
+#define AL(ptr, n) do { if ((rc = c->alloc(&(ptr), (n)))) { delete c; return rc; } } while (0)
+  AL(c->W[0], 4 * cs); AL(c->W[1], 4 * cs); AL(c->E[0], 4 * cs); AL(c->E[1], 4 * cs);
+  AL(A.Uold, 4 * cs); AL(A.dE, 8 * cs); AL(A.F, 4 * A.fs); AL(A.R, 4 * cs);
+  if (params->time_scheme == 1) AL(A.R2, 4 * cs);
+  if (params->liou_fix) AL(A.ssw, cs);
+  AL(A.partial, (size_t)S * c->nblk_v * 4);
+  AL(A.ctl, (size_t)S);
+  double* dg; double* dr; double* ts;
+  AL(dg, (size_t)S); AL(dr, (size_t)S);
+  std::vector<double> tst(params->n_time_stamps + 1);
+  double tdt = params->final_time / params->n_time_stamps;
+  for (size_t i = 0; i < tst.size(); ++i) tst[i] = i * tdt;
+  AL(ts, tst.size());
+#undef AL
+  HIP_TRY(hipMemcpy(ts, tst.data(), tst.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->h_gamma.assign(S, params->gamma);
+  c->h_R.assign(S, params->gas_const);
+  HIP_TRY(hipMemcpy(dg, c->h_gamma.data(), S * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dr, c->h_R.data(), S * sizeof(double), hipMemcpyHostToDevice));
+  A.gamma = dg; A.gasR = dr; A.tstamps = ts;
+  A.log = nullptr; A.log_cap = 0;
+  A.fo = tsm::FluxOpts{params->ev_alpha, params->ev_beta, params->tvb_M, params->nd_active};
+  A.vp = tsm::ViscParams{params->mu_ref, params->T_ref, params->T_0, params->omega,
+                         params->prandtl, params->mu_model, params->visc_active,
+                         params->heat_active};
+  A.ns = params->model == 1;
+  A.ns_samples = S;
+  A.ignore_done = 0;
+  c->SP = tsm::StepParams{params->min_residue, params->final_time, params->time_step,
+                          params->cfl, (long long)params->max_iter, params->unsteady,
+                          params->n_time_stamps};
+  std::vector<tsm::Ctl> ctl(S);
+  for (auto& q : ctl) { q = tsm::Ctl{}; q.res_total = 1e20; q.counter = 1; q.done = 1; }
+  HIP_TRY(hipMemcpy(A.ctl, ctl.data(), S * sizeof(tsm::Ctl), hipMemcpyHostToDevice));
+  c->bind_state();
+  *out = c;
+  return TSM_OK;
+}
+
+extern "C" void tsm_destroy(tsm_ctx* c) { delete c; }
+
+extern "C" int tsm_set_sample_material(tsm_ctx* c, int s, double gamma, double R) {
+  if (!c || s < 0 || s >= c->S || !(gamma > 1.0) || !(R > 0.0)) return fail(TSM_E_ARG, "bad argument");
+  c->h_gamma[s] = gamma; c->h_R[s] = R;
+  HIP_TRY(hipMemcpy((void*)(c->A.gamma + s), &gamma, sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy((void*)(c->A.gasR + s), &R, sizeof(double), hipMemcpyHostToDevice));
+  return TSM_OK;
+}
+
+extern "C" int tsm_set_state(tsm_ctx* c, int s, const double* prim) {
+  if (!c || !prim || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  const int nv = c->G.nv;
+  for (int i = 0; i < nv; ++i)
+    if (!(prim[4 * i] > 0.0) || !(prim[4 * i + 3] > 0.0))
+      return fail(TSM_E_STATE, "initial temperature and pressure must be positive (vertex " +
+                                   std::to_string(i) + ")");
+  std::vector<double> soa(4 * (size_t)nv);
+  for (int i = 0; i < nv; ++i)
+    for (int k = 0; k < 4; ++k) soa[(size_t)k * nv + i] = prim[4 * i + k];
+  const size_t cs = c->A.cs, off = (size_t)s * nv;
+  HIP_TRY(hipStreamSynchronize(c->st));
+  for (int k = 0; k < 4; ++k)
+    HIP_TRY(hipMemcpy(c->W[c->cur] + k * cs + off, soa.data() + (size_t)k * nv,
+                      nv * sizeof(double), hipMemcpyHostToDevice));
+  c->bind_state();
+  tsm::launch_prim2ent(c->G, c->A, s, c->st);
+  HIP_TRY(hipGetLastError());
+  tsm::Ctl q{};
+  q.res_total = 1e20; q.counter = 1; q.done = 0;
+  HIP_TRY(hipMemcpyAsync(c->A.ctl + s, &q, sizeof(q), hipMemcpyHostToDevice, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  return TSM_OK;
+}
+
+namespace {
+int download_vec(tsm_ctx* c, const double* base, int ncomp, int s, double* out) {
+  const int nv = c->G.nv;
+  const size_t cs = c->A.cs, off = (size_t)s * nv;
+  std::vector<double> soa((size_t)ncomp * nv);
+  HIP_TRY(hipStreamSynchronize(c->st));
+  for (int k = 0; k < ncomp; ++k)
+    HIP_TRY(hipMemcpy(soa.data() + (size_t)k * nv, base + k * cs + off, nv * sizeof(double),
+                      hipMemcpyDeviceToHost));
+  for (int i = 0; i < nv; ++i)
+    for (int k = 0; k < ncomp; ++k) out[(size_t)ncomp * i + k] = soa[(size_t)k * nv + i];
+  return TSM_OK;
+}
+}  // namespace
+
+extern "C" int tsm_get_state(tsm_ctx* c, int s, double* prim) {
+  if (!c || !prim || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  return download_vec(c, c->W[c->cur], 4, s, prim);
+}
+
+extern "C" int tsm_gradients(tsm_ctx* c, int s, double* dE) {
+  if (!c || !dE || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  c->bind_state();
+  c->A.ignore_done = 1;
+  tsm::launch_gradient(c->G, c->A, c->S, c->st);
+  c->A.ignore_done = 0;
+  HIP_TRY(hipGetLastError());
+  return download_vec(c, c->A.dE, 8, s, dE);
+}
+
+extern "C" int tsm_residual(tsm_ctx* c, int s, double* R) {
+  if (!c || !R || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  c->bind_state();
+  c->A.ignore_done = 1;
+  if (c->P.liou_fix) tsm::launch_ssw(c->G, c->A, c->S, c->st);
+  tsm::launch_gradient(c->G, c->A, c->S, c->st);
+  tsm::launch_flux_update(c->P.flux, c->P.reconstruct, c->G, c->A, c->S, 0, 0, 0, c->st, true, true);
+  c->A.ignore_done = 0;
+  HIP_TRY(hipGetLastError());
+  return download_vec(c, c->A.R, 4, s, R);
+}
+
+extern "C" int tsm_compute_dt(tsm_ctx* c, int s, double* dt) {
+  if (!c || !dt || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  c->bind_state();
+  c->A.ignore_done = 1;
+  tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
+  c->A.ignore_done = 0;
+  HIP_TRY(hipGetLastError());
+  tsm::Ctl q;
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(&q, c->A.ctl + s, sizeof(q), hipMemcpyDeviceToHost));
+  *dt = q.dt;
+  return TSM_OK;
+}
+
+namespace {
+// One full time step for every live sample (src/fv.cc:2277-2316).
+void enqueue_step(tsm_ctx* c, int log_slot) {
+  const int nrk = c->P.time_scheme == 0 ? 3 : 4;
+  const int mode = c->P.time_scheme == 0 ? 1 : 2;
+  c->bind_state();
+  tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
+  if (c->P.liou_fix) tsm::launch_ssw(c->G, c->A, c->S, c->st);
+  for (int r = 0; r < nrk; ++r) {
+    c->bind_state();
+    tsm::launch_gradient(c->G, c->A, c->S, c->st);
+    tsm::launch_flux_update(c->P.flux, c->P.reconstruct, c->G, c->A, c->S, r, r == nrk - 1, mode,
+                            c->st, true, true);
+    c->cur ^= 1;
+  }
+  c->bind_state();
+  tsm::launch_step_final(c->G, c->A, c->SP, c->S, log_slot, c->st);
+}
+}  // namespace
+
+extern "C" int tsm_run(tsm_ctx* c, long nsteps, long* done, double* log) {
+  if (!c || nsteps < 0) return fail(TSM_E_ARG, "bad argument");
+  const int S = c->S;
+  std::vector<tsm::Ctl> before(S), after(S);
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(before.data(), c->A.ctl, S * sizeof(tsm::Ctl), hipMemcpyDeviceToHost));
+  if (log && nsteps > 0) {
+    if (c->A.log_cap < nsteps) {
+      double* p = nullptr;
+      int rc = c->alloc(&p, (size_t)S * nsteps * 4);
+      if (rc) return rc;
+      c->A.log = p;
+      c->A.log_cap = (int)nsteps;
+    }
+  }
+  for (long k = 0; k < nsteps; ++k) enqueue_step(c, log ? (int)k : -1);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(after.data(), c->A.ctl, S * sizeof(tsm::Ctl), hipMemcpyDeviceToHost));
+  if (done)
+    for (int s = 0; s < S; ++s) done[s] = (long)(after[s].iter - before[s].iter);
+  if (log && nsteps > 0) {
+    long n0 = (long)(after[0].iter - before[0].iter);
+    std::vector<double> buf((size_t)nsteps * 4);
+    HIP_TRY(hipMemcpy(buf.data(), c->A.log, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (long k = 0; k < nsteps * 4; ++k) log[k] = (k < n0 * 4) ? buf[k] : 0.0;
+  }
+  c->A.log = (log ? c->A.log : c->A.log);
+  return TSM_OK;
+}
+
+extern "C" int tsm_time(tsm_ctx* c, int s, double* t, long* iter) {
+  if (!c || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  tsm::Ctl q;
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(&q, c->A.ctl + s, sizeof(q), hipMemcpyDeviceToHost));
+  if (t) *t = q.t;
+  if (iter) *iter = (long)q.iter;
+  return TSM_OK;
+}
+
+namespace {
+int ensure_red(tsm_ctx* c, size_t n) {
+  if (c->red_cap >= n) return TSM_OK;
+  int rc = c->alloc(&c->red, n);
+  if (rc) return rc;
+  c->red_cap = n;
+  return TSM_OK;
+}
+}  // namespace
+
+extern "C" int tsm_global(tsm_ctx* c, int s, double* ke, double* ent) {
+  if (!c || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  int nb = c->nblk_v, rc = ensure_red(c, (size_t)nb * 24);
+  if (rc) return rc;
+  c->bind_state();
+  tsm::launch_global(c->G, c->A, s, c->red, c->st);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> h((size_t)nb * 2);
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(h.data(), c->red, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double K = 0, E = 0;
+  for (int b = 0; b < nb; ++b) { K += h[2 * b]; E += h[2 * b + 1]; }
+  if (ke) *ke = K;
+  if (ent) *ent = E;
+  return TSM_OK;
+}
+
+extern "C" int tsm_error_norms(tsm_ctx* c, int s, const double* exact, double* out) {
+  if (!c || !exact || !out || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  const int nv = c->G.nv, nb = c->nblk_v;
+  int rc = ensure_red(c, (size_t)nb * 24);
+  if (rc) return rc;
+  double* dex = nullptr;
+  HIP_TRY(hipMalloc(&dex, 4 * (size_t)nv * sizeof(double)));
+  hipError_t e = hipMemcpy(dex, exact, 4 * (size_t)nv * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { (void)hipFree(dex); return fail(TSM_E_HIP, "upload exact"); }
+  c->bind_state();
+  tsm::launch_errnorm(c->G, c->A, s, dex, c->red, c->st);
+  e = hipGetLastError();
+  std::vector<double> h((size_t)nb * 24);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), c->red, h.size() * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(dex);
+  if (e != hipSuccess) return fail(TSM_E_HIP, hipGetErrorString(e));
+  double L1[8] = {0}, L2[8] = {0}, Li[8] = {0};
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 8; ++k) {
+      L1[k] += h[24 * b + k];
+      L2[k] += h[24 * b + 8 + k];
+      Li[k] = std::fmax(Li[k], h[24 * b + 16 + k]);
+    }
+  for (int k = 0; k < 8; ++k) {
+    out[3 * k] = L1[k]; out[3 * k + 1] = std::sqrt(L2[k]); out[3 * k + 2] = Li[k];
+  }
+  return TSM_OK;
+}
+
+extern "C" int tsm_geometry_sizes(tsm_ctx* c, int* nf, int* nb) {
+  if (!c) return fail(TSM_E_ARG, "bad argument");
+  if (nf) *nf = (int)c->geo.face.size();
+  if (nb) *nb = (int)c->geo.bface.size();
+  return TSM_OK;
+}
+
+extern "C" int tsm_geometry(tsm_ctx* c, int* fv, double* fn, double* dc, double* mc) {
+  if (!c) return fail(TSM_E_ARG, "bad argument");
+  const auto& g = c->geo;
+  for (size_t f = 0; f < g.face.size(); ++f) {
+    if (fv) { fv[2 * f] = g.face[f].v[0]; fv[2 * f + 1] = g.face[f].v[1]; }
+    if (fn) { fn[2 * f] = g.face[f].nx; fn[2 * f + 1] = g.face[f].ny; }
+  }
+  for (int i = 0; i < g.nv; ++i) {
+    if (dc) dc[i] = g.dcarea[i];
+    if (mc) mc[i] = g.mcarea[i];
+  }
+  return TSM_OK;
+}
+
+extern "C" int tsm_bench(tsm_ctx* c, int nsteps, double* us) {
+  if (!c || nsteps < 1 || !us) return fail(TSM_E_ARG, "bad argument");
+  const int nrk = c->P.time_scheme == 0 ? 3 : 4;
+  const int mode = c->P.time_scheme == 0 ? 1 : 2;
+  hipEvent_t ev[16];
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  double acc[4] = {0, 0, 0, 0};
+  float ms = 0;
+  HIP_TRY(hipStreamSynchronize(c->st));
+  for (int k = 0; k < nsteps; ++k) {
+    c->bind_state();
+    HIP_TRY(hipEventRecord(ev[0], c->st));
+    tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
+    HIP_TRY(hipEventRecord(ev[1], c->st));
+    if (c->P.liou_fix) tsm::launch_ssw(c->G, c->A, c->S, c->st);
+    int e = 2;
+    for (int r = 0; r < nrk; ++r) {
+      c->bind_state();
+      HIP_TRY(hipEventRecord(ev[e++], c->st));
+      tsm::launch_gradient(c->G, c->A, c->S, c->st);
+      HIP_TRY(hipEventRecord(ev[e++], c->st));
+      tsm::launch_flux_update(c->P.flux, c->P.reconstruct, c->G, c->A, c->S, r, r == nrk - 1,
+                              mode, c->st, true, false);
+      HIP_TRY(hipEventRecord(ev[e++], c->st));
+      tsm::launch_flux_update(c->P.flux, c->P.reconstruct, c->G, c->A, c->S, r, r == nrk - 1,
+                              mode, c->st, false, true);
+      c->cur ^= 1;
+    }
+    c->bind_state();
+    tsm::launch_step_final(c->G, c->A, c->SP, c->S, -1, c->st);
+    HIP_TRY(hipEventRecord(ev[e++], c->st));
+    HIP_TRY(hipEventSynchronize(ev[e - 1]));
+    HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); acc[0] += ms;
+    for (int r = 0; r < nrk; ++r) {
+      HIP_TRY(hipEventElapsedTime(&ms, ev[2 + 3 * r], ev[3 + 3 * r])); acc[1] += ms;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[3 + 3 * r], ev[4 + 3 * r])); acc[2] += ms;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[4 + 3 * r], ev[(r == nrk - 1) ? e - 1 : 5 + 3 * r])); acc[3] += ms;
+    }
+  }
+  // whole steps without interleaved events
+  HIP_TRY(hipEventRecord(ev[0], c->st));
+  for (int k = 0; k < nsteps; ++k) enqueue_step(c, -1);
+  HIP_TRY(hipEventRecord(ev[1], c->st));
+  HIP_TRY(hipEventSynchronize(ev[1]));
+  HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  us[0] = 1e3 * acc[0] / nsteps;
+  us[1] = 1e3 * acc[1] / (nsteps * (double)nrk);
+  us[2] = 1e3 * acc[2] / (nsteps * (double)nrk);
+  us[3] = 1e3 * acc[3] / (nsteps * (double)nrk);
+  us[4] = 1e3 * ms / nsteps;
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return TSM_OK;
+}

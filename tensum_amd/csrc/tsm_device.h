@@ -1,0 +1,91 @@
+// tsm_device.h -- device-side structures shared by the kernels and the host
+// context of libtsm.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "../../include/tsm.h"
+#include "tsm_physics.h"
+
+namespace tsm {
+
+// Per-sample loop control, resident in device memory so that whole steps can
+// be queued without a host round trip (src/fv.cc:2230-2395 loop variables).
+struct Ctl {
+  double t, dt, res0, res_total;
+  long long iter;
+  int counter, reached, done, pad;
+};
+
+// Read-only mesh arrays (device pointers).
+struct DevGeo {
+  int nv, nf, nb, nc;
+  const double *x, *y, *dcarea, *gfact, *weight, *carea;
+  const int* fv;        // 2*nf: left, right vertex of each face (reference order)
+  const double* fn;     // 2*nf: dual-face normal
+  const int* vfe_ptr;   // nv+1
+  const int* vfe;       // per vertex: faces (kind 0 left / 1 right) then bfaces (kind 2), bits 31..30
+  const int* vc_ptr;    // nv+1
+  const int* vc;        // per vertex: cells ascending; sign bit = cell has boundary corrections
+  const int* cv;        // 3*nc
+  const int* cbf;       // 2*nc: non-periodic bfaces of the cell (bface order), -1 padded
+  const int* cnb;       // nc: cell has a non-periodic bface
+  const int* bv;        // 2*nb (oriented)
+  const double* bn;     // 2*nb outward edge normal
+  const int* btype;     // nb
+  const double* bstate; // 4*nb (T,u,v,p) constant boundary state
+  const int* pset;      // nv: periodic set id or -1
+  const int* pptr;      // nsets+1
+  const int* pv;        // members
+};
+
+struct ViscParams {
+  double mu_ref, T_ref, T_0, omega, prandtl;
+  int mu_model, visc_active, heat_active;
+};
+
+struct StepParams {
+  double min_residue, final_time, time_step, cfl;
+  long long max_iter;
+  int unsteady;
+  int n_stamps;
+};
+
+// Mutable state and scratch (device pointers).  Components are strided by
+// cs = n_samples*nv (vertex arrays) and fs = n_samples*nf (face arrays).
+struct StageArgs {
+  size_t cs, fs, off;
+  const double* W_in; double* W_out;
+  double* E_in; double* E_out;
+  double* Uold;
+  double* dE;      // 8 components
+  double* F;       // 4 components per face
+  double* R;       // 4 components (residual export)
+  double* R2;      // 4 components (rk4)
+  double* ssw;
+  double* partial; // per-block reduction scratch
+  Ctl* ctl;
+  const double* gamma; const double* gasR;
+  const double* tstamps;
+  double* log; int log_cap;
+  FluxOpts fo;
+  ViscParams vp;
+  int ns;          // Navier-Stokes model active
+  int ns_samples;  // number of samples
+  int ignore_done; // single-call API entry points compute regardless of the loop state
+};
+
+void launch_flux_update(int flux, int rec, const DevGeo& G, const StageArgs& A, int S, int r,
+                        int last, int mode, hipStream_t st, bool do_flux, bool do_update);
+void launch_gradient(const DevGeo& G, const StageArgs& A, int S, hipStream_t st);
+void launch_dt(const DevGeo& G, const StageArgs& A, const StepParams& P, int S, hipStream_t st);
+void launch_step_final(const DevGeo& G, const StageArgs& A, const StepParams& P, int S,
+                       int log_slot, hipStream_t st);
+void launch_ssw(const DevGeo& G, const StageArgs& A, int S, hipStream_t st);
+void launch_prim2ent(const DevGeo& G, const StageArgs& A, int s, hipStream_t st);
+void launch_global(const DevGeo& G, const StageArgs& A, int s, double* out, hipStream_t st);
+void launch_errnorm(const DevGeo& G, const StageArgs& A, int s, const double* ex, double* out,
+                    hipStream_t st);
+
+}  // namespace tsm

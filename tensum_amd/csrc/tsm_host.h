@@ -1,0 +1,52 @@
+// tsm_host.h -- host-side data structures of libtsm (geometry, layout).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tsm.h"
+
+namespace tsm {
+
+struct Face {
+  int v[2];
+  int type;        // boundary tag (>= 0) or -1 for an interior face
+  bool periodic;
+  int lcell, rcell;
+  double cx, cy;   // edge midpoint
+  double nx, ny;   // dual-face normal (boundary faces: half dual face)
+  double measure;
+};
+
+struct BFace {
+  int v[2];
+  int tag;
+  int bc;          // TSM_BC_*
+  bool periodic;
+  int lcell;
+  double nx, ny, measure;
+};
+
+// Dual-mesh geometry of one NPART = 1 partition, in reference order.
+struct Geometry {
+  int nv = 0, nc = 0;
+  std::vector<double> x, y;
+  std::vector<int> cell;            // 3 per cell
+  std::vector<double> cell_area;
+  std::vector<double> ccx, ccy;     // cell centroids
+  std::vector<int> cell_slot_face;  // 3 per cell: face ids (reference slot order)
+  std::vector<uint8_t> cell_has_nonper_bface;
+  std::vector<Face> face;
+  std::vector<BFace> bface;
+  std::vector<double> mcarea, dcarea;
+  std::vector<std::vector<int>> periodic;  // P_ij sets, member order of the file
+  std::vector<int> periodic_of;            // set id per vertex or -1
+  std::vector<double> weight;              // 1/|P| for periodic vertices else 1
+};
+
+// Builds faces, normals and areas exactly as Grid::preproc does for one
+// partition (src/grid_preproc.cc:1147-1194).  Returns false with msg set.
+bool build_geometry(const tsm_mesh& m, int cell_type, Geometry& g, std::string& msg);
+
+}  // namespace tsm

@@ -1,0 +1,791 @@
+// tsm_kernels.hip -- gfx950 kernels of the FV residual + RK stage loop.
+//
+// Layout: every per-vertex quantity is structure-of-arrays FP64, one array of
+// n_samples*n_vertex values per component (sample-major), so a wavefront of
+// 64 consecutive vertices reads 512 contiguous bytes per component.  The
+// sample index is the grid's y dimension.  Residuals are summed by gathering
+// along a vertex CSR whose entries are in the reference's face order, so each
+// vertex adds its face fluxes in exactly the order the reference's face-loop
+// scatter does (src/fv.cc:1071-1097); no float atomics are used anywhere and
+// two runs give bit-identical results.
+#include <hip/hip_runtime.h>
+
+#include "tsm_device.h"
+#include "tsm_physics.h"
+
+namespace tsm {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Prim load_prim(const double* W, size_t cs, size_t i) {
+  return Prim{W[i], W[cs + i], W[2 * cs + i], W[3 * cs + i]};
+}
+__device__ __forceinline__ void store_prim(double* W, size_t cs, size_t i, const Prim& w) {
+  W[i] = w.T; W[cs + i] = w.u; W[2 * cs + i] = w.v; W[3 * cs + i] = w.p;
+}
+__device__ __forceinline__ Ent load_ent(const double* E, size_t cs, size_t i) {
+  return Ent{E[i], E[cs + i], E[2 * cs + i], E[3 * cs + i]};
+}
+__device__ __forceinline__ void store_ent(double* E, size_t cs, size_t i, const Ent& e) {
+  E[i] = e.f; E[cs + i] = e.vx; E[2 * cs + i] = e.vy; E[3 * cs + i] = e.l;
+}
+
+// Block-wide deterministic sum / min of NV values per thread (blockDim 256).
+template <int NV>
+__device__ __forceinline__ void block_sum(double* v, double* lds) {
+  // wave-level tree (64 lanes), fixed pattern
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    v[k] = x;
+  }
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[wid * NV + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = lds[k];
+      for (int w = 1; w < nw; ++w) s += lds[w * NV + k];
+      v[k] = s;
+    }
+  }
+}
+__device__ __forceinline__ double block_min(double x, double* lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmin(x, __shfl_down(x, off, 64));
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = blockDim.x >> 6;
+    for (int w = 1; w < nw; ++w) x = fmin(x, lds[w]);
+  }
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// One-state boundary condition, src/bc.h:615-654
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bc_apply1(int type, const double* bs, double nx, double ny,
+                                          Prim& st) {
+  switch (type) {
+    case TSM_BC_SLIP: {
+      double im = 1.0 / sqrt(nx * nx + ny * ny);
+      double ux = nx * im, uy = ny * im;
+      double fact = st.u * ux + st.v * uy;
+      st.u += ux * -fact;
+      st.v += uy * -fact;
+      break;
+    }
+    case TSM_BC_NOSLIP: st.u = bs[1]; st.v = bs[2]; st.T = bs[0]; break;
+    case TSM_BC_PRESSURE: st.p = bs[3]; break;
+    case TSM_BC_INLET: st = Prim{bs[0], bs[1], bs[2], bs[3]}; break;
+    default: break;
+  }
+}
+// Two-state boundary condition, src/bc.h:569-610
+__device__ __forceinline__ void bc_apply2(int type, const double* bs, double nx, double ny,
+                                          Prim& s0, Prim& s1) {
+  switch (type) {
+    case TSM_BC_SLIP: {
+      double im = 1.0 / sqrt(nx * nx + ny * ny);
+      double ux = nx * im, uy = ny * im;
+      double fact = s0.u * ux + s0.v * uy;
+      s0.u += ux * -fact;
+      s0.v += uy * -fact;
+      s1 = s0;
+      break;
+    }
+    case TSM_BC_NOSLIP:
+      s0.u = bs[1]; s0.v = bs[2];
+      s1.u = s0.u; s1.v = s0.v; s1.p = s0.p;
+      s0.T = bs[0]; s1.T = bs[0];
+      break;
+    case TSM_BC_PRESSURE: s1.p = bs[3]; break;
+    case TSM_BC_INLET: s0 = Prim{bs[0], bs[1], bs[2], bs[3]}; s1 = s0; break;
+    case TSM_BC_OUTLET: s1 = s0; break;
+    case TSM_BC_FARFIELD: s1 = Prim{bs[0], bs[1], bs[2], bs[3]}; break;
+    default: break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cell gradient G_c = sum_j N_cj V(v_j) with the boundary correction of
+// src/fv.cc:887-966 (unscaled: G_c = 2 A_c grad V).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void cell_gradient(const DevGeo& G, const StageArgs& A, const Gas& gas,
+                                              int c, bool corr, double* g) {
+  const size_t cs = A.cs, off = A.off;
+  int a = G.cv[3 * c], b = G.cv[3 * c + 1], d = G.cv[3 * c + 2];
+  double xa = G.x[a], ya = G.y[a], xb = G.x[b], yb = G.y[b], xd = G.x[d], yd = G.y[d];
+  // cell normals, src/grid_preproc.cc:336-362: N0 = dr12, N1 = dr20, N2 = dr01
+  double n0x = (yb - yd), n0y = -(xb - xd);
+  double n1x = (yd - ya), n1y = -(xd - xa);
+  double n2x = (ya - yb), n2y = -(xa - xb);
+  Ent ea = load_ent(A.E_in, cs, off + a), eb = load_ent(A.E_in, cs, off + b),
+      ed = load_ent(A.E_in, cs, off + d);
+  const double qa[4] = {ea.f, ea.vx, ea.vy, ea.l};
+  const double qb[4] = {eb.f, eb.vx, eb.vy, eb.l};
+  const double qd[4] = {ed.f, ed.vx, ed.vy, ed.l};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    g[2 * k] = n0x * qa[k] + n1x * qb[k] + n2x * qd[k];
+    g[2 * k + 1] = n0y * qa[k] + n1y * qb[k] + n2y * qd[k];
+  }
+  if (corr) {
+    for (int t = 0; t < 2; ++t) {
+      int bf = G.cbf[2 * c + t];
+      if (bf < 0) break;
+      int v0 = G.bv[2 * bf], v1 = G.bv[2 * bf + 1];
+      double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
+      int type = G.btype[bf];
+      Ent E0 = load_ent(A.E_in, cs, off + v0), E1 = load_ent(A.E_in, cs, off + v1);
+      Ent q0 = E0, q1 = E1;
+      if (type == TSM_BC_SLIP || type == TSM_BC_NOSLIP || type == TSM_BC_PRESSURE ||
+          type == TSM_BC_INLET) {
+        Prim s0 = load_prim(A.W_in, cs, off + v0), s1 = load_prim(A.W_in, cs, off + v1);
+        bc_apply1(type, G.bstate + 4 * bf, nx, ny, s0);
+        bc_apply1(type, G.bstate + 4 * bf, nx, ny, s1);
+        q0 = prim2ent(gas, s0);
+        q1 = prim2ent(gas, s1);
+      }
+      const double dq[4] = {(q0.f + q1.f - E0.f - E1.f) / 2.0, (q0.vx + q1.vx - E0.vx - E1.vx) / 2.0,
+                            (q0.vy + q1.vy - E0.vy - E1.vy) / 2.0, (q0.l + q1.l - E0.l - E1.l) / 2.0};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        g[2 * k] += nx * dq[k];
+        g[2 * k + 1] += ny * dq[k];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K1: vertex gradients (src/fv.cc:699-1059, src/reduce_gradient_periodic.cc)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gradient(DevGeo G, StageArgs A) {
+  const int s = blockIdx.y;
+  if (!A.ignore_done && A.ctl[s].done) return;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= G.nv) return;
+  const size_t cs = A.cs;
+  StageArgs As = A;
+  As.off = (size_t)s * G.nv;
+  Gas gas{A.gamma[s], A.gasR[s]};
+  int p = G.pset[v];
+  int m0, m1;
+  if (p < 0) { m0 = -1; m1 = 0; } else { m0 = G.pptr[p]; m1 = G.pptr[p + 1]; }
+  double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int mi = m0; mi < m1; ++mi) {
+    int m = (mi < 0) ? v : G.pv[mi];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int e = G.vc_ptr[m]; e < G.vc_ptr[m + 1]; ++e) {
+      int ce = G.vc[e];
+      int c = ce & 0x7fffffff;
+      double g[8];
+      cell_gradient(G, As, gas, c, ce < 0, g);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] += g[k];
+    }
+    if (p < 0) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) tot[k] = acc[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) tot[k] += acc[k] * 1.0;
+    }
+  }
+  double fact = G.gfact[v];
+  double* dE = A.dE + As.off + v;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) dE[(size_t)k * cs] = tot[k] * fact;
+}
+
+// ---------------------------------------------------------------------------
+// K2: edge fluxes (src/fv.cc:1071-1097, src/material.cc:27-74)
+// ---------------------------------------------------------------------------
+template <int FLUX, int REC>
+__global__ __launch_bounds__(256) void k_flux(DevGeo G, StageArgs A) {
+  const int s = blockIdx.y;
+  if (!A.ignore_done && A.ctl[s].done) return;
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= G.nf) return;
+  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  Gas gas{A.gamma[s], A.gasR[s]};
+  int l = G.fv[2 * f], r = G.fv[2 * f + 1];
+  double nx = G.fn[2 * f], ny = G.fn[2 * f + 1];
+  double area = sqrt(nx * nx + ny * ny);
+  double ia = 1.0 / area;
+  double unx = nx * ia, uny = ny * ia;
+  Prim Wl = load_prim(A.W_in, cs, off + l), Wr = load_prim(A.W_in, cs, off + r);
+  Cons F;
+  if (FLUX == F_ROE || FLUX == F_RUS) {
+    double gl[8], gr[8];
+    if (REC != R_FIRST) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        gl[k] = A.dE[(size_t)k * cs + off + l];
+        gr[k] = A.dE[(size_t)k * cs + off + r];
+      }
+    }
+    double rx = G.x[r] - G.x[l], ry = G.y[r] - G.y[l];
+    double ssw = 0.0;
+    if (FLUX == F_ROE && A.ssw) ssw = A.ssw[off + l] + A.ssw[off + r];
+    F = tecno_flux<FLUX == F_ROE, REC>(gas, A.fo, rx, ry, Wl, Wr, gl, gr, unx, uny, ssw);
+  } else if (FLUX == F_KEPEC) {
+    F = kepec_flux(gas, Wl, Wr, unx, uny);
+  } else if (FLUX == F_KEP) {
+    F = kep_flux(gas, Wl, Wr, unx, uny);
+  } else if (FLUX == F_ROEEC) {
+    F = roe_ec_flux(gas, Wl, Wr, unx, uny);
+  } else {
+    F = avg_flux(gas, Wl, Wr, unx, uny);
+  }
+  const size_t fs = A.fs, fo = (size_t)s * G.nf + f;
+  A.F[fo] = F.m * area;
+  A.F[fs + fo] = F.mx * area;
+  A.F[2 * fs + fo] = F.my * area;
+  A.F[3 * fs + fo] = F.e * area;
+}
+
+// Boundary flux contribution of bface b at its vertex m (src/fv.cc:1100-1189)
+template <int FLUX, int REC>
+__device__ __forceinline__ Cons bface_flux(const DevGeo& G, const StageArgs& A, const Gas& gas,
+                                           int b, const Prim& wm, double xm, double ym) {
+  int type = G.btype[b];
+  double nx = G.bn[2 * b], ny = G.bn[2 * b + 1];
+  Prim s0 = wm, s1 = wm;
+  bc_apply2(type, G.bstate + 4 * b, nx, ny, s0, s1);
+  if (type == TSM_BC_FARFIELD) {
+    double area = sqrt(nx * nx + ny * ny);
+    double ia = 1.0 / area;
+    double unx = nx * ia, uny = ny * ia;
+    Cons F;
+    if (FLUX == F_ROE || FLUX == F_RUS) {
+      const double z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      F = tecno_flux<FLUX == F_ROE, REC>(gas, A.fo, 0.0 * xm, 0.0 * ym, s0, s1, z, z, unx, uny, 0.0);
+    } else if (FLUX == F_KEPEC) {
+      F = kepec_flux(gas, s0, s1, unx, uny);
+    } else if (FLUX == F_KEP) {
+      F = kep_flux(gas, s0, s1, unx, uny);
+    } else if (FLUX == F_ROEEC) {
+      F = roe_ec_flux(gas, s0, s1, unx, uny);
+    } else {
+      F = avg_flux(gas, s0, s1, unx, uny);
+    }
+    F.m *= area; F.mx *= area; F.my *= area; F.e *= area;
+    return F;
+  }
+  return inviscid_bflux(gas, s0, s1, nx, ny);
+}
+
+// Viscous flux of cell c delivered to its local vertex slot j
+// (src/fv.cc:1197-1232, src/material.cc:158-230).
+__device__ __forceinline__ Cons viscous_part(const DevGeo& G, const StageArgs& A, const Gas& gas,
+                                             int c, int j) {
+  const size_t cs = A.cs, off = A.off;
+  int a = G.cv[3 * c], b = G.cv[3 * c + 1], d = G.cv[3 * c + 2];
+  Ent ea = load_ent(A.E_in, cs, off + a), eb = load_ent(A.E_in, cs, off + b),
+      ed = load_ent(A.E_in, cs, off + d);
+  const double fact = 1.0 / 3.0;
+  Ent s;
+  s.f = ea.f * fact + eb.f * fact + ed.f * fact;
+  s.vx = ea.vx * fact + eb.vx * fact + ed.vx * fact;
+  s.vy = ea.vy * fact + eb.vy * fact + ed.vy * fact;
+  s.l = ea.l * fact + eb.l * fact + ed.l * fact;
+  double dE[8];
+  cell_gradient(G, A, gas, c, G.cnb[c] != 0, dE);
+  double cf = 1.0 / (2.0 * G.carea[c]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) dE[k] *= cf;
+  double xa = G.x[a], ya = G.y[a], xb = G.x[b], yb = G.y[b], xd = G.x[d], yd = G.y[d];
+  double nx, ny;
+  if (j == 0) { nx = (yb - yd); ny = -(xb - xd); }
+  else if (j == 1) { nx = (yd - ya); ny = -(xd - xa); }
+  else { nx = (ya - yb); ny = -(xa - xb); }
+  const double R = gas.R;
+  double T = -1 / (R * s.l);
+  double mu;
+  if (A.vp.mu_model == 0) mu = A.vp.mu_ref;
+  else if (A.vp.mu_model == 1) mu = A.vp.mu_ref * pow(T, 1.5) / (T + A.vp.T_0);
+  else mu = A.vp.mu_ref * pow(T / A.vp.T_ref, A.vp.omega);
+  double Cp = gas.gamma * gas.R / (gas.gamma - 1.0);
+  double k = mu * Cp / A.vp.prandtl;
+  if (!A.vp.visc_active) mu = 0.0;
+  if (!A.vp.heat_active) k = 0.0;
+  double e1 = s.vx / s.l, e2 = s.vy / s.l, e4 = 1.0 / s.l;
+  double c1 = 4.0 * mu / 3.0, c2 = k / R, c3 = 2 * mu / 3.0;
+  // dE components: [0,1] f, [2,3] vx, [4,5] vy, [6,7] l  (x, y)
+  double d1x = dE[2], d1y = dE[3], d2x = dE[4], d2y = dE[5], d4x = dE[6], d4y = dE[7];
+  double K11_1 = c1 * (-d1x + e1 * d4x);
+  double K11_2 = mu * (-d2x + e2 * d4x);
+  double K11_4 = c1 * e1 * d1x + mu * (e2 * d2x) + (-c1 * e1 * e1 - mu * (e2 * e2) + c2 * e4) * d4x;
+  double K12_1 = c3 * (d2y - e2 * d4y);
+  double K12_2 = mu * (-d1y + e1 * d4y);
+  double K12_4 = mu * e2 * d1y - c3 * e1 * d2y - (mu - c3) * e1 * e2 * d4y;
+  double K21_1 = mu * (-d2x + e2 * d4x);
+  double K21_2 = c3 * (d1x - e1 * d4x);
+  double K21_4 = -c3 * e2 * d1x + mu * e1 * d2x - (mu - c3) * e1 * e2 * d4x;
+  double K22_1 = mu * (-d1y + e1 * d4y);
+  double K22_2 = c1 * (-d2y + e2 * d4y);
+  double K22_4 = mu * (e1 * d1y) + c1 * e2 * d2y + (-mu * (e1 * e1) - c1 * e2 * e2 + c2 * e4) * d4y;
+  Cons F;
+  F.m = 0.0;
+  F.mx = -((K11_1 + K12_1) * nx + (K21_1 + K22_1) * ny) * e4;
+  F.my = -((K11_2 + K12_2) * nx + (K21_2 + K22_2) * ny) * e4;
+  F.e = -((K11_4 + K12_4) * nx + (K21_4 + K22_4) * ny) * e4;
+  return F;
+}
+
+// ---------------------------------------------------------------------------
+// K3: residual gather + (optionally) RK update
+//   MODE 0: write R only; MODE 1: ssprk3 stage; MODE 2: rk4 stage
+// (src/fv.cc:1066-1192 gather side, src/reduce_residual_periodic.cc,
+//  src/fv.cc:1635-1707 update, src/fv.cc:1801-1879 norm partials)
+// ---------------------------------------------------------------------------
+template <int FLUX, int REC, int MODE>
+__global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, int last) {
+  __shared__ double lds[4 * 4];
+  const int s = blockIdx.y;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  if (!A.ignore_done && A.ctl[s].done) {
+    // a finished sample keeps its state across the buffer swap
+    if (MODE != 0 && v < G.nv) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        A.W_out[(size_t)k * cs + off + v] = A.W_in[(size_t)k * cs + off + v];
+        A.E_out[(size_t)k * cs + off + v] = A.E_in[(size_t)k * cs + off + v];
+      }
+    }
+    return;
+  }
+  StageArgs As = A;
+  As.off = off;
+  Gas gas{A.gamma[s], A.gasR[s]};
+  double part[4] = {0, 0, 0, 0};
+  if (v < G.nv) {
+    int p = G.pset[v];
+    int m0, m1;
+    if (p < 0) { m0 = -1; m1 = 0; } else { m0 = G.pptr[p]; m1 = G.pptr[p + 1]; }
+    double tot[4] = {0, 0, 0, 0};
+    const size_t fs = A.fs, fo = (size_t)s * G.nf;
+    for (int mi = m0; mi < m1; ++mi) {
+      int m = (mi < 0) ? v : G.pv[mi];
+      double acc[4] = {0, 0, 0, 0};
+      Prim wm;
+      bool have_wm = false;
+      for (int e = G.vfe_ptr[m]; e < G.vfe_ptr[m + 1]; ++e) {
+        unsigned ent = (unsigned)G.vfe[e];
+        unsigned kind = ent >> 30, idx = ent & 0x3fffffffu;
+        if (kind < 2) {
+          const double* F = A.F + fo + idx;
+          double f0 = F[0], f1 = F[fs], f2 = F[2 * fs], f3 = F[3 * fs];
+          if (kind == 0) { acc[0] += f0; acc[1] += f1; acc[2] += f2; acc[3] += f3; }
+          else { acc[0] -= f0; acc[1] -= f1; acc[2] -= f2; acc[3] -= f3; }
+        } else {
+          if (!have_wm) { wm = load_prim(A.W_in, cs, off + m); have_wm = true; }
+          Cons Fb = bface_flux<FLUX, REC>(G, A, gas, (int)idx, wm, G.x[m], G.y[m]);
+          acc[0] += Fb.m * 0.5; acc[1] += Fb.mx * 0.5; acc[2] += Fb.my * 0.5; acc[3] += Fb.e * 0.5;
+        }
+      }
+      if (A.ns) {
+        for (int e = G.vc_ptr[m]; e < G.vc_ptr[m + 1]; ++e) {
+          int c = G.vc[e] & 0x7fffffff;
+          int j = (G.cv[3 * c] == m) ? 0 : ((G.cv[3 * c + 1] == m) ? 1 : 2);
+          Cons Fv = viscous_part(G, As, gas, c, j);
+          const double cm = -(0.5 * 1.0);
+          acc[0] += Fv.m * cm; acc[1] += Fv.mx * cm; acc[2] += Fv.my * cm; acc[3] += Fv.e * cm;
+        }
+      }
+      if (p < 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) tot[k] = acc[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) tot[k] += acc[k] * 1.0;
+      }
+    }
+    if (MODE == 0) {
+      double* R = A.R + off + v;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) R[(size_t)k * cs] = tot[k];
+    } else {
+      double area = G.dcarea[v];
+      if (last) {
+        double w = G.weight[v];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          double q = tot[k] / area;
+          part[k] = (q * q) * w;
+        }
+      }
+      double dt = A.ctl[s].dt;
+      double factor = dt / (1.0 * area);
+      Prim w0 = load_prim(A.W_in, cs, off + v);
+      Cons U;
+      const double* Uo = A.Uold + off + v;
+      if (MODE == 1) {
+        const double a_rk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
+        const double b_rk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+        U = prim2con(gas, w0);
+        double b = b_rk[r], a = a_rk[r], c = -factor * b_rk[r];
+        U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
+        U.m += Uo[0] * a + tot[0] * c;
+        U.mx += Uo[cs] * a + tot[1] * c;
+        U.my += Uo[2 * cs] * a + tot[2] * c;
+        U.e += Uo[3 * cs] * a + tot[3] * c;
+      } else {
+        const double rks[4] = {1.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
+        double* R2 = A.R2 + off + v;
+        double ir = 1.0 / rks[r];
+        double q[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          double r2 = (r == 0 ? 0.0 : R2[(size_t)k * cs]) + tot[k] * ir;
+          R2[(size_t)k * cs] = r2;
+          q[k] = (r < 3) ? tot[k] : r2;
+        }
+        double c = (r < 3) ? -factor * rks[r < 3 ? r + 1 : 3] : -factor / 6.0;
+        U.m = Uo[0] * 1.0 + q[0] * c;
+        U.mx = Uo[cs] * 1.0 + q[1] * c;
+        U.my = Uo[2 * cs] * 1.0 + q[2] * c;
+        U.e = Uo[3 * cs] * 1.0 + q[3] * c;
+      }
+      Prim w1 = con2prim(gas, U);
+      store_prim(A.W_out, cs, off + v, w1);
+      store_ent(A.E_out, cs, off + v, prim2ent(gas, w1));
+    }
+  }
+  if (MODE != 0 && last) {
+    block_sum<4>(part, lds);
+    if (threadIdx.x == 0) {
+      double* P = A.partial + ((size_t)s * gridDim.x + blockIdx.x) * 4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) P[k] = part[k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K0: store U_old + local time step + block min (src/fv.cc:1499-1630)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dt(DevGeo G, StageArgs A, StepParams P) {
+  __shared__ double lds[4];
+  const int s = blockIdx.y;
+  Ctl& c = A.ctl[s];
+  // loop condition of src/fv.cc:2273-2275, evaluated at the start of a step
+  bool done = !(c.res_total > P.min_residue && (P.unsteady ? true : c.iter < P.max_iter) &&
+                c.t < P.final_time);
+  if (A.ignore_done) done = false;
+  if (done) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) c.done = 1;
+    return;
+  }
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  Gas gas{A.gamma[s], A.gasR[s]};
+  double dtv = 1.0e20;
+  if (v < G.nv) {
+    Prim w = load_prim(A.W_in, cs, off + v);
+    Cons U = prim2con(gas, w);
+    double* Uo = A.Uold + off + v;
+    Uo[0] = U.m; Uo[cs] = U.mx; Uo[2 * cs] = U.my; Uo[3 * cs] = U.e;
+    if (P.time_step > 0.0) {
+      dtv = P.time_step;
+    } else {
+      int p = G.pset[v];
+      int m0, m1;
+      if (p < 0) { m0 = -1; m1 = 0; } else { m0 = G.pptr[p]; m1 = G.pptr[p + 1]; }
+      double tot = 0.0;
+      for (int mi = m0; mi < m1; ++mi) {
+        int m = (mi < 0) ? v : G.pv[mi];
+        Prim wm = (m == v) ? w : load_prim(A.W_in, cs, off + m);
+        double cm = sound_speed(gas, wm);
+        double lam = 0.0;
+        for (int e = G.vfe_ptr[m]; e < G.vfe_ptr[m + 1]; ++e) {
+          unsigned ent = (unsigned)G.vfe[e];
+          unsigned kind = ent >> 30, idx = ent & 0x3fffffffu;
+          if (kind < 2) {
+            double nx = G.fn[2 * idx], ny = G.fn[2 * idx + 1];
+            double area = sqrt(nx * nx + ny * ny);
+            double vn = wm.u * nx + wm.v * ny;
+            lam += (fabs(vn) + cm * area) * 1.0;
+          } else {
+            double nx = G.bn[2 * idx], ny = G.bn[2 * idx + 1];
+            double area = sqrt(nx * nx + ny * ny);
+            double vn = wm.u * nx + wm.v * ny;
+            lam += 0.5 * (fabs(vn) + cm * area);
+          }
+        }
+        if (A.ns) {
+          double mu;
+          double T = wm.T;
+          if (A.vp.mu_model == 0) mu = A.vp.mu_ref;
+          else if (A.vp.mu_model == 1) mu = A.vp.mu_ref * pow(T, 1.5) / (T + A.vp.T_0);
+          else mu = A.vp.mu_ref * pow(T / A.vp.T_ref, A.vp.omega);
+          double rho = density(gas, wm);
+          for (int e = G.vfe_ptr[m]; e < G.vfe_ptr[m + 1]; ++e) {
+            unsigned ent = (unsigned)G.vfe[e];
+            unsigned kind = ent >> 30, idx = ent & 0x3fffffffu;
+            if (kind >= 2) continue;
+            double nx = G.fn[2 * idx], ny = G.fn[2 * idx + 1];
+            double area = sqrt(nx * nx + ny * ny);
+            int l = G.fv[2 * idx], rr = G.fv[2 * idx + 1];
+            double dx = G.x[rr] * 1.0 + G.x[l] * -1.0, dy = G.y[rr] * 1.0 + G.y[l] * -1.0;
+            double dl = sqrt(dx * dx + dy * dy);
+            lam += (mu * area / dl / rho) * 1.0;
+          }
+        }
+        tot = (p < 0) ? lam : tot + lam * 1.0;
+      }
+      dtv = P.cfl * G.dcarea[v] / tot;
+    }
+  }
+  double bm = block_min(dtv, lds);
+  if (threadIdx.x == 0) A.partial[(size_t)s * gridDim.x + blockIdx.x] = bm;
+}
+
+// Global dt + time-stamp clamp (src/fv.cc:1591-1618), one block per sample.
+__global__ __launch_bounds__(256) void k_dt_final(StageArgs A, StepParams P, int nblk) {
+  __shared__ double lds[4];
+  const int s = blockIdx.x;
+  Ctl& c = A.ctl[s];
+  if (!A.ignore_done && c.done) return;
+  double x = 1.0e20;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) x = fmin(x, A.partial[(size_t)s * nblk + i]);
+  x = block_min(x, lds);
+  if (threadIdx.x == 0) {
+    double dt = x;
+    if (P.unsteady) {
+      // the reference indexes past its stamp table if t falls an ulp short of
+      // final_time (src/fv.cc:1610); the last stamp is used instead
+      double target = A.tstamps[min(c.counter, P.n_stamps)];
+      if (c.t + dt > target) {
+        dt = target - c.t;
+        c.reached = 1;
+      }
+    }
+    c.dt = dt;
+  }
+}
+
+// Residual norm + time advance (src/fv.cc:1801-1879, :2307-2364)
+__global__ __launch_bounds__(256) void k_step_final(StageArgs A, StepParams P, int nblk, int nv,
+                                                    int log_slot) {
+  __shared__ double lds[4 * 4];
+  const int s = blockIdx.x;
+  Ctl& c = A.ctl[s];
+  if (c.done) return;
+  double part[4] = {0, 0, 0, 0};
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+    const double* q = A.partial + ((size_t)s * nblk + i) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) part[k] += q[k];
+  }
+  block_sum<4>(part, lds);
+  if (threadIdx.x == 0) {
+    double n[4];
+    for (int k = 0; k < 4; ++k) n[k] = sqrt(part[k]) / nv;
+    double tot = sqrt((n[0] * n[0]) + (n[1] * n[1] + n[2] * n[2]) + (n[3] * n[3]));
+    if (c.iter == 0) {
+      c.res0 = tot;
+      if (c.res0 == 0.0) c.res0 = 1.0;
+    }
+    c.res_total = tot / c.res0;
+    c.iter += 1;
+    c.t += c.dt;
+    if (c.reached && P.unsteady) { c.counter += 1; c.reached = 0; }
+    if (A.log && log_slot >= 0) {
+      double* L = A.log + ((size_t)s * A.log_cap + log_slot) * 4;
+      L[0] = (double)c.iter; L[1] = c.dt; L[2] = c.t; L[3] = c.res_total;
+    }
+  }
+}
+
+// Liou carbuncle sensor (src/compute_ssw.cc:7-39)
+__global__ void k_ssw_zero(DevGeo G, StageArgs A) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (size_t)G.nv * A.ns_samples) A.ssw[i] = 0.0;
+}
+__global__ __launch_bounds__(256) void k_ssw(DevGeo G, StageArgs A) {
+  const int s = blockIdx.y;
+  if (!A.ignore_done && A.ctl[s].done) return;
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= G.nf) return;
+  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  Gas gas{A.gamma[s], A.gasR[s]};
+  int l = G.fv[2 * f], r = G.fv[2 * f + 1];
+  double nx = G.fn[2 * f], ny = G.fn[2 * f + 1];
+  Prim wl = load_prim(A.W_in, cs, off + l), wr = load_prim(A.W_in, cs, off + r);
+  double ui = wl.u * nx + wl.v * ny, uj = wr.u * nx + wr.v * ny;
+  double ai = sound_speed(gas, wl), aj = sound_speed(gas, wr);
+  double area = sqrt(nx * nx + ny * ny);
+  double l1i = ui - ai * area, l1j = uj - aj * area;
+  double l5i = ui + ai * area, l5j = uj + aj * area;
+  if ((l1i > 0 && l1j < 0) || (l5i > 0 && l5j < 0)) {
+    A.ssw[off + l] = 1.0;
+    A.ssw[off + r] = 1.0;
+  }
+}
+
+// Entropy variables of a freshly set state
+__global__ void k_prim2ent(DevGeo G, StageArgs A, int s) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= G.nv) return;
+  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  Gas gas{A.gamma[s], A.gasR[s]};
+  Prim w = load_prim(A.W_in, cs, off + v);
+  store_ent(A.E_in, cs, off + v, prim2ent(gas, w));
+}
+
+// Global integrals (src/fv.cc:2175-2225) and error norms (src/fv.cc:1712-1795)
+__global__ __launch_bounds__(256) void k_global(DevGeo G, StageArgs A, int s, double* out) {
+  __shared__ double lds[4 * 2];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  Gas gas{A.gamma[s], A.gasR[s]};
+  double q[2] = {0, 0};
+  if (v < G.nv) {
+    Prim w = load_prim(A.W_in, cs, off + v);
+    double wt = G.weight[v], da = G.dcarea[v];
+    double rho = density(gas, w);
+    q[0] = (0.5 * rho * (w.u * w.u + w.v * w.v) * da) * wt;
+    q[1] = (-rho * entropy(gas, w) / (gas.gamma - 1.0)) * da * wt;
+  }
+  block_sum<2>(q, lds);
+  if (threadIdx.x == 0) { out[2 * blockIdx.x] = q[0]; out[2 * blockIdx.x + 1] = q[1]; }
+}
+
+__global__ __launch_bounds__(256) void k_errnorm(DevGeo G, StageArgs A, int s, const double* ex,
+                                                 double* out) {
+  __shared__ double lds[4 * 24];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  Gas gas{A.gamma[s], A.gasR[s]};
+  double q[24];
+#pragma unroll
+  for (int k = 0; k < 24; ++k) q[k] = 0.0;
+  double mx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (v < G.nv) {
+    Prim w = load_prim(A.W_in, cs, off + v);
+    Prim pe{ex[4 * v], ex[4 * v + 1], ex[4 * v + 2], ex[4 * v + 3]};
+    double aw = G.dcarea[v], wt = G.weight[v];
+    double e[8];
+    e[0] = fabs(pe.T - w.T); e[1] = fabs(pe.p - w.p); e[2] = fabs(pe.u - w.u);
+    e[3] = fabs(pe.v - w.v); e[4] = 0.0;
+    e[5] = fabs(density(gas, pe) - density(gas, w));
+    double me = sqrt(pe.u * pe.u + pe.v * pe.v) / sound_speed(gas, pe);
+    double mw = sqrt(w.u * w.u + w.v * w.v) / sound_speed(gas, w);
+    e[6] = fabs(me - mw);
+    e[7] = fabs(entropy(gas, pe) - entropy(gas, w));
+#pragma unroll
+    for (int k = 0; k < 5; ++k) { q[k] = e[k] * (aw * wt); q[8 + k] = e[k] * e[k] * (aw * wt); mx[k] = e[k]; }
+#pragma unroll
+    for (int k = 5; k < 8; ++k) { q[k] = e[k] * aw * wt; q[8 + k] = e[k] * e[k] * aw * wt; mx[k] = e[k]; }
+  }
+  // sums of the 16 L1/L2 terms; maxima through the same tree with fmax
+  block_sum<16>(q, lds);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    double x = mx[k];
+    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_down(x, o, 64));
+    mx[k] = x;
+  }
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 8; ++k) lds[wid * 8 + k] = mx[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 8; ++k) {
+      double x = lds[k];
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w) x = fmax(x, lds[w * 8 + k]);
+      mx[k] = x;
+    }
+    double* o = out + (size_t)blockIdx.x * 24;
+    for (int k = 0; k < 16; ++k) o[k] = q[k];
+    for (int k = 0; k < 8; ++k) o[16 + k] = mx[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launch dispatch
+// ---------------------------------------------------------------------------
+template <int FLUX, int REC>
+static void launch_stage_t(const DevGeo& G, const StageArgs& A, int S, int r, int last, int mode,
+                           hipStream_t st, bool do_flux, bool do_update) {
+  dim3 blk(256);
+  dim3 gv((G.nv + 255) / 256, S), gf((G.nf + 255) / 256, S);
+  if (do_flux) hipLaunchKernelGGL((k_flux<FLUX, REC>), gf, blk, 0, st, G, A);
+  if (do_update) {
+    if (mode == 0) hipLaunchKernelGGL((k_update<FLUX, REC, 0>), gv, blk, 0, st, G, A, r, last);
+    else if (mode == 1) hipLaunchKernelGGL((k_update<FLUX, REC, 1>), gv, blk, 0, st, G, A, r, last);
+    else hipLaunchKernelGGL((k_update<FLUX, REC, 2>), gv, blk, 0, st, G, A, r, last);
+  }
+}
+
+template <int FLUX>
+static void launch_rec(int rec, const DevGeo& G, const StageArgs& A, int S, int r, int last,
+                       int mode, hipStream_t st, bool df, bool du) {
+  switch (rec) {
+    case R_FIRST: launch_stage_t<FLUX, R_FIRST>(G, A, S, r, last, mode, st, df, du); break;
+    case R_SECOND: launch_stage_t<FLUX, R_SECOND>(G, A, S, r, last, mode, st, df, du); break;
+    case R_MUSCL: launch_stage_t<FLUX, R_MUSCL>(G, A, S, r, last, mode, st, df, du); break;
+    case R_ALBADA: launch_stage_t<FLUX, R_ALBADA>(G, A, S, r, last, mode, st, df, du); break;
+    case R_MINMOD: launch_stage_t<FLUX, R_MINMOD>(G, A, S, r, last, mode, st, df, du); break;
+    case R_TVB: launch_stage_t<FLUX, R_TVB>(G, A, S, r, last, mode, st, df, du); break;
+    default: launch_stage_t<FLUX, R_ENO2>(G, A, S, r, last, mode, st, df, du); break;
+  }
+}
+
+void launch_flux_update(int flux, int rec, const DevGeo& G, const StageArgs& A, int S, int r,
+                        int last, int mode, hipStream_t st, bool df, bool du) {
+  switch (flux) {
+    case F_ROE: launch_rec<F_ROE>(rec, G, A, S, r, last, mode, st, df, du); break;
+    case F_RUS: launch_rec<F_RUS>(rec, G, A, S, r, last, mode, st, df, du); break;
+    // the central fluxes do not reconstruct: one instantiation each
+    case F_KEPEC: launch_stage_t<F_KEPEC, R_FIRST>(G, A, S, r, last, mode, st, df, du); break;
+    case F_ROEEC: launch_stage_t<F_ROEEC, R_FIRST>(G, A, S, r, last, mode, st, df, du); break;
+    case F_KEP: launch_stage_t<F_KEP, R_FIRST>(G, A, S, r, last, mode, st, df, du); break;
+    default: launch_stage_t<F_AVG, R_FIRST>(G, A, S, r, last, mode, st, df, du); break;
+  }
+}
+
+void launch_gradient(const DevGeo& G, const StageArgs& A, int S, hipStream_t st) {
+  hipLaunchKernelGGL(k_gradient, dim3((G.nv + 255) / 256, S), dim3(256), 0, st, G, A);
+}
+void launch_dt(const DevGeo& G, const StageArgs& A, const StepParams& P, int S, hipStream_t st) {
+  int nb = (G.nv + 255) / 256;
+  hipLaunchKernelGGL(k_dt, dim3(nb, S), dim3(256), 0, st, G, A, P);
+  hipLaunchKernelGGL(k_dt_final, dim3(S), dim3(256), 0, st, A, P, nb);
+}
+void launch_step_final(const DevGeo& G, const StageArgs& A, const StepParams& P, int S,
+                       int log_slot, hipStream_t st) {
+  int nb = (G.nv + 255) / 256;
+  hipLaunchKernelGGL(k_step_final, dim3(S), dim3(256), 0, st, A, P, nb, G.nv, log_slot);
+}
+void launch_ssw(const DevGeo& G, const StageArgs& A, int S, hipStream_t st) {
+  size_t n = (size_t)G.nv * S;
+  hipLaunchKernelGGL(k_ssw_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, G, A);
+  hipLaunchKernelGGL(k_ssw, dim3((G.nf + 255) / 256, S), dim3(256), 0, st, G, A);
+}
+void launch_prim2ent(const DevGeo& G, const StageArgs& A, int s, hipStream_t st) {
+  hipLaunchKernelGGL(k_prim2ent, dim3((G.nv + 255) / 256), dim3(256), 0, st, G, A, s);
+}
+void launch_global(const DevGeo& G, const StageArgs& A, int s, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_global, dim3((G.nv + 255) / 256), dim3(256), 0, st, G, A, s, out);
+}
+void launch_errnorm(const DevGeo& G, const StageArgs& A, int s, const double* ex, double* out,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_errnorm, dim3((G.nv + 255) / 256), dim3(256), 0, st, G, A, s, ex, out);
+}
+
+}  // namespace tsm

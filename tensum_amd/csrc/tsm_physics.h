@@ -1,0 +1,340 @@
+// tsm_physics.h -- pointwise gas physics and numerical fluxes, device side.
+//
+// Restates, for 2-D FP64 states (T, u, v, p), the reference's pointwise
+// functions with the same operation order (src/material.h, src/material.cc,
+// src/kepec.cc, src/kep.cc, src/roe_ec.cc, src/simple_avg.cc,
+// src/kepes_tecno_roe.cc, src/kepes_tecno_rusanov.cc, src/reconstruct.cc,
+// src/limiter.h).  The reference's z components are identically zero in 2-D
+// and every term they contribute is an exact +-0, so they are not carried.
+// Compiled with -ffp-contract=off so that no multiply-add is fused.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace tsm {
+
+enum { F_ROE = 0, F_RUS = 1, F_KEPEC = 2, F_ROEEC = 3, F_KEP = 4, F_AVG = 5 };
+enum { R_FIRST = 0, R_SECOND = 1, R_MUSCL = 2, R_ALBADA = 3, R_MINMOD = 4,
+       R_TVB = 5, R_ENO2 = 6 };
+
+struct Gas {
+  double gamma, R;
+};
+
+struct Prim { double T, u, v, p; };
+struct Cons { double m, mx, my, e; };
+struct Ent { double f, vx, vy, l; };
+
+__device__ __forceinline__ double density(const Gas& g, const Prim& w) {
+  return w.p / (g.R * w.T);
+}
+__device__ __forceinline__ double sound_speed(const Gas& g, const Prim& w) {
+  return sqrt(g.gamma * g.R * w.T);
+}
+__device__ __forceinline__ double entropy(const Gas& g, const Prim& w) {
+  return log(w.p) - g.gamma * log(density(g, w));
+}
+
+// src/material.h:24-42
+__device__ __forceinline__ double logavg(double a, double b) {
+  double xi = b / a;
+  double f = (xi - 1.0) / (xi + 1.0);
+  double u = f * f;
+  double F;
+  if (u < 1.0e-2) {
+    double u2 = u * u;
+    double u3 = u2 * u;
+    F = 1.0 + u / 3.0 + u2 / 5.0 + u3 / 7.0;
+  } else {
+    F = log(xi) / 2.0 / f;
+  }
+  return 0.5 * (a + b) / F;
+}
+
+// src/material.h:501-513
+__device__ __forceinline__ Cons prim2con(const Gas& g, const Prim& w) {
+  Cons c;
+  double rho = density(g, w);
+  c.m = rho;
+  c.mx = w.u * rho;
+  c.my = w.v * rho;
+  c.e = w.p / (g.gamma - 1.0) + 0.5 * (w.u * w.u + w.v * w.v) * rho;
+  return c;
+}
+// src/material.h:518-529
+__device__ __forceinline__ Prim con2prim(const Gas& g, const Cons& c) {
+  Prim w;
+  double ir = 1.0 / c.m;
+  w.u = c.mx * ir;
+  w.v = c.my * ir;
+  w.p = (g.gamma - 1.0) * (c.e - 0.5 * (c.mx * c.mx + c.my * c.my) / c.m);
+  w.T = w.p / (g.R * c.m);
+  return w;
+}
+// src/material.h:534-546
+__device__ __forceinline__ Ent prim2ent(const Gas& g, const Prim& w) {
+  Ent e;
+  double rho = density(g, w);
+  double beta = rho / (2 * w.p);
+  double s = entropy(g, w);
+  e.f = (g.gamma - s) / (g.gamma - 1) - (w.u * w.u + w.v * w.v) * beta;
+  e.vx = w.u * (2 * beta);
+  e.vy = w.v * (2 * beta);
+  e.l = -2 * beta;
+  return e;
+}
+
+// src/limiter.h:14-77
+__device__ __forceinline__ double minmod_tvd(double a, double b) {
+  if (a < 0.0 && b < 0.0) return -fmin(-a, -b);
+  if (a > 0.0 && b > 0.0) return fmin(a, b);
+  return 0.0;
+}
+__device__ __forceinline__ double minabs(double a, double b) {
+  return (fabs(a) < fabs(b)) ? a : b;
+}
+
+// src/reconstruct.cc:101-134
+template <int REC>
+__device__ __forceinline__ double limited_slope(double ul, double ur, double h, double tvbM) {
+  if (REC == R_ALBADA) {
+    const double eps = 1.0e-14;
+    double s = fmax(0.0, (2 * ul * ur + eps) / (ul * ul + ur * ur + eps));
+    const double K = 1.0 / 3.0;
+    return 0.25 * s * ((1.0 - K * s) * ul + (1.0 + K * s) * ur);
+  } else if (REC == R_MINMOD) {
+    return 0.5 * minmod_tvd(ul, ur);
+  } else if (REC == R_ENO2) {
+    return 0.5 * minabs(ul, ur);
+  } else {  // R_TVB
+    double c = 0.5 * (ul + ur);
+    double r = (fabs(c) < tvbM * (h * h)) ? c : minmod_tvd(ul, ur);
+    return 0.5 * r;
+  }
+}
+
+// src/reconstruct.cc:13-96: one component
+template <int REC>
+__device__ __forceinline__ void reconstruct1(double zl, double zr, double gl, double gr,
+                                             double h, double tvbM, double& rl, double& rr) {
+  if (REC == R_FIRST) {
+    rl = zl; rr = zr;
+  } else if (REC == R_SECOND) {
+    rl = zl + 0.5 * gl;
+    rr = zr - 0.5 * gr;
+  } else if (REC == R_MUSCL) {
+    const double K = 1.0 / 3.0;
+    double ds = zr - zl;
+    double dl = 2.0 * gl - ds;
+    rl = zl + (dl * (1.0 - K) + ds * (1.0 + K)) * 0.25;
+    double dr = 2.0 * gr - ds;
+    rr = zr - (dr * (1.0 - K) + ds * (1.0 + K)) * 0.25;
+  } else {
+    double ds = zr - zl;
+    double dl = 2.0 * gl - ds;
+    rl = zl + limited_slope<REC>(dl, ds, h, tvbM);
+    double dr = 2.0 * gr - ds;
+    rr = zr - limited_slope<REC>(dr, ds, h, tvbM);
+  }
+}
+
+struct FluxOpts {
+  double ev_alpha, ev_beta, tvbM;
+  int nd_active;
+};
+
+// src/kepec.cc:10-38 (per unit normal)
+__device__ __forceinline__ Cons kepec_flux(const Gas& g, const Prim& l, const Prim& r,
+                                           double unx, double uny) {
+  double rhol = density(g, l), rhor = density(g, r);
+  double rho = logavg(rhol, rhor);
+  double vel2 = 0.5 * ((l.u * l.u + l.v * l.v) + (r.u * r.u + r.v * r.v));
+  double betal = 0.5 / (g.R * l.T);
+  double betar = 0.5 / (g.R * r.T);
+  double beta = logavg(betal, betar);
+  double p = 0.5 * (rhol + rhor) / (betal + betar);
+  double vx = l.u * 0.5 + r.u * 0.5, vy = l.v * 0.5 + r.v * 0.5;
+  double vn = vx * unx + vy * uny;
+  Cons F;
+  F.m = rho * vn;
+  F.mx = unx * p + vx * F.m;
+  F.my = uny * p + vy * F.m;
+  F.e = 0.5 * (1.0 / ((g.gamma - 1.0) * beta) - vel2) * F.m + (F.mx * vx + F.my * vy);
+  return F;
+}
+
+// src/kep.cc:10-38
+__device__ __forceinline__ Cons kep_flux(const Gas& g, const Prim& l, const Prim& r,
+                                         double unx, double uny) {
+  double rhol = density(g, l), rhor = density(g, r);
+  double g1 = g.gamma - 1.0;
+  double cl = sound_speed(g, l), cr = sound_speed(g, r);
+  double Hl = (cl * cl) / g1 + 0.5 * (l.u * l.u + l.v * l.v);
+  double Hr = (cr * cr) / g1 + 0.5 * (r.u * r.u + r.v * r.v);
+  double p = 0.5 * (l.p + r.p);
+  double rho = 0.5 * (rhol + rhor);
+  double H = 0.5 * (Hl + Hr);
+  double vx = l.u * 0.5 + r.u * 0.5, vy = l.v * 0.5 + r.v * 0.5;
+  double vn = vx * unx + vy * uny;
+  Cons F;
+  F.m = rho * vn;
+  F.mx = unx * p + vx * F.m;
+  F.my = uny * p + vy * F.m;
+  F.e = rho * H * vn;
+  return F;
+}
+
+// src/roe_ec.cc:10-44
+__device__ __forceinline__ Cons roe_ec_flux(const Gas& g, const Prim& l, const Prim& r,
+                                            double unx, double uny) {
+  double rhol = density(g, l), rhor = density(g, r);
+  double pl = l.p, pr = r.p;
+  double bl = sqrt(rhol / pl), br = sqrt(rhor / pr);
+  double z1b = 0.5 * (bl + br);
+  double zx = l.u * (0.5 * bl) + r.u * (0.5 * br);
+  double zy = l.v * (0.5 * bl) + r.v * (0.5 * br);
+  double z5b = 0.5 * (bl * pl + br * pr);
+  double zn = zx * unx + zy * uny;
+  double z1l = logavg(bl, br);
+  double z5l = logavg(bl * pl, br * pr);
+  Cons F;
+  F.m = z5l * zn;
+  double c1 = z5b / z1b, c2 = F.m / z1b;
+  F.mx = unx * c1 + zx * c2;
+  F.my = uny * c1 + zy * c2;
+  F.e = 0.5 * ((g.gamma + 1.0) / (g.gamma - 1.0) * F.m / z1l + (F.mx * zx + F.my * zy)) / z1b;
+  return F;
+}
+
+// src/simple_avg.cc:10-46
+__device__ __forceinline__ Cons avg_flux(const Gas& g, const Prim& l, const Prim& r,
+                                         double unx, double uny) {
+  double rhol = density(g, l), rhor = density(g, r);
+  double El = 0.5 * rhol * (l.u * l.u + l.v * l.v) + l.p / (g.gamma - 1);
+  double Er = 0.5 * rhor * (r.u * r.u + r.v * r.v) + r.p / (g.gamma - 1);
+  double vnl = l.u * unx + l.v * uny, vnr = r.u * unx + r.v * uny;
+  double flm = rhol * vnl, frm = rhor * vnr;
+  double flx = unx * l.p + l.u * flm, fly = uny * l.p + l.v * flm, fle = (El + l.p) * vnl;
+  double frx = unx * r.p + r.u * frm, fry = uny * r.p + r.v * frm, fre = (Er + r.p) * vnr;
+  Cons F;
+  F.m = (0.0 + flm * 0.5) + frm * 0.5;
+  F.mx = (0.0 + flx * 0.5) + frx * 0.5;
+  F.my = (0.0 + fly * 0.5) + fry * 0.5;
+  F.e = (0.0 + fle * 0.5) + fre * 0.5;
+  return F;
+}
+
+// src/kepes_tecno_roe.cc:11-173, src/kepes_tecno_rusanov.cc:11-164.
+// gl/gr: left/right vertex gradients of (f, vx, vy, l), x then y.
+template <bool ROE, int REC>
+__device__ __forceinline__ Cons tecno_flux(const Gas& g, const FluxOpts& o, double rx, double ry,
+                                           const Prim& l, const Prim& r, const double* gl,
+                                           const double* gr, double unx, double uny,
+                                           double ssw) {
+  const double gam = g.gamma, R = g.R;
+  double rhol = density(g, l), rhor = density(g, r);
+  double rho = logavg(rhol, rhor);
+  double betal = 0.5 / (R * l.T);
+  double betar = 0.5 / (R * r.T);
+  double beta = logavg(betal, betar);
+  double a = sqrt(0.5 * gam / beta);
+  double p = 0.5 * (rhol + rhor) / (betal + betar);
+  double vx = l.u * 0.5 + r.u * 0.5, vy = l.v * 0.5 + r.v * 0.5;
+  double vn = vx * unx + vy * uny;
+  double vel2 = 0.5 * ((l.u * l.u + l.v * l.v) + (r.u * r.u + r.v * r.v));
+
+  // KEPEC central part with the averages above (identical values to a
+  // separate kepec_flux call).
+  Cons F;
+  F.m = rho * vn;
+  F.mx = unx * p + vx * F.m;
+  F.my = uny * p + vy * F.m;
+  F.e = 0.5 * (1.0 / ((gam - 1.0) * beta) - vel2) * F.m + (F.mx * vx + F.my * vy);
+  if (!o.nd_active) return F;
+
+  double vsq = vx * vx + vy * vy;
+  double H = a * a / (gam - 1.0) + 0.5 * vsq;
+  double v1 = vx * uny - vy * unx;
+  double const1 = sqrt(rho / gam);
+  double psq = sqrt(p);
+  double s0 = sqrt(0.5) * const1, s1 = sqrt(gam - 1.0) * const1;
+  // scaled right eigenvectors, Rm[row][col]; rows: mass, x-mom, y-mom, energy
+  double Rm[4][4];
+  Rm[0][0] = 1 * s0;               Rm[0][1] = 1 * s1;     Rm[0][2] = 0 * psq;     Rm[0][3] = 1 * s0;
+  Rm[1][0] = (vx - a * unx) * s0;  Rm[1][1] = vx * s1;    Rm[1][2] = uny * psq;   Rm[1][3] = (vx + a * unx) * s0;
+  Rm[2][0] = (vy - a * uny) * s0;  Rm[2][1] = vy * s1;    Rm[2][2] = -unx * psq;  Rm[2][3] = (vy + a * uny) * s0;
+  Rm[3][0] = (H - a * vn) * s0;    Rm[3][1] = (0.5 * vsq) * s1; Rm[3][2] = v1 * psq; Rm[3][3] = (H + a * vn) * s0;
+
+  double Lam[4];
+  if (ROE) {
+    double vnl = l.u * unx + l.v * uny, vnr = r.u * unx + r.v * uny;
+    double al = sound_speed(g, l), ar = sound_speed(g, r);
+    double l2 = fabs(vn), l3 = fabs(vn);
+    if (ssw > 0) l3 = fmax(l3, a);
+    Lam[0] = (1 + o.ev_beta) * fabs(vn - a) + o.ev_alpha * fabs((vnl - al) - (vnr - ar));
+    Lam[1] = l2;
+    Lam[2] = l3;
+    Lam[3] = (1 + o.ev_beta) * fabs(vn + a) + o.ev_alpha * fabs((vnl + al) - (vnr + ar));
+  } else {
+    double lr = fabs(vn) + fabs(a);
+    Lam[0] = Lam[1] = Lam[2] = Lam[3] = lr;
+  }
+  double sl = entropy(g, l), sr = entropy(g, r);
+  double Vl[4] = {(gam - sl) / (gam - 1.0) - betal * (l.u * l.u + l.v * l.v), 2 * l.u * betal,
+                  2 * l.v * betal, -2 * betal};
+  double Vr[4] = {(gam - sr) / (gam - 1.0) - betar * (r.u * r.u + r.v * r.v), 2 * r.u * betar,
+                  2 * r.v * betar, -2 * betar};
+  double h = 0.0;
+  if (REC == R_TVB) h = sqrt(rx * rx + ry * ry);
+  double dZ[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double zl = 0.0, zr = 0.0, sgl = 0.0, sgr = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      zl += Rm[j][i] * Vl[j];
+      zr += Rm[j][i] * Vr[j];
+    }
+    if (REC != R_FIRST) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        double gdl = gl[2 * j] * rx + gl[2 * j + 1] * ry;
+        double gdr = gr[2 * j] * rx + gr[2 * j + 1] * ry;
+        sgl += Rm[j][i] * gdl;
+        sgr += Rm[j][i] * gdr;
+      }
+    }
+    double zij, zji;
+    reconstruct1<REC>(zl, zr, sgl, sgr, h, o.tvbM, zij, zji);
+    dZ[i] = zji - zij;
+  }
+  double D[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double d = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) d += Rm[i][j] * Lam[j] * dZ[j];
+    D[i] = d;
+  }
+  F.m -= 0.5 * D[0];
+  F.mx -= 0.5 * D[1];
+  F.my -= 0.5 * D[2];
+  F.e -= 0.5 * D[3];
+  return F;
+}
+
+// src/material.cc:110-127
+__device__ __forceinline__ Cons inviscid_bflux(const Gas& g, const Prim& w, const Prim& wb,
+                                               double nx, double ny) {
+  double vnb = wb.u * nx + wb.v * ny;
+  double vn = w.u * nx + w.v * ny;
+  double rho = density(g, w);
+  Cons F;
+  F.m = rho * vnb;
+  F.mx = nx * wb.p + w.u * F.m;
+  F.my = ny * wb.p + w.v * F.m;
+  F.e = (0.5 * rho * (w.u * w.u + w.v * w.v) + g.gamma * w.p / (g.gamma - 1.0)) * vnb +
+        (wb.p - w.p) * vn;
+  return F;
+}
+
+}  // namespace tsm

@@ -1,0 +1,178 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("gas_const", C.c_double), ("prandtl", C.c_double),
+                ("mu_ref", C.c_double), ("T_ref", C.c_double), ("T_0", C.c_double),
+                ("omega", C.c_double),
+                ("model", C.c_int), ("mu_model", C.c_int), ("flux", C.c_int),
+                ("reconstruct", C.c_int), ("tvb_M", C.c_double),
+                ("ev_alpha", C.c_double), ("ev_beta", C.c_double),
+                ("nd_active", C.c_int), ("visc_active", C.c_int), ("heat_active", C.c_int),
+                ("time_scheme", C.c_int), ("unsteady", C.c_int),
+                ("cfl", C.c_double), ("time_step", C.c_double), ("final_time", C.c_double),
+                ("min_residue", C.c_double), ("max_iter", C.c_long),
+                ("n_time_stamps", C.c_int), ("cell_type", C.c_int), ("liou_fix", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: make -C oracle")
+        L = C.CDLL(ORACLE_SO)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_void_p, C.POINTER(OrcParams)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_error.restype = C.c_char_p
+        L.orc_n_face.argtypes = [C.c_void_p]
+        L.orc_n_bface.argtypes = [C.c_void_p]
+        L.orc_faces.argtypes = [C.c_void_p] * 4
+        L.orc_bfaces.argtypes = [C.c_void_p] * 4
+        L.orc_areas.argtypes = [C.c_void_p] * 4
+        L.orc_set_state.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_get_state.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_gradients.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_residual.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_compute_dt.restype = C.c_double
+        L.orc_compute_dt.argtypes = [C.c_void_p]
+        L.orc_run.restype = C.c_long
+        L.orc_run.argtypes = [C.c_void_p, C.c_long, C.c_void_p]
+        L.orc_time.restype = C.c_double
+        L.orc_time.argtypes = [C.c_void_p]
+        L.orc_error_norms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_global.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_logavg.restype = C.c_double
+        L.orc_logavg.argtypes = [C.c_double, C.c_double]
+        L.orc_num_flux.argtypes = [C.POINTER(OrcParams)] + [C.c_void_p] * 7 + [C.c_double, C.c_void_p]
+        L.orc_limited_slope.restype = C.c_double
+        L.orc_limited_slope.argtypes = [C.POINTER(OrcParams), C.c_double, C.c_double, C.c_double]
+        L.orc_reconstruct.argtypes = [C.POINTER(OrcParams), C.c_double] + [C.c_void_p] * 6
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def params(p):
+    """OrcParams from a tensum_amd.Params (same field layout as tsm_params)."""
+    c = p.to_c()
+    o = OrcParams()
+    for name, _ in OrcParams._fields_:
+        setattr(o, name, getattr(c, name))
+    return o
+
+
+class Oracle:
+    def __init__(self, mesh, prm):
+        L = lib()
+        self._mc, self._keep = mesh.to_c()
+        self._prm = params(prm)
+        h = L.orc_create(C.byref(self._mc), C.byref(self._prm))
+        if not h:
+            raise RuntimeError("oracle: " + L.orc_error().decode())
+        self._h = C.c_void_p(h)
+        self.nv = mesh.n_vertex
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def geometry(self):
+        L = lib()
+        nf, nb = L.orc_n_face(self._h), L.orc_n_bface(self._h)
+        fv = np.empty(2 * nf, np.int32); fn = np.empty(2 * nf); ft = np.empty(nf, np.int32)
+        L.orc_faces(self._h, _p(fv), _p(fn), _p(ft))
+        bv = np.empty(2 * nb, np.int32); bn = np.empty(2 * nb); lc = np.empty(nb, np.int32)
+        L.orc_bfaces(self._h, _p(bv), _p(bn), _p(lc))
+        dc, mc = np.empty(self.nv), np.empty(self.nv)
+        ca = np.empty(self._mc.n_cell)
+        L.orc_areas(self._h, _p(dc), _p(mc), _p(ca))
+        return dict(face_v=fv.reshape(-1, 2), face_n=fn.reshape(-1, 2), face_type=ft,
+                    bface_v=bv.reshape(-1, 2), bface_n=bn.reshape(-1, 2), lcell=lc,
+                    dcarea=dc, mcarea=mc, cell_area=ca)
+
+    def set_state(self, prim):
+        a = np.ascontiguousarray(prim, dtype=np.float64).reshape(self.nv, 4)
+        lib().orc_set_state(self._h, _p(a))
+
+    def get_state(self):
+        a = np.empty((self.nv, 4))
+        lib().orc_get_state(self._h, _p(a))
+        return a
+
+    def gradients(self):
+        a = np.empty((self.nv, 8))
+        lib().orc_gradients(self._h, _p(a))
+        return a
+
+    def residual(self):
+        a = np.empty((self.nv, 4))
+        lib().orc_residual(self._h, _p(a))
+        return a
+
+    def compute_dt(self):
+        return lib().orc_compute_dt(self._h)
+
+    def run(self, nsteps, want_log=False):
+        log = np.zeros((max(nsteps, 1), 4))
+        n = lib().orc_run(self._h, nsteps, _p(log))
+        if n < 0:
+            raise RuntimeError("oracle: " + lib().orc_error().decode())
+        return (n, log[:n]) if want_log else n
+
+    def time(self):
+        return lib().orc_time(self._h)
+
+    def error_norms(self, exact):
+        a = np.ascontiguousarray(exact, dtype=np.float64).reshape(self.nv, 4)
+        out = np.empty(24)
+        lib().orc_error_norms(self._h, _p(a), _p(out))
+        return out.reshape(8, 3)
+
+    def global_integrals(self):
+        k, e = C.c_double(), C.c_double()
+        lib().orc_global(self._h, C.byref(k), C.byref(e))
+        return k.value, e.value
+
+
+def num_flux(prm, xl, xr, Wl, Wr, dEl, dEr, n, ssw=0.0):
+    P = params(prm)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (xl, xr, Wl, Wr, dEl, dEr, n)]
+    F = np.empty(4)
+    lib().orc_num_flux(C.byref(P), *[_p(a) for a in arrs], float(ssw), _p(F))
+    return F
+
+
+def logavg(a, b):
+    return lib().orc_logavg(float(a), float(b))
+
+
+def limited_slope(prm, ul, ur, h=0.0):
+    return lib().orc_limited_slope(C.byref(params(prm)), float(ul), float(ur), float(h))
+
+
+def reconstruct(prm, h, zl, zr, gl, gr):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (zl, zr, gl, gr)]
+    rl, rr = np.empty(4), np.empty(4)
+    lib().orc_reconstruct(C.byref(params(prm)), float(h), *[_p(a) for a in arrs], _p(rl), _p(rr))
+    return rl, rr
