@@ -1,0 +1,316 @@
+"""Pins of the oracle's mesh, geometry and solver functions (CPU only).
+
+The reference has no tests or golden vectors (its only numbers are README
+airfoil forces and the survey's probe runs of its binary).  These checks fix
+the oracle by invariants and closed forms that a plausible mistake breaks:
+ * geometry (src/grid_preproc.cc): dual areas tile the domain, every dual
+   cell is closed, normals point from the first to the second face vertex,
+   median dual area = sum of A/3;
+ * gradients (src/fv.cc:699-1059): exact for linear entropy-variable fields;
+ * residual + RK (src/fv.cc:1066-1707): free-stream preservation, discrete
+   conservation on a periodic box, second-order convergence on the
+   isentropic vortex, and the vortex error norms of the reference binary's
+   own 129^2 run (density L1 2.440e-1, L2 6.422e-2, Linf 5.687e-2, 2042 steps,
+   on a different random jittered mesh of the same spacing and jitter);
+ * viscous residual (src/fv.cc:1197-1232): second-order convergence to the
+   analytic viscous term mu * Laplacian(u) of a divergence-free field;
+ * norms/integrals (src/fv.cc:1712-1795, :2175-2225): closed forms.
+The mesh generator is checked against an independent numpy restatement of its
+documented bit stream.
+"""
+import numpy as np
+import pytest
+
+import tensum_amd as T
+import oracle_binding as O
+from cases import (conserved, rel_l2, riemann, riemann_mesh, taylor_green, tgv_mesh, vortex,
+                   vortex_mesh)
+
+
+# ---- mesh generator ---------------------------------------------------------
+def _np_generator(n, box, jitter, seed):
+    mask = (1 << 64) - 1
+    state = [seed & mask]
+
+    def nxt():
+        state[0] = (state[0] + 0x9E3779B97F4A7C15) & mask
+        z = state[0]
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+        return (z ^ (z >> 31)) & mask
+
+    u = lambda: (nxt() >> 11) * 2.0 ** -53
+    x0, x1, y0, y1 = box
+    hx, hy = (x1 - x0) / (n - 1), (y1 - y0) / (n - 1)
+    X, Y = np.empty(n * n), np.empty(n * n)
+    for j in range(n):
+        for i in range(n):
+            px = x1 if i == n - 1 else x0 + i * hx
+            py = y1 if j == n - 1 else y0 + j * hy
+            if 0 < i < n - 1 and 0 < j < n - 1:
+                ux, uy = u(), u()
+                px += (jitter * (2.0 * ux - 1.0)) * hx
+                py += (jitter * (2.0 * uy - 1.0)) * hy
+            X[j * n + i], Y[j * n + i] = px, py
+    cells = []
+    for j in range(n - 1):
+        for i in range(n - 1):
+            a = j * n + i
+            b, c, d = a + 1, a + n + 1, a + n
+            cells += [a, b, c, a, c, d] if u() < 0.5 else [a, b, d, b, c, d]
+    return X, Y, np.array(cells, dtype=np.int32)
+
+
+def test_generator_matches_documented_stream():
+    box = (-5.0, 5.0, -5.0, 5.0)
+    m = T.jittered_mesh(9, box=box, jitter=0.2, seed=7, periodic=True)
+    X, Y, C = _np_generator(9, box, 0.2, 7)
+    assert np.array_equal(m.x, X) and np.array_equal(m.y, Y) and np.array_equal(m.cell_v, C)
+    # periodic sets: corners + (n-2) left/right + (n-2) bottom/top pairs
+    assert m.periodic_ptr.size - 1 == 2 * (9 - 2) + 1
+
+
+# ---- geometry ---------------------------------------------------------------
+def _weights(mesh):
+    w = np.ones(mesh.n_vertex)
+    for a, b in zip(mesh.periodic_ptr[:-1], mesh.periodic_ptr[1:]):
+        w[mesh.periodic_v[a:b]] = 1.0 / (b - a)
+    return w
+
+
+@pytest.mark.parametrize("cell", ["median", "voronoi"])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_dual_cells_tile_the_domain_and_close(cell, periodic):
+    mesh = vortex_mesh(17) if periodic else riemann_mesh(17)
+    area = 100.0 if periodic else 1.0
+    o = O.Oracle(mesh, T.Params(cell=cell))
+    g = o.geometry()
+    assert np.sum(g["dcarea"] * _weights(mesh)) == pytest.approx(area, rel=1e-13)
+    assert np.sum(g["cell_area"]) == pytest.approx(area, rel=1e-13)
+    # closure: for every vertex the outward normals of its dual faces (+ half
+    # of its boundary edges) sum to zero
+    acc = np.zeros((mesh.n_vertex, 2))
+    np.add.at(acc, g["face_v"][:, 0], g["face_n"])
+    np.add.at(acc, g["face_v"][:, 1], -g["face_n"])
+    if not periodic:
+        np.add.at(acc, g["bface_v"][:, 0], 0.5 * g["bface_n"])
+        np.add.at(acc, g["bface_v"][:, 1], 0.5 * g["bface_n"])
+        assert np.abs(acc).max() < 1e-14
+    else:
+        interior = _weights(mesh) == 1.0
+        assert np.abs(acc[interior]).max() < 1e-13
+    # every dual normal points from the first vertex towards the second
+    d = np.stack([mesh.x, mesh.y], 1)
+    e = d[g["face_v"][:, 1]] - d[g["face_v"][:, 0]]
+    assert np.all(np.sum(e * g["face_n"], 1) > 0)
+
+
+def test_median_area_is_a_third_of_the_cells():
+    mesh = riemann_mesh(13)
+    g = O.Oracle(mesh, T.Params()).geometry()
+    acc = np.zeros(mesh.n_vertex)
+    np.add.at(acc, mesh.cell_v.reshape(-1, 3).ravel(), np.repeat(g["cell_area"], 3) / 3)
+    np.testing.assert_allclose(g["dcarea"], acc, rtol=1e-14)
+    # boundary faces first, in file order, then interior faces
+    nb = mesh.bface_tag.size
+    assert np.all(g["face_type"][:nb] >= 0) and np.all(g["face_type"][nb:] == -1)
+
+
+def test_product_host_geometry_is_bitwise_the_oracle():
+    for mesh in (riemann_mesh(21), vortex_mesh(21)):
+        for cell in ("median", "voronoi"):
+            a = T.host_geometry(mesh, cell)
+            b = O.Oracle(mesh, T.Params(cell=cell)).geometry()
+            for k in ("face_v", "face_n", "bface_v", "bface_n", "dcarea", "mcarea"):
+                assert np.array_equal(a[k], b[k]), (cell, k)
+
+
+# ---- gradients --------------------------------------------------------------
+def _state_from_linear_entropy_vars(x, y, gam=1.4):
+    """Primitive state whose entropy variables are linear in (x, y)."""
+    f = 2.0 + 0.1 * x - 0.05 * y
+    vx = 0.3 + 0.2 * x + 0.1 * y
+    vy = -0.2 + 0.05 * x - 0.15 * y
+    l = -1.0 - 0.1 * x + 0.05 * y
+    beta = -l / 2
+    u, v = vx / (2 * beta), vy / (2 * beta)
+    s = gam - (gam - 1) * (f + (u * u + v * v) * beta)
+    rho = np.exp((s + np.log(2 * beta)) / (1 - gam))
+    p = rho / (2 * beta)
+    return np.stack([p / rho, u, v, p], 1), np.array([[0.1, -0.05], [0.2, 0.1], [0.05, -0.15],
+                                                      [-0.1, 0.05]])
+
+
+def test_gradients_exact_for_linear_entropy_variables():
+    mesh = riemann_mesh(17)
+    W, G = _state_from_linear_entropy_vars(mesh.x, mesh.y)
+    o = O.Oracle(mesh, T.Params())
+    o.set_state(W)
+    dE = o.gradients().reshape(-1, 4, 2)
+    np.testing.assert_allclose(dE, np.broadcast_to(G, dE.shape), rtol=0, atol=2e-13)
+
+
+# ---- residual and time stepping --------------------------------------------
+@pytest.mark.parametrize("flux", ["kepes_tecno_roe", "kepes_tecno_rusanov", "kepec", "roe_ec",
+                                  "kep", "simple_avg"])
+def test_free_stream_is_preserved(flux):
+    mesh = vortex_mesh(17)
+    W = np.tile([1.3, 0.7, -0.4, 1.1], (mesh.n_vertex, 1))
+    o = O.Oracle(mesh, T.Params(flux=flux, reconstruct="minmod", cfl=0.8, final_time=10.0))
+    o.set_state(W)
+    assert np.abs(o.residual()).max() < 1e-13
+    o.run(20)
+    np.testing.assert_allclose(o.get_state(), W, rtol=1e-13)
+
+
+@pytest.mark.parametrize("flux,rec", [("kepes_tecno_roe", "minmod"),
+                                      ("kepes_tecno_rusanov", "van_albada"),
+                                      ("roe_ec", "first")])
+def test_periodic_box_conserves_mass_momentum_energy(flux, rec):
+    mesh = vortex_mesh(25)
+    w = _weights(mesh)
+    o = O.Oracle(mesh, T.Params(flux=flux, reconstruct=rec, cfl=0.8, final_time=10.0))
+    o.set_state(vortex(mesh.x, mesh.y))
+    a = o.geometry()["dcarea"] * w
+    q0 = (conserved(o.get_state()) * a[:, None]).sum(0)
+    o.run(60)
+    q1 = (conserved(o.get_state()) * a[:, None]).sum(0)
+    np.testing.assert_allclose(q1, q0, rtol=1e-13, atol=1e-12)
+
+
+def test_vortex_second_order_convergence():
+    prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=1.0)
+    errs = []
+    for n in (33, 65, 129):   # 17^2 resolves the unit vortex core too coarsely
+        mesh = vortex_mesh(n)
+        o = O.Oracle(mesh, prm)
+        o.set_state(vortex(mesh.x, mesh.y))
+        o.run(10 ** 6)
+        assert o.time() == pytest.approx(1.0, abs=1e-14)
+        errs.append(o.error_norms(vortex(mesh.x, mesh.y, 1.0))[5, 1])
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(rates > 1.6), (errs, rates)
+
+
+@pytest.mark.slow_cpu
+def test_vortex_129_matches_reference_probe():
+    """The reference binary's 129^2 run (KEPES-TeCNO-Roe, minmod, CFL 0.8,
+    t = 10) gave density L1/L2/Linf 2.440e-1 / 6.422e-2 / 5.687e-2 in 2042
+    steps on a throwaway jittered mesh; this mesh has the same spacing and
+    jitter but other random draws, so agreement within a few percent is the
+    expected result of a correct restatement."""
+    mesh = vortex_mesh(129)
+    o = O.Oracle(mesh, T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8,
+                                final_time=10.0))
+    o.set_state(vortex(mesh.x, mesh.y))
+    n = o.run(10 ** 6)
+    e = o.error_norms(vortex(mesh.x, mesh.y, 10.0))[5]
+    assert abs(n - 2042) / 2042 < 0.03
+    np.testing.assert_allclose(e, [2.440e-1, 6.422e-2, 5.687e-2], rtol=0.05)
+
+
+def test_rk4_matches_ssprk3_in_the_small_dt_limit():
+    mesh = vortex_mesh(17)
+    outs = []
+    for scheme in ("ssprk3", "rk4"):
+        o = O.Oracle(mesh, T.Params(flux="kepec", time_scheme=scheme, cfl=0.0, time_step=1e-3,
+                                    final_time=10.0))
+        o.set_state(vortex(mesh.x, mesh.y))
+        o.run(20)
+        outs.append(o.get_state())
+    assert rel_l2(outs[0], outs[1]) < 1e-10
+
+
+def test_shock_problem_keeps_positive_states_for_1000_stages():
+    mesh = riemann_mesh(33)
+    o = O.Oracle(mesh, T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5,
+                                final_time=0.8))
+    o.set_state(riemann(mesh.x, mesh.y))
+    assert o.run(334) == 334
+    W = o.get_state()
+    assert np.all(W[:, 0] > 0) and np.all(W[:, 3] > 0) and np.all(np.isfinite(W))
+
+
+# ---- viscous residual -------------------------------------------------------
+def _tg_uniform_thermo(mesh):
+    W = taylor_green(mesh.x, mesh.y)
+    W[:, 3] = W[0, 3]  # uniform pressure and temperature: only shear stresses
+    W[:, 0] = W[0, 0]
+    return W
+
+
+def _viscous_part(mesh, W, mu):
+    ns = O.Oracle(mesh, T.Params(model="ns", flux="kepec", mu_ref=mu, prandtl=0.72, cfl=0.5,
+                                 final_time=1.0))
+    eu = O.Oracle(mesh, T.Params(model="euler", flux="kepec", cfl=0.5, final_time=1.0))
+    ns.set_state(W)
+    eu.set_state(W)
+    return ns.residual() - eu.residual(), ns.geometry()
+
+
+def test_viscous_momentum_equals_p1_galerkin_stress():
+    """With uniform T and rho the entropy-variable K-matrix form must reduce to
+    the P1 Galerkin weak form R_j = sum_c A_c tau_c . grad(phi_j), tau the
+    Stokes stress of the piecewise-linear velocity (independent assembly)."""
+    mu = 0.05
+    mesh = tgv_mesh(17)
+    W = _tg_uniform_thermo(mesh)
+    Rv, g = _viscous_part(mesh, W, mu)
+    P = np.stack([mesh.x, mesh.y], 1)
+    cv = mesh.cell_v.reshape(-1, 3)
+    R = np.zeros((mesh.n_vertex, 2))
+    for c in cv:
+        p = P[c]
+        A = 0.5 * ((p[1, 0] - p[0, 0]) * (p[2, 1] - p[0, 1]) - (p[1, 1] - p[0, 1]) * (p[2, 0] - p[0, 0]))
+        # grad phi_j = rot(opposite edge) / (2A)
+        gphi = np.array([[p[(j + 1) % 3, 1] - p[(j + 2) % 3, 1],
+                          p[(j + 2) % 3, 0] - p[(j + 1) % 3, 0]] for j in range(3)]) / (2 * A)
+        gu, gv = gphi.T @ W[c, 1], gphi.T @ W[c, 2]
+        div = gu[0] + gv[1]
+        tau = mu * np.array([[2 * gu[0] - 2 / 3 * div, gu[1] + gv[0]],
+                             [gu[1] + gv[0], 2 * gv[1] - 2 / 3 * div]])
+        R[c] += A * gphi @ tau.T
+    # periodic duplicates carry the sum of their members
+    for a, b in zip(mesh.periodic_ptr[:-1], mesh.periodic_ptr[1:]):
+        R[mesh.periodic_v[a:b]] = R[mesh.periodic_v[a:b]].sum(0)
+    np.testing.assert_allclose(Rv[:, 1:3], R, rtol=0, atol=1e-12 * np.abs(R).max())
+    assert np.abs(Rv[:, 0]).max() == 0.0   # no viscous mass flux
+
+
+def test_viscous_dissipation_converges_to_exact_integral():
+    """Weak-form check: sum_j u_j . R_j approximates the viscous dissipation
+    integral of tau : grad u = 4 mu pi^2 for the Taylor-Green field."""
+    mu = 0.05
+    errs = []
+    for n in (17, 33, 65):
+        mesh = tgv_mesh(n)
+        W = _tg_uniform_thermo(mesh)
+        Rv, g = _viscous_part(mesh, W, mu)
+        phi = np.sum(_weights(mesh) * (W[:, 1] * Rv[:, 1] + W[:, 2] * Rv[:, 2]))
+        errs.append(abs(phi - 4 * mu * np.pi ** 2))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert errs[-1] < 2e-3 * 4 * mu * np.pi ** 2 and np.all(rates > 1.5), (errs, rates)
+
+
+# ---- norms and integrals ----------------------------------------------------
+def test_error_norms_closed_form():
+    mesh = vortex_mesh(17)
+    o = O.Oracle(mesh, T.Params())
+    W = vortex(mesh.x, mesh.y)
+    o.set_state(W)
+    assert np.abs(o.error_norms(W)).max() == 0.0
+    d = 1e-3
+    E = W.copy()
+    E[:, 1] += d   # u
+    e = o.error_norms(E)
+    assert e[2, 0] == pytest.approx(d * 100.0, rel=1e-10)      # u: L1 = d * area
+    assert e[2, 1] == pytest.approx(d * 10.0, rel=1e-10)       # L2 = d * sqrt(area)
+    assert e[2, 2] == pytest.approx(d, rel=1e-10)              # Linf
+
+
+def test_taylor_green_kinetic_energy_is_pi_squared():
+    mesh = tgv_mesh(129)
+    o = O.Oracle(mesh, T.Params())
+    o.set_state(taylor_green(mesh.x, mesh.y))
+    ke, _ = o.global_integrals()
+    assert ke == pytest.approx(np.pi ** 2, rel=2e-3)
