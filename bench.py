@@ -23,19 +23,28 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes(nv, nf, nc):
-    """Algorithmic HBM bytes per launch of each kernel (FP64 8 B, int32 4 B;
-    every array counted once per launch)."""
+def kernel_bytes(nv, nf, nc, tiles):
+    """Algorithmic HBM bytes per launch of each kernel: FP64 8 B, int32 4 B,
+    every array counted once per launch, halo re-reads not counted."""
+    fl, halo = tiles["faces"], tiles["halo"]
     return {
-        # U_old write, W read, dcarea, face/bface CSR, face normals
+        # CSR kernels
         "dt": 76 * nv + 24 * nf,
-        # entropy vars, coords, 1/(6 mcarea), cell ids, vertex->cell CSR, dE write
         "gradient": 124 * nv + 24 * nc,
-        # face ids + normal, W/dE/coords of the vertices, flux write
         "flux": 112 * nv + 56 * nf,
-        # CSR, fluxes, W, U_old, dcarea, weight, W + entropy write
         "update": 148 * nv + 40 * nf,
+        # tiled kernels: W + coords + 1/(6 mcarea) + cell entries (3 per cell,
+        # 4 B) + CSR offsets + halo ids, dE write
+        "grad_tile": (32 + 16 + 8 + 4 + 64) * nv + 12 * nc + 4 * halo,
+        # W, dE, coords, U_old source W^n, dcarea, weight, offsets, W write;
+        # local faces (slot pair 4 B + normal 16 B), 16-bit gather entries
+        "fu_tile": (32 + 64 + 16 + 32 + 8 + 8 + 4 + 32) * nv + 20 * fl + 4 * nf + 4 * halo,
     }
+
+
+# Whole-step algorithmic bytes per vertex-stage of an SSP-RK3 step at the
+# 1025^2 size (3 (256 V + 24 E + 12 T) + (40 V + 24 E) divided by 3 V).
+STEP_BYTES_PER_VERTEX_STAGE = 389.2
 
 
 def main():
@@ -44,9 +53,22 @@ def main():
     ap.add_argument("--steps", type=int, default=334)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1025)
+    ap.add_argument("--tile", type=int, default=0, help="owned vertices per tile (0: 128)")
+    ap.add_argument("--path", default="auto", choices=["auto", "csr"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 1 GiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": x.numel() * 8, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
               file=sys.stderr)
@@ -60,7 +82,7 @@ def main():
     n = args.n
     mesh = riemann_mesh(n, seed=1)
     prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8,
-                   time_scheme="ssprk3")
+                   time_scheme="ssprk3", kernel_path=args.path, tile_vertices=args.tile)
     w0 = riemann(mesh.x, mesh.y)
     print(f"bench: mesh {n}x{n} ({mesh.n_vertex} vertices), building context", file=sys.stderr,
           flush=True)
@@ -84,17 +106,24 @@ def main():
 
     # per-kernel durations (HIP events on the library's stream)
     ku = s.bench(20)
-    kb = kernel_bytes(nv, nf, nc)
-    per_step = {"dt": ku["dt_us"], "gradient": 3 * ku["gradient_us"], "flux": 3 * ku["flux_us"],
-                "update": 3 * ku["update_us"]}
+    tiles = T.host_tiling(mesh, prm.tile_vertices or 128)
+    kb = kernel_bytes(nv, nf, nc, tiles)
+    if ku["dt_us"] == 0.0:  # tiled kernels
+        per_launch = {"grad_tile": ku["gradient_us"], "fu_tile": ku["flux_us"]}
+    else:
+        per_launch = {"dt": ku["dt_us"], "gradient": ku["gradient_us"], "flux": ku["flux_us"],
+                      "update": ku["update_us"]}
+    per_step = {k: v * (1 if k == "dt" else 3) for k, v in per_launch.items()}
     dom = max(per_step, key=per_step.get)
-    dur_us = ku[{"dt": "dt_us", "gradient": "gradient_us", "flux": "flux_us",
-                 "update": "update_us"}[dom]]
+    dur_us = per_launch[dom]
     achieved = kb[dom] / (dur_us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None,
-                "kernel_us": {k: round(v, 2) for k, v in ku.items()}}
+                "algorithmic_bytes_per_launch": kb[dom],
+                "kernel_us": {k: round(v, 2) for k, v in ku.items()},
+                "step_frac_at_389B_per_vertex_stage":
+                    round(value * STEP_BYTES_PER_VERTEX_STAGE / (HBM_PEAK_GBS * 1e9), 4)}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -60,6 +60,12 @@ typedef struct tsm_params {
   int n_time_stamps;
   int cell_type;      /* 0 median, 1 voronoi */
   int liou_fix;
+  /* Kernel family (no counterpart in the reference): 0 = LDS-tiled stage
+     kernels where the configuration allows (Euler, SSP-RK3, no liou_fix),
+     CSR-gather kernels otherwise; 1 = always the CSR-gather kernels.  Both
+     give the reference's summation order. */
+  int kernel_path;
+  int tile_vertices;  /* owned vertices per tile of the tiled kernels (0: 128) */
 } tsm_params;
 
 /* NPART = 1 part file content (readme/README.md:155-201), 0-based gids. */
@@ -129,10 +135,17 @@ int tsm_host_geometry(const tsm_mesh* mesh, int cell_type, int* face_v, double* 
                       int* bface_v, double* bface_n, double* dcarea, double* mcarea,
                       int* n_face, int* n_bface);
 
+/* Host-only: builds and bounds-checks the vertex tiling of the tiled kernels.
+   stats[6] = tiles, periodic tiles, max slots, max local faces, halo entries,
+   local face entries. */
+int tsm_host_tiling(const tsm_mesh* mesh, int cell_type, int tile_vertices, int* stats);
+
 /* Times nsteps full steps on the context stream with HIP events and reports
-   the per-kernel average durations in microseconds:
-   us[0] dt pass, us[1] gradient, us[2] edge flux, us[3] gather+update,
-   us[4] whole step.  No host synchronisation inside the timed region. */
+   average durations in microseconds.  CSR kernels: us[0] dt pass, us[1]
+   gradient, us[2] edge flux, us[3] gather + update (per stage).  Tiled
+   kernels: us[0] 0, us[1] gradient, us[2] fused flux + gather + update (per
+   stage), us[3] step finalisation (per step).  us[4]: whole step, timed
+   without interleaved events. */
 int tsm_bench(tsm_ctx* ctx, int nsteps, double* us);
 
 /* Seeded jittered-triangle lattice on [x0,x1]x[y0,y1] with n x n vertices

@@ -37,7 +37,8 @@ class TsmParams(C.Structure):
                 ("time_scheme", C.c_int), ("unsteady", C.c_int),
                 ("cfl", C.c_double), ("time_step", C.c_double), ("final_time", C.c_double),
                 ("min_residue", C.c_double), ("max_iter", C.c_long),
-                ("n_time_stamps", C.c_int), ("cell_type", C.c_int), ("liou_fix", C.c_int)]
+                ("n_time_stamps", C.c_int), ("cell_type", C.c_int), ("liou_fix", C.c_int),
+                ("kernel_path", C.c_int), ("tile_vertices", C.c_int)]
 
 
 class TsmMesh(C.Structure):
@@ -81,6 +82,8 @@ class Params:
     time_stamps: int = 1
     cell: str = "median"
     liou_fix: bool = False
+    kernel_path: str = "auto"      # "auto" (tiled where supported) or "csr"
+    tile_vertices: int = 0
 
     def to_c(self):
         p = TsmParams()
@@ -98,6 +101,8 @@ class Params:
         p.min_residue, p.max_iter, p.n_time_stamps = self.min_residue, self.max_iter, self.time_stamps
         p.cell_type = {"median": 0, "voronoi": 1}[self.cell]
         p.liou_fix = int(self.liou_fix)
+        p.kernel_path = {"auto": 0, "csr": 1}[self.kernel_path]
+        p.tile_vertices = self.tile_vertices
         return p
 
 
@@ -174,6 +179,7 @@ def lib():
         L.tsm_mesh_jittered_sizes.argtypes = [C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 5
         L.tsm_mesh_jittered.argtypes = ([C.c_int] + [C.c_double] * 5 + [C.c_ulonglong, C.c_int]
                                         + [C.c_void_p] * 7)
+        L.tsm_host_tiling.argtypes = [C.POINTER(TsmMesh), C.c_int, C.c_int, C.c_void_p]
         L.tsm_host_geometry.argtypes = [C.POINTER(TsmMesh), C.c_int] + [C.c_void_p] * 6 + [C.POINTER(C.c_int)] * 2
         _lib = L
     return _lib
@@ -226,6 +232,16 @@ def host_geometry(mesh: Mesh, cell="median"):
                                _dptr(bvv), _dptr(bn), _dptr(dc), _dptr(mc), C.byref(nf), C.byref(nb)))
     return dict(face_v=fv.reshape(-1, 2), face_n=fn.reshape(-1, 2), bface_v=bvv.reshape(-1, 2),
                 bface_n=bn.reshape(-1, 2), dcarea=dc, mcarea=mc)
+
+
+def host_tiling(mesh: Mesh, tile_vertices=128, cell="median"):
+    """Builds and bounds-checks the tiled kernels' vertex tiling on the host."""
+    m, keep = mesh.to_c()
+    st = np.zeros(6, dtype=np.int32)
+    _check(lib().tsm_host_tiling(C.byref(m), {"median": 0, "voronoi": 1}[cell], tile_vertices,
+                                 _dptr(st)))
+    return dict(zip(("tiles", "periodic_tiles", "max_slots", "max_faces", "halo", "faces"),
+                    map(int, st)))
 
 
 class Solver:

@@ -39,9 +39,15 @@ struct tsm_ctx {
   tsm::StageArgs A{};
   tsm::StepParams SP{};
   std::vector<void*> allocs;
-  double* W[2] = {nullptr, nullptr};
+  double* W[3] = {nullptr, nullptr, nullptr};
   double* E[2] = {nullptr, nullptr};
-  int cur = 0;
+  int cur = 0;              // W[cur] / E[cur] hold the current state
+  bool tiled = false;       // LDS-tiled stage kernels in use
+  bool e_valid = true;      // E[cur] == prim2ent(W[cur])
+  tsm::Tiling tiling;
+  tsm::TileGeo TG{};
+  double* dtpart = nullptr;
+  std::vector<int> perm;    // part-file vertex id -> device vertex id
   double* red = nullptr;   // reduction partials for global / error norms
   size_t red_cap = 0;
   std::vector<double> h_gamma, h_R;
@@ -73,6 +79,20 @@ struct tsm_ctx {
   void bind_state() {
     A.W_in = W[cur]; A.W_out = W[cur ^ 1];
     A.E_in = E[cur]; A.E_out = E[cur ^ 1];
+  }
+  // Makes W[cur] the state of the CSR kernels' double buffer (W[0]/W[1]) and
+  // refreshes the entropy variables they read.
+  void csr_ready() {
+    if (cur == 2) {
+      (void)hipMemcpyAsync(W[0], W[2], 4 * A.cs * sizeof(double), hipMemcpyDeviceToDevice, st);
+      cur = 0;
+      e_valid = false;
+    }
+    bind_state();
+    if (!e_valid) {
+      for (int s = 0; s < S; ++s) tsm::launch_prim2ent(G, A, s, st);
+      e_valid = true;
+    }
   }
 };
 
@@ -128,30 +148,47 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     delete c;
     return fail(TSM_E_MESH, "mesh too large for 30-bit face ids");
   }
-  // ---- host layout --------------------------------------------------------
+  // ---- path and vertex numbering -----------------------------------------
+  c->tiled = params->kernel_path == 0 && params->model == 0 && params->time_scheme == 0 &&
+             !params->liou_fix;
+  std::vector<int> perm(nv), inv(nv);
+  if (c->tiled) {
+    int tb = params->tile_vertices > 0 ? params->tile_vertices : 128;
+    if (tb > 256) { delete c; return fail(TSM_E_ARG, "tile_vertices must be <= 256"); }
+    if (!tsm::build_tiles(g, tb, c->tiling, msg) || !tsm::validate_tiles(g, c->tiling, msg)) {
+      delete c;
+      return fail(TSM_E_MESH, "tiling: " + msg);
+    }
+    perm = c->tiling.perm;
+    inv = c->tiling.inv;
+  } else {
+    for (int i = 0; i < nv; ++i) perm[i] = inv[i] = i;
+  }
+  c->perm = perm;
+  // ---- host layout (device vertex ids) -----------------------------------
   std::vector<int> fv(2 * (size_t)nf);
   std::vector<double> fn(2 * (size_t)nf);
   for (int f = 0; f < nf; ++f) {
-    fv[2 * f] = g.face[f].v[0]; fv[2 * f + 1] = g.face[f].v[1];
+    fv[2 * f] = perm[g.face[f].v[0]]; fv[2 * f + 1] = perm[g.face[f].v[1]];
     fn[2 * f] = g.face[f].nx; fn[2 * f + 1] = g.face[f].ny;
   }
   // vertex -> face/bface CSR in reference order
   std::vector<int> vfe_ptr(nv + 1, 0), vfe;
-  for (int f = 0; f < nf; ++f) { vfe_ptr[g.face[f].v[0] + 1]++; vfe_ptr[g.face[f].v[1] + 1]++; }
+  for (int f = 0; f < nf; ++f) { vfe_ptr[fv[2 * f] + 1]++; vfe_ptr[fv[2 * f + 1] + 1]++; }
   for (int b = 0; b < nb; ++b)
-    if (!g.bface[b].periodic) { vfe_ptr[g.bface[b].v[0] + 1]++; vfe_ptr[g.bface[b].v[1] + 1]++; }
+    if (!g.bface[b].periodic) { vfe_ptr[perm[g.bface[b].v[0]] + 1]++; vfe_ptr[perm[g.bface[b].v[1]] + 1]++; }
   for (int i = 0; i < nv; ++i) vfe_ptr[i + 1] += vfe_ptr[i];
   vfe.resize(vfe_ptr[nv]);
   {
     std::vector<int> pos(vfe_ptr.begin(), vfe_ptr.end() - 1);
     for (int f = 0; f < nf; ++f) {
-      vfe[pos[g.face[f].v[0]]++] = (int)((0u << 30) | (unsigned)f);
-      vfe[pos[g.face[f].v[1]]++] = (int)((1u << 30) | (unsigned)f);
+      vfe[pos[fv[2 * f]]++] = (int)((0u << 30) | (unsigned)f);
+      vfe[pos[fv[2 * f + 1]]++] = (int)((1u << 30) | (unsigned)f);
     }
     for (int b = 0; b < nb; ++b) {
       if (g.bface[b].periodic) continue;
-      vfe[pos[g.bface[b].v[0]]++] = (int)((2u << 30) | (unsigned)b);
-      vfe[pos[g.bface[b].v[1]]++] = (int)((2u << 30) | (unsigned)b);
+      vfe[pos[perm[g.bface[b].v[0]]]++] = (int)((2u << 30) | (unsigned)b);
+      vfe[pos[perm[g.bface[b].v[1]]]++] = (int)((2u << 30) | (unsigned)b);
     }
   }
   // cell -> non-periodic bfaces (bface order)
@@ -163,33 +200,41 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     else if (cbf[2 * cl + 1] < 0) cbf[2 * cl + 1] = b;
     else { delete c; return fail(TSM_E_MESH, "cell with three boundary faces"); }
   }
+  std::vector<int> cv(3 * (size_t)nc);
+  for (size_t k = 0; k < cv.size(); ++k) cv[k] = perm[g.cell[k]];
   std::vector<int> vc_ptr(nv + 1, 0), vc;
   for (int k = 0; k < nc; ++k)
-    for (int j = 0; j < 3; ++j) vc_ptr[g.cell[3 * k + j] + 1]++;
+    for (int j = 0; j < 3; ++j) vc_ptr[cv[3 * k + j] + 1]++;
   for (int i = 0; i < nv; ++i) vc_ptr[i + 1] += vc_ptr[i];
   vc.resize(vc_ptr[nv]);
   {
     std::vector<int> pos(vc_ptr.begin(), vc_ptr.end() - 1);
     for (int k = 0; k < nc; ++k)
       for (int j = 0; j < 3; ++j)
-        vc[pos[g.cell[3 * k + j]]++] = (cbf[2 * k] >= 0) ? (int)(0x80000000u | (unsigned)k) : k;
+        vc[pos[cv[3 * k + j]]++] = (cbf[2 * k] >= 0) ? (int)(0x80000000u | (unsigned)k) : k;
   }
   std::vector<int> cnb(nc);
   for (int k = 0; k < nc; ++k) cnb[k] = (cbf[2 * k] >= 0) ? 1 : 0;
   std::vector<int> bvv(2 * (size_t)nb), btype(nb);
   std::vector<double> bn(2 * (size_t)nb), bstate(4 * (size_t)nb, 0.0);
   for (int b = 0; b < nb; ++b) {
-    bvv[2 * b] = g.bface[b].v[0]; bvv[2 * b + 1] = g.bface[b].v[1];
+    bvv[2 * b] = perm[g.bface[b].v[0]]; bvv[2 * b + 1] = perm[g.bface[b].v[1]];
     bn[2 * b] = g.bface[b].nx; bn[2 * b + 1] = g.bface[b].ny;
     btype[b] = g.bface[b].bc;
     for (int i = 0; i < mesh->n_bc; ++i)
       if (mesh->bc_tag[i] == g.bface[b].tag)
         for (int k = 0; k < 4; ++k) bstate[4 * b + k] = mesh->bc_state ? mesh->bc_state[4 * i + k] : 0.0;
   }
-  std::vector<double> gfact(nv);
-  for (int i = 0; i < nv; ++i) gfact[i] = 1.0 / (6.0 * g.mcarea[i]);
-  std::vector<int> pset(g.periodic_of), pptr(1, 0), pv;
-  for (auto& P : g.periodic) { for (int v : P) pv.push_back(v); pptr.push_back((int)pv.size()); }
+  std::vector<double> xs(nv), ys(nv), dca(nv), gfact(nv), wgt(nv);
+  std::vector<int> pset(nv);
+  for (int i = 0; i < nv; ++i) {
+    int o = inv[i];
+    xs[i] = g.x[o]; ys[i] = g.y[o]; dca[i] = g.dcarea[o]; wgt[i] = g.weight[o];
+    gfact[i] = 1.0 / (6.0 * g.mcarea[o]);
+    pset[i] = g.periodic_of[o];
+  }
+  std::vector<int> pptr(1, 0), pv;
+  for (auto& P : g.periodic) { for (int v : P) pv.push_back(perm[v]); pptr.push_back((int)pv.size()); }
 
   // ---- device ------------------------------------------------------------
   hipError_t he = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
@@ -197,12 +242,22 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   tsm::DevGeo& G = c->G;
   G.nv = nv; G.nf = nf; G.nb = nb; G.nc = nc;
 #define UP(field, vec) do { if ((rc = c->upload(&G.field, vec))) { delete c; return rc; } } while (0)
-  UP(x, g.x); UP(y, g.y); UP(dcarea, g.dcarea); UP(gfact, gfact); UP(weight, g.weight);
+  UP(x, xs); UP(y, ys); UP(dcarea, dca); UP(gfact, gfact); UP(weight, wgt);
   UP(carea, g.cell_area);
   UP(fv, fv); UP(fn, fn); UP(vfe_ptr, vfe_ptr); UP(vfe, vfe); UP(vc_ptr, vc_ptr); UP(vc, vc);
-  UP(cv, g.cell); UP(cbf, cbf); UP(cnb, cnb); UP(bv, bvv); UP(bn, bn); UP(btype, btype);
+  UP(cv, cv); UP(cbf, cbf); UP(cnb, cnb); UP(bv, bvv); UP(bn, bn); UP(btype, btype);
   UP(bstate, bstate); UP(pset, pset); UP(pptr, pptr); UP(pv, pv);
 #undef UP
+  if (c->tiled) {
+    const tsm::Tiling& T = c->tiling;
+    tsm::TileGeo& TG = c->TG;
+    TG.ntiles = T.ntiles; TG.tb = T.tb; TG.max_slots = T.max_slots; TG.max_faces = T.max_faces;
+#define UT(field, vec) do { if ((rc = c->upload(&TG.field, vec))) { delete c; return rc; } } while (0)
+    UT(vbeg, T.vbeg); UT(hbeg, T.hbeg); UT(halo, T.halo); UT(fbeg, T.fbeg); UT(bbeg, T.bbeg);
+    UT(tbf, T.tbf); UT(gptr, T.gptr); UT(cptr, T.cptr); UT(cgid, T.cgid); UT(tper, T.tper);
+    UT(pinfo, T.pinfo); UT(f_lr, T.f_lr); UT(cent, T.cent); UT(f_n, T.f_n); UT(gent, T.gent);
+#undef UT
+  }
   tsm::StageArgs& A = c->A;
   const int S = n_samples;
   A.cs = (size_t)S * nv;
@@ -211,11 +266,13 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   c->nblk_v = (nv + 255) / 256;
   const size_t cs = A.cs;
 #define AL(ptr, n) do { if ((rc = c->alloc(&(ptr), (n)))) { delete c; return rc; } } while (0)
-  AL(c->W[0], 4 * cs); AL(c->W[1], 4 * cs); AL(c->E[0], 4 * cs); AL(c->E[1], 4 * cs);
+  AL(c->W[0], 4 * cs); AL(c->W[1], 4 * cs); AL(c->W[2], 4 * cs);
+  AL(c->E[0], 4 * cs); AL(c->E[1], 4 * cs);
   AL(A.Uold, 4 * cs); AL(A.dE, 8 * cs); AL(A.F, 4 * A.fs); AL(A.R, 4 * cs);
   if (params->time_scheme == 1) AL(A.R2, 4 * cs);
   if (params->liou_fix) AL(A.ssw, cs);
-  AL(A.partial, (size_t)S * c->nblk_v * 4);
+  AL(A.partial, (size_t)S * std::max(c->nblk_v, c->tiling.ntiles) * 4);
+  if (c->tiled) AL(c->dtpart, (size_t)S * c->tiling.ntiles);
   AL(A.ctl, (size_t)S);
   double* dg; double* dr; double* ts;
   AL(dg, (size_t)S); AL(dr, (size_t)S);
@@ -268,9 +325,10 @@ extern "C" int tsm_set_state(tsm_ctx* c, int s, const double* prim) {
                                    std::to_string(i) + ")");
   std::vector<double> soa(4 * (size_t)nv);
   for (int i = 0; i < nv; ++i)
-    for (int k = 0; k < 4; ++k) soa[(size_t)k * nv + i] = prim[4 * i + k];
+    for (int k = 0; k < 4; ++k) soa[(size_t)k * nv + c->perm[i]] = prim[4 * i + k];
   const size_t cs = c->A.cs, off = (size_t)s * nv;
   HIP_TRY(hipStreamSynchronize(c->st));
+  c->csr_ready();
   for (int k = 0; k < 4; ++k)
     HIP_TRY(hipMemcpy(c->W[c->cur] + k * cs + off, soa.data() + (size_t)k * nv,
                       nv * sizeof(double), hipMemcpyHostToDevice));
@@ -294,7 +352,7 @@ int download_vec(tsm_ctx* c, const double* base, int ncomp, int s, double* out) 
     HIP_TRY(hipMemcpy(soa.data() + (size_t)k * nv, base + k * cs + off, nv * sizeof(double),
                       hipMemcpyDeviceToHost));
   for (int i = 0; i < nv; ++i)
-    for (int k = 0; k < ncomp; ++k) out[(size_t)ncomp * i + k] = soa[(size_t)k * nv + i];
+    for (int k = 0; k < ncomp; ++k) out[(size_t)ncomp * i + k] = soa[(size_t)k * nv + c->perm[i]];
   return TSM_OK;
 }
 }  // namespace
@@ -306,7 +364,7 @@ extern "C" int tsm_get_state(tsm_ctx* c, int s, double* prim) {
 
 extern "C" int tsm_gradients(tsm_ctx* c, int s, double* dE) {
   if (!c || !dE || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
-  c->bind_state();
+  c->csr_ready();
   c->A.ignore_done = 1;
   tsm::launch_gradient(c->G, c->A, c->S, c->st);
   c->A.ignore_done = 0;
@@ -316,7 +374,7 @@ extern "C" int tsm_gradients(tsm_ctx* c, int s, double* dE) {
 
 extern "C" int tsm_residual(tsm_ctx* c, int s, double* R) {
   if (!c || !R || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
-  c->bind_state();
+  c->csr_ready();
   c->A.ignore_done = 1;
   if (c->P.liou_fix) tsm::launch_ssw(c->G, c->A, c->S, c->st);
   tsm::launch_gradient(c->G, c->A, c->S, c->st);
@@ -341,7 +399,8 @@ extern "C" int tsm_compute_dt(tsm_ctx* c, int s, double* dt) {
 }
 
 namespace {
-// One full time step for every live sample (src/fv.cc:2277-2316).
+// One full time step for every live sample with the CSR-gather kernels
+// (src/fv.cc:2277-2316).
 void enqueue_step(tsm_ctx* c, int log_slot) {
   const int nrk = c->P.time_scheme == 0 ? 3 : 4;
   const int mode = c->P.time_scheme == 0 ? 1 : 2;
@@ -357,6 +416,45 @@ void enqueue_step(tsm_ctx* c, int log_slot) {
   }
   c->bind_state();
   tsm::launch_step_final(c->G, c->A, c->SP, c->S, log_slot, c->st);
+}
+
+// One SSP-RK3 step with the tiled kernels.  W[cur] holds W^n throughout; the
+// stages write the two other buffers and the last one writes W^{n+1} back
+// in place (each vertex reads its own W^n for U_old before overwriting it).
+void enqueue_tiled_step(tsm_ctx* c, int log_slot, hipEvent_t* ev = nullptr) {
+  double* Wn = c->W[c->cur];
+  double* Wa = c->W[(c->cur + 1) % 3];
+  double* Wb = c->W[(c->cur + 2) % 3];
+  const double* ins[3] = {Wn, Wa, Wb};
+  double* outs[3] = {Wa, Wb, Wn};
+  int e = 0;
+  for (int r = 0; r < 3; ++r) {
+    c->A.W_in = ins[r];
+    c->A.W_out = outs[r];
+    if (ev) (void)hipEventRecord(ev[e++], c->st);
+    tsm::launch_tile_gradient(c->G, c->TG, c->A, c->S, c->st);
+    if (ev) (void)hipEventRecord(ev[e++], c->st);
+    tsm::launch_tile_stage(c->P.flux, c->P.reconstruct, c->G, c->TG, c->A, c->SP, Wn, c->dtpart,
+                           c->S, r, c->st);
+  }
+  if (ev) (void)hipEventRecord(ev[e++], c->st);
+  tsm::launch_step_final_tiled(c->G, c->A, c->SP, c->S, c->TG.ntiles, c->dtpart, log_slot, c->st);
+  if (ev) (void)hipEventRecord(ev[e++], c->st);
+  c->bind_state();
+  c->e_valid = false;
+}
+
+// Steps of the current path; the tiled path first forms dt and the loop
+// test of the first step with the CSR time-step kernel.
+void enqueue_steps(tsm_ctx* c, long nsteps, bool with_log) {
+  if (c->tiled) {
+    c->bind_state();
+    if (nsteps > 0) tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
+    for (long k = 0; k < nsteps; ++k) enqueue_tiled_step(c, with_log ? (int)k : -1);
+  } else {
+    c->csr_ready();
+    for (long k = 0; k < nsteps; ++k) enqueue_step(c, with_log ? (int)k : -1);
+  }
 }
 }  // namespace
 
@@ -375,7 +473,7 @@ extern "C" int tsm_run(tsm_ctx* c, long nsteps, long* done, double* log) {
       c->A.log_cap = (int)nsteps;
     }
   }
-  for (long k = 0; k < nsteps; ++k) enqueue_step(c, log ? (int)k : -1);
+  enqueue_steps(c, nsteps, log != nullptr);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->st));
   HIP_TRY(hipMemcpy(after.data(), c->A.ctl, S * sizeof(tsm::Ctl), hipMemcpyDeviceToHost));
@@ -433,9 +531,12 @@ extern "C" int tsm_error_norms(tsm_ctx* c, int s, const double* exact, double* o
   const int nv = c->G.nv, nb = c->nblk_v;
   int rc = ensure_red(c, (size_t)nb * 24);
   if (rc) return rc;
+  std::vector<double> ex(4 * (size_t)nv);
+  for (int i = 0; i < nv; ++i)
+    for (int k = 0; k < 4; ++k) ex[4 * (size_t)c->perm[i] + k] = exact[4 * (size_t)i + k];
   double* dex = nullptr;
   HIP_TRY(hipMalloc(&dex, 4 * (size_t)nv * sizeof(double)));
-  hipError_t e = hipMemcpy(dex, exact, 4 * (size_t)nv * sizeof(double), hipMemcpyHostToDevice);
+  hipError_t e = hipMemcpy(dex, ex.data(), 4 * (size_t)nv * sizeof(double), hipMemcpyHostToDevice);
   if (e != hipSuccess) { (void)hipFree(dex); return fail(TSM_E_HIP, "upload exact"); }
   c->bind_state();
   tsm::launch_errnorm(c->G, c->A, s, dex, c->red, c->st);
@@ -481,53 +582,72 @@ extern "C" int tsm_geometry(tsm_ctx* c, int* fv, double* fn, double* dc, double*
 
 extern "C" int tsm_bench(tsm_ctx* c, int nsteps, double* us) {
   if (!c || nsteps < 1 || !us) return fail(TSM_E_ARG, "bad argument");
-  const int nrk = c->P.time_scheme == 0 ? 3 : 4;
-  const int mode = c->P.time_scheme == 0 ? 1 : 2;
   hipEvent_t ev[16];
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   double acc[4] = {0, 0, 0, 0};
   float ms = 0;
   HIP_TRY(hipStreamSynchronize(c->st));
-  for (int k = 0; k < nsteps; ++k) {
+  if (c->tiled) {
     c->bind_state();
-    HIP_TRY(hipEventRecord(ev[0], c->st));
     tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
-    HIP_TRY(hipEventRecord(ev[1], c->st));
-    if (c->P.liou_fix) tsm::launch_ssw(c->G, c->A, c->S, c->st);
-    int e = 2;
-    for (int r = 0; r < nrk; ++r) {
+    for (int k = 0; k < nsteps; ++k) {
+      enqueue_tiled_step(c, -1, ev);
+      HIP_TRY(hipEventSynchronize(ev[7]));
+      for (int r = 0; r < 3; ++r) {
+        HIP_TRY(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1])); acc[1] += ms;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[2 * r + 1], ev[2 * r + 2])); acc[2] += ms;
+      }
+      HIP_TRY(hipEventElapsedTime(&ms, ev[6], ev[7])); acc[3] += ms;
+    }
+    us[0] = 0.0;
+    us[1] = 1e3 * acc[1] / (3.0 * nsteps);
+    us[2] = 1e3 * acc[2] / (3.0 * nsteps);
+    us[3] = 1e3 * acc[3] / nsteps;
+  } else {
+    c->csr_ready();
+    const int nrk = c->P.time_scheme == 0 ? 3 : 4;
+    const int mode = c->P.time_scheme == 0 ? 1 : 2;
+    for (int k = 0; k < nsteps; ++k) {
       c->bind_state();
+      HIP_TRY(hipEventRecord(ev[0], c->st));
+      tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
+      HIP_TRY(hipEventRecord(ev[1], c->st));
+      if (c->P.liou_fix) tsm::launch_ssw(c->G, c->A, c->S, c->st);
+      int e = 2;
+      for (int r = 0; r < nrk; ++r) {
+        c->bind_state();
+        HIP_TRY(hipEventRecord(ev[e++], c->st));
+        tsm::launch_gradient(c->G, c->A, c->S, c->st);
+        HIP_TRY(hipEventRecord(ev[e++], c->st));
+        tsm::launch_flux_update(c->P.flux, c->P.reconstruct, c->G, c->A, c->S, r, r == nrk - 1,
+                                mode, c->st, true, false);
+        HIP_TRY(hipEventRecord(ev[e++], c->st));
+        tsm::launch_flux_update(c->P.flux, c->P.reconstruct, c->G, c->A, c->S, r, r == nrk - 1,
+                                mode, c->st, false, true);
+        c->cur ^= 1;
+      }
+      c->bind_state();
+      tsm::launch_step_final(c->G, c->A, c->SP, c->S, -1, c->st);
       HIP_TRY(hipEventRecord(ev[e++], c->st));
-      tsm::launch_gradient(c->G, c->A, c->S, c->st);
-      HIP_TRY(hipEventRecord(ev[e++], c->st));
-      tsm::launch_flux_update(c->P.flux, c->P.reconstruct, c->G, c->A, c->S, r, r == nrk - 1,
-                              mode, c->st, true, false);
-      HIP_TRY(hipEventRecord(ev[e++], c->st));
-      tsm::launch_flux_update(c->P.flux, c->P.reconstruct, c->G, c->A, c->S, r, r == nrk - 1,
-                              mode, c->st, false, true);
-      c->cur ^= 1;
+      HIP_TRY(hipEventSynchronize(ev[e - 1]));
+      HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); acc[0] += ms;
+      for (int r = 0; r < nrk; ++r) {
+        HIP_TRY(hipEventElapsedTime(&ms, ev[2 + 3 * r], ev[3 + 3 * r])); acc[1] += ms;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[3 + 3 * r], ev[4 + 3 * r])); acc[2] += ms;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[4 + 3 * r], ev[(r == nrk - 1) ? e - 1 : 5 + 3 * r])); acc[3] += ms;
+      }
     }
-    c->bind_state();
-    tsm::launch_step_final(c->G, c->A, c->SP, c->S, -1, c->st);
-    HIP_TRY(hipEventRecord(ev[e++], c->st));
-    HIP_TRY(hipEventSynchronize(ev[e - 1]));
-    HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1])); acc[0] += ms;
-    for (int r = 0; r < nrk; ++r) {
-      HIP_TRY(hipEventElapsedTime(&ms, ev[2 + 3 * r], ev[3 + 3 * r])); acc[1] += ms;
-      HIP_TRY(hipEventElapsedTime(&ms, ev[3 + 3 * r], ev[4 + 3 * r])); acc[2] += ms;
-      HIP_TRY(hipEventElapsedTime(&ms, ev[4 + 3 * r], ev[(r == nrk - 1) ? e - 1 : 5 + 3 * r])); acc[3] += ms;
-    }
+    us[0] = 1e3 * acc[0] / nsteps;
+    us[1] = 1e3 * acc[1] / (nsteps * (double)nrk);
+    us[2] = 1e3 * acc[2] / (nsteps * (double)nrk);
+    us[3] = 1e3 * acc[3] / (nsteps * (double)nrk);
   }
   // whole steps without interleaved events
-  HIP_TRY(hipEventRecord(ev[0], c->st));
-  for (int k = 0; k < nsteps; ++k) enqueue_step(c, -1);
-  HIP_TRY(hipEventRecord(ev[1], c->st));
-  HIP_TRY(hipEventSynchronize(ev[1]));
-  HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
-  us[0] = 1e3 * acc[0] / nsteps;
-  us[1] = 1e3 * acc[1] / (nsteps * (double)nrk);
-  us[2] = 1e3 * acc[2] / (nsteps * (double)nrk);
-  us[3] = 1e3 * acc[3] / (nsteps * (double)nrk);
+  HIP_TRY(hipEventRecord(ev[14], c->st));
+  enqueue_steps(c, nsteps, false);
+  HIP_TRY(hipEventRecord(ev[15], c->st));
+  HIP_TRY(hipEventSynchronize(ev[15]));
+  HIP_TRY(hipEventElapsedTime(&ms, ev[14], ev[15]));
   us[4] = 1e3 * ms / nsteps;
   for (auto& e : ev) (void)hipEventDestroy(e);
   return TSM_OK;

@@ -40,6 +40,15 @@ struct DevGeo {
   const int* pv;        // members
 };
 
+// Tile arrays of the LDS-staged kernels (device pointers, new vertex ids).
+struct TileGeo {
+  int ntiles, tb, max_slots, max_faces;
+  const int *vbeg, *hbeg, *halo, *fbeg, *bbeg, *tbf, *gptr, *cptr, *cgid, *tper, *pinfo;
+  const unsigned *f_lr, *cent;
+  const double* f_n;
+  const unsigned short* gent;
+};
+
 struct ViscParams {
   double mu_ref, T_ref, T_0, omega, prandtl;
   int mu_model, visc_active, heat_active;
@@ -75,6 +84,17 @@ struct StageArgs {
   int ns_samples;  // number of samples
   int ignore_done; // single-call API entry points compute regardless of the loop state
 };
+
+// LDS-staged tile kernels (Euler, SSP-RK3)
+size_t tile_grad_lds(const TileGeo& T);
+size_t tile_fu_lds(const TileGeo& T, int flux, int rec);
+void launch_tile_gradient(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
+                          hipStream_t st);
+void launch_tile_stage(int flux, int rec, const DevGeo& G, const TileGeo& T, const StageArgs& A,
+                       const StepParams& P, const double* Wn, double* dtpart, int S, int r,
+                       hipStream_t st);
+void launch_step_final_tiled(const DevGeo& G, const StageArgs& A, const StepParams& P, int S,
+                             int nblk, const double* dtpart, int log_slot, hipStream_t st);
 
 void launch_flux_update(int flux, int rec, const DevGeo& G, const StageArgs& A, int S, int r,
                         int last, int mode, hipStream_t st, bool do_flux, bool do_update);

@@ -45,6 +45,29 @@ struct Geometry {
   std::vector<double> weight;              // 1/|P| for periodic vertices else 1
 };
 
+// Vertex tiling for the LDS-staged kernels (see tiles.cpp).  Vertex ids in
+// every array are the new (tile-contiguous) ids.
+struct Tiling {
+  int tb = 0, ntiles = 0, n_periodic_tiles = 0, max_slots = 0, max_faces = 0;
+  std::vector<int> perm, inv;        // old -> new, new -> old
+  std::vector<int> vbeg;             // ntiles+1: owned range
+  std::vector<int> hbeg, halo;       // halo slots
+  std::vector<int> fbeg;             // ntiles+1: local faces
+  std::vector<uint32_t> f_lr;        // left slot | right slot << 16
+  std::vector<double> f_n;           // 2 per local face
+  std::vector<int> bbeg, tbf;        // tile boundary faces (global bface ids)
+  std::vector<int> gptr;             // nv+1
+  std::vector<uint16_t> gent;        // bit15 right end, bit14 bface, low 14 bits local index
+  std::vector<int> cptr;             // nv+1
+  std::vector<uint32_t> cent;        // 3 x 10-bit slots | correction flag << 30
+  std::vector<int> cgid;             // global cell id per cell entry
+  std::vector<int> tper;             // tile holds periodic sets
+  std::vector<int> pinfo;            // per new vertex: first member slot << 8 | set size, or -1
+};
+
+bool build_tiles(const Geometry& g, int tb, Tiling& T, std::string& msg);
+bool validate_tiles(const Geometry& g, const Tiling& T, std::string& msg);
+
 // Builds faces, normals and areas exactly as Grid::preproc does for one
 // partition (src/grid_preproc.cc:1147-1194).  Returns false with msg set.
 bool build_geometry(const tsm_mesh& m, int cell_type, Geometry& g, std::string& msg);

@@ -238,7 +238,8 @@ __global__ __launch_bounds__(256) void k_flux(DevGeo G, StageArgs A) {
     double rx = G.x[r] - G.x[l], ry = G.y[r] - G.y[l];
     double ssw = 0.0;
     if (FLUX == F_ROE && A.ssw) ssw = A.ssw[off + l] + A.ssw[off + r];
-    F = tecno_flux<FLUX == F_ROE, REC>(gas, A.fo, rx, ry, Wl, Wr, gl, gr, unx, uny, ssw);
+    Side Ls = make_side<true, FLUX == F_ROE>(gas, Wl), Rs = make_side<true, FLUX == F_ROE>(gas, Wr);
+    F = tecno_flux<FLUX == F_ROE, REC>(gas, A.fo, rx, ry, Ls, Rs, gl, gr, unx, uny, ssw);
   } else if (FLUX == F_KEPEC) {
     F = kepec_flux(gas, Wl, Wr, unx, uny);
   } else if (FLUX == F_KEP) {
@@ -270,7 +271,8 @@ __device__ __forceinline__ Cons bface_flux(const DevGeo& G, const StageArgs& A, 
     Cons F;
     if (FLUX == F_ROE || FLUX == F_RUS) {
       const double z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      F = tecno_flux<FLUX == F_ROE, REC>(gas, A.fo, 0.0 * xm, 0.0 * ym, s0, s1, z, z, unx, uny, 0.0);
+      Side L0 = make_side<true, FLUX == F_ROE>(gas, s0), L1 = make_side<true, FLUX == F_ROE>(gas, s1);
+      F = tecno_flux<FLUX == F_ROE, REC>(gas, A.fo, 0.0 * xm, 0.0 * ym, L0, L1, z, z, unx, uny, 0.0);
     } else if (FLUX == F_KEPEC) {
       F = kepec_flux(gas, s0, s1, unx, uny);
     } else if (FLUX == F_KEP) {
@@ -564,7 +566,8 @@ __global__ __launch_bounds__(256) void k_dt_final(StageArgs A, StepParams P, int
   x = block_min(x, lds);
   if (threadIdx.x == 0) {
     double dt = x;
-    if (P.unsteady) {
+    // a fixed time_step returns before the time-stamp clamp (src/fv.cc:1503-1509)
+    if (P.unsteady && !(P.time_step > 0.0)) {
       // the reference indexes past its stamp table if t falls an ulp short of
       // final_time (src/fv.cc:1610); the last stamp is used instead
       double target = A.tstamps[min(c.counter, P.n_stamps)];

@@ -142,6 +142,23 @@ struct FluxOpts {
   int nd_active;
 };
 
+// Per-vertex quantities the two-point fluxes use, computed once per vertex with
+// the reference's own expressions (Density, 0.5/(R T), Entropy, sound_speed).
+struct Side {
+  Prim w;
+  double rho, beta, s, c;
+};
+template <bool NEED_S, bool NEED_C>
+__device__ __forceinline__ Side make_side(const Gas& g, const Prim& w) {
+  Side q;
+  q.w = w;
+  q.rho = density(g, w);
+  q.beta = 0.5 / (g.R * w.T);
+  q.s = NEED_S ? (log(w.p) - g.gamma * log(q.rho)) : 0.0;
+  q.c = NEED_C ? sound_speed(g, w) : 0.0;
+  return q;
+}
+
 // src/kepec.cc:10-38 (per unit normal)
 __device__ __forceinline__ Cons kepec_flux(const Gas& g, const Prim& l, const Prim& r,
                                            double unx, double uny) {
@@ -152,6 +169,25 @@ __device__ __forceinline__ Cons kepec_flux(const Gas& g, const Prim& l, const Pr
   double betar = 0.5 / (g.R * r.T);
   double beta = logavg(betal, betar);
   double p = 0.5 * (rhol + rhor) / (betal + betar);
+  double vx = l.u * 0.5 + r.u * 0.5, vy = l.v * 0.5 + r.v * 0.5;
+  double vn = vx * unx + vy * uny;
+  Cons F;
+  F.m = rho * vn;
+  F.mx = unx * p + vx * F.m;
+  F.my = uny * p + vy * F.m;
+  F.e = 0.5 * (1.0 / ((g.gamma - 1.0) * beta) - vel2) * F.m + (F.mx * vx + F.my * vy);
+  return F;
+}
+
+// kepec_flux with the per-vertex density and beta already formed (same values)
+__device__ __forceinline__ Cons kepec_flux_s(const Gas& g, const Side& L, const Side& Rr,
+                                             double unx, double uny) {
+  const Prim& l = L.w;
+  const Prim& r = Rr.w;
+  double rho = logavg(L.rho, Rr.rho);
+  double vel2 = 0.5 * ((l.u * l.u + l.v * l.v) + (r.u * r.u + r.v * r.v));
+  double beta = logavg(L.beta, Rr.beta);
+  double p = 0.5 * (L.rho + Rr.rho) / (L.beta + Rr.beta);
   double vx = l.u * 0.5 + r.u * 0.5, vy = l.v * 0.5 + r.v * 0.5;
   double vn = vx * unx + vy * uny;
   Cons F;
@@ -227,14 +263,15 @@ __device__ __forceinline__ Cons avg_flux(const Gas& g, const Prim& l, const Prim
 // gl/gr: left/right vertex gradients of (f, vx, vy, l), x then y.
 template <bool ROE, int REC>
 __device__ __forceinline__ Cons tecno_flux(const Gas& g, const FluxOpts& o, double rx, double ry,
-                                           const Prim& l, const Prim& r, const double* gl,
+                                           const Side& L, const Side& Rr, const double* gl,
                                            const double* gr, double unx, double uny,
                                            double ssw) {
-  const double gam = g.gamma, R = g.R;
-  double rhol = density(g, l), rhor = density(g, r);
+  const double gam = g.gamma;
+  const Prim& l = L.w;
+  const Prim& r = Rr.w;
+  double rhol = L.rho, rhor = Rr.rho;
   double rho = logavg(rhol, rhor);
-  double betal = 0.5 / (R * l.T);
-  double betar = 0.5 / (R * r.T);
+  double betal = L.beta, betar = Rr.beta;
   double beta = logavg(betal, betar);
   double a = sqrt(0.5 * gam / beta);
   double p = 0.5 * (rhol + rhor) / (betal + betar);
@@ -267,7 +304,7 @@ __device__ __forceinline__ Cons tecno_flux(const Gas& g, const FluxOpts& o, doub
   double Lam[4];
   if (ROE) {
     double vnl = l.u * unx + l.v * uny, vnr = r.u * unx + r.v * uny;
-    double al = sound_speed(g, l), ar = sound_speed(g, r);
+    double al = L.c, ar = Rr.c;
     double l2 = fabs(vn), l3 = fabs(vn);
     if (ssw > 0) l3 = fmax(l3, a);
     Lam[0] = (1 + o.ev_beta) * fabs(vn - a) + o.ev_alpha * fabs((vnl - al) - (vnr - ar));
@@ -278,13 +315,21 @@ __device__ __forceinline__ Cons tecno_flux(const Gas& g, const FluxOpts& o, doub
     double lr = fabs(vn) + fabs(a);
     Lam[0] = Lam[1] = Lam[2] = Lam[3] = lr;
   }
-  double sl = entropy(g, l), sr = entropy(g, r);
+  double sl = L.s, sr = Rr.s;
   double Vl[4] = {(gam - sl) / (gam - 1.0) - betal * (l.u * l.u + l.v * l.v), 2 * l.u * betal,
                   2 * l.v * betal, -2 * betal};
   double Vr[4] = {(gam - sr) / (gam - 1.0) - betar * (r.u * r.u + r.v * r.v), 2 * r.u * betar,
                   2 * r.v * betar, -2 * betar};
   double h = 0.0;
   if (REC == R_TVB) h = sqrt(rx * rx + ry * ry);
+  double gdl[4], gdr[4];
+  if (REC != R_FIRST) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      gdl[j] = gl[2 * j] * rx + gl[2 * j + 1] * ry;
+      gdr[j] = gr[2 * j] * rx + gr[2 * j + 1] * ry;
+    }
+  }
   double dZ[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -297,10 +342,8 @@ __device__ __forceinline__ Cons tecno_flux(const Gas& g, const FluxOpts& o, doub
     if (REC != R_FIRST) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        double gdl = gl[2 * j] * rx + gl[2 * j + 1] * ry;
-        double gdr = gr[2 * j] * rx + gr[2 * j + 1] * ry;
-        sgl += Rm[j][i] * gdl;
-        sgr += Rm[j][i] * gdr;
+        sgl += Rm[j][i] * gdl[j];
+        sgr += Rm[j][i] * gdr[j];
       }
     }
     double zij, zji;

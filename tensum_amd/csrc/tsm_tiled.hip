@@ -1,0 +1,573 @@
+// tsm_tiled.hip -- LDS-staged stage kernels (one workgroup per vertex tile).
+//
+// Per SSP-RK3 stage two kernels run over all tiles (grid = tiles x samples):
+//   k_grad_tile: W of the tile's owned + halo slots -> entropy variables in
+//     LDS (prim2ent once per slot) -> Green-Gauss vertex gradients of the
+//     owned vertices -> dE (src/fv.cc:699-1059);
+//   k_fu_tile: W, dE, coordinates of the slots -> per-slot density, beta,
+//     entropy (and sound speed) in LDS -> every local face flux into LDS ->
+//     per owned vertex gather in reference face order, periodic set sums
+//     through LDS, boundary fluxes, SSP-RK3 update (src/fv.cc:1066-1192,
+//     :1635-1707).  The last stage also forms the residual-norm partials
+//     (src/fv.cc:1801-1879) and the next step's local time step
+//     (src/fv.cc:1499-1621) from the new state, so a step needs no separate
+//     time-step pass.
+// Every sum is taken in the same order as in the reference; no atomics.
+#include <hip/hip_runtime.h>
+
+#include "tsm_device.h"
+#include "tsm_physics.h"
+
+namespace tsm {
+
+namespace {
+
+__device__ __forceinline__ Prim ld_prim(const double* W, size_t cs, size_t i) {
+  return Prim{W[i], W[cs + i], W[2 * cs + i], W[3 * cs + i]};
+}
+
+template <int NV>
+__device__ __forceinline__ void bsum(double* v, double* red) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    v[k] = x;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[wid * NV + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = red[k];
+      for (int w = 1; w < nw; ++w) s += red[w * NV + k];
+      v[k] = s;
+    }
+  }
+}
+__device__ __forceinline__ double bmin(double x, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_down(x, o, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int w = 1; w < nw; ++w) x = fmin(x, red[w]);
+  }
+  return x;
+}
+
+// One-state boundary condition (src/bc.h:615-654), used by the gradient's
+// boundary correction.
+__device__ __forceinline__ void bc1(int type, const double* bs, double nx, double ny, Prim& st) {
+  switch (type) {
+    case TSM_BC_SLIP: {
+      double im = 1.0 / sqrt(nx * nx + ny * ny);
+      double ux = nx * im, uy = ny * im;
+      double fact = st.u * ux + st.v * uy;
+      st.u += ux * -fact;
+      st.v += uy * -fact;
+      break;
+    }
+    case TSM_BC_NOSLIP: st.u = bs[1]; st.v = bs[2]; st.T = bs[0]; break;
+    case TSM_BC_PRESSURE: st.p = bs[3]; break;
+    case TSM_BC_INLET: st = Prim{bs[0], bs[1], bs[2], bs[3]}; break;
+    default: break;
+  }
+}
+// Two-state boundary condition (src/bc.h:569-610).
+__device__ __forceinline__ void bc2(int type, const double* bs, double nx, double ny, Prim& s0,
+                                    Prim& s1) {
+  switch (type) {
+    case TSM_BC_SLIP: {
+      double im = 1.0 / sqrt(nx * nx + ny * ny);
+      double ux = nx * im, uy = ny * im;
+      double fact = s0.u * ux + s0.v * uy;
+      s0.u += ux * -fact;
+      s0.v += uy * -fact;
+      s1 = s0;
+      break;
+    }
+    case TSM_BC_NOSLIP:
+      s0.u = bs[1]; s0.v = bs[2];
+      s1.u = s0.u; s1.v = s0.v; s1.p = s0.p;
+      s0.T = bs[0]; s1.T = bs[0];
+      break;
+    case TSM_BC_PRESSURE: s1.p = bs[3]; break;
+    case TSM_BC_INLET: s0 = Prim{bs[0], bs[1], bs[2], bs[3]}; s1 = s0; break;
+    case TSM_BC_OUTLET: s1 = s0; break;
+    case TSM_BC_FARFIELD: s1 = Prim{bs[0], bs[1], bs[2], bs[3]}; break;
+    default: break;
+  }
+}
+
+template <int FLUX, int REC>
+__device__ __forceinline__ Cons pair_flux(const Gas& gas, const FluxOpts& fo, double rx, double ry,
+                                          const Side& L, const Side& R, const double* gl,
+                                          const double* gr, double unx, double uny) {
+  if (FLUX == F_ROE || FLUX == F_RUS)
+    return tecno_flux<FLUX == F_ROE, REC>(gas, fo, rx, ry, L, R, gl, gr, unx, uny, 0.0);
+  if (FLUX == F_KEPEC) return kepec_flux_s(gas, L, R, unx, uny);
+  if (FLUX == F_KEP) return kep_flux(gas, L.w, R.w, unx, uny);
+  if (FLUX == F_ROEEC) return roe_ec_flux(gas, L.w, R.w, unx, uny);
+  return avg_flux(gas, L.w, R.w, unx, uny);
+}
+
+// Boundary flux of bface b at a vertex with state wm (src/fv.cc:1100-1189)
+template <int FLUX, int REC>
+__device__ __forceinline__ Cons bflux(const DevGeo& G, const FluxOpts& fo, const Gas& gas, int b,
+                                      const Prim& wm) {
+  int type = G.btype[b];
+  double nx = G.bn[2 * b], ny = G.bn[2 * b + 1];
+  Prim s0 = wm, s1 = wm;
+  bc2(type, G.bstate + 4 * b, nx, ny, s0, s1);
+  if (type == TSM_BC_FARFIELD) {
+    double area = sqrt(nx * nx + ny * ny);
+    double ia = 1.0 / area;
+    const double z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr bool TEC = FLUX == F_ROE || FLUX == F_RUS;
+    Side L = make_side<TEC, FLUX == F_ROE>(gas, s0), R = make_side<TEC, FLUX == F_ROE>(gas, s1);
+    Cons F = pair_flux<FLUX, REC>(gas, fo, 0.0, 0.0, L, R, z, z, nx * ia, ny * ia);
+    F.m *= area; F.mx *= area; F.my *= area; F.e *= area;
+    return F;
+  }
+  return inviscid_bflux(gas, s0, s1, nx, ny);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// gradient kernel
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grad_tile(DevGeo G, TileGeo TG, StageArgs A) {
+  extern __shared__ double lds[];
+  const int t = blockIdx.x, s = blockIdx.y;
+  if (A.ctl[s].done) return;
+  const int MS = TG.max_slots, TB = TG.tb;
+  double* SE = lds;            // 4 x MS entropy variables
+  double* SX = lds + 4 * MS;   // 2 x MS coordinates
+  double* SP = lds + 6 * MS;   // 8 x TB periodic exchange
+  const int b0 = TG.vbeg[t], nown = TG.vbeg[t + 1] - b0;
+  const int h0 = TG.hbeg[t], ns = nown + TG.hbeg[t + 1] - h0;
+  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  const Gas gas{A.gamma[s], A.gasR[s]};
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    const int gid = (i < nown) ? b0 + i : TG.halo[h0 + i - nown];
+    const Ent e = prim2ent(gas, ld_prim(A.W_in, cs, off + gid));
+    SE[i] = e.f; SE[MS + i] = e.vx; SE[2 * MS + i] = e.vy; SE[3 * MS + i] = e.l;
+    SX[i] = G.x[gid]; SX[MS + i] = G.y[gid];
+  }
+  __syncthreads();
+  const int tid = threadIdx.x;
+  const bool own = tid < nown;
+  const int v = b0 + tid;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (own) {
+    for (int k = TG.cptr[v]; k < TG.cptr[v + 1]; ++k) {
+      const unsigned e = TG.cent[k];
+      const int a = e & 1023, b = (e >> 10) & 1023, d = (e >> 20) & 1023;
+      const double xa = SX[a], ya = SX[MS + a], xb = SX[b], yb = SX[MS + b], xd = SX[d], yd = SX[MS + d];
+      // cell normals, src/grid_preproc.cc:336-362: N0 = dr12, N1 = dr20, N2 = dr01
+      const double n0x = (yb - yd), n0y = -(xb - xd);
+      const double n1x = (yd - ya), n1y = -(xd - xa);
+      const double n2x = (ya - yb), n2y = -(xa - xb);
+      double g[8];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double ea = SE[q * MS + a], eb = SE[q * MS + b], ed = SE[q * MS + d];
+        g[2 * q] = n0x * ea + n1x * eb + n2x * ed;
+        g[2 * q + 1] = n0y * ea + n1y * eb + n2y * ed;
+      }
+      if (e >> 30) {  // boundary correction, src/fv.cc:926-966
+        const int c = TG.cgid[k];
+        for (int z = 0; z < 2; ++z) {
+          const int bf = G.cbf[2 * c + z];
+          if (bf < 0) break;
+          const int v0 = G.bv[2 * bf], v1 = G.bv[2 * bf + 1];
+          const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
+          const int type = G.btype[bf];
+          const Prim w0 = ld_prim(A.W_in, cs, off + v0), w1 = ld_prim(A.W_in, cs, off + v1);
+          const Ent E0 = prim2ent(gas, w0), E1 = prim2ent(gas, w1);
+          Ent q0 = E0, q1 = E1;
+          if (type == TSM_BC_SLIP || type == TSM_BC_NOSLIP || type == TSM_BC_PRESSURE ||
+              type == TSM_BC_INLET) {
+            Prim s0 = w0, s1 = w1;
+            bc1(type, G.bstate + 4 * bf, nx, ny, s0);
+            bc1(type, G.bstate + 4 * bf, nx, ny, s1);
+            q0 = prim2ent(gas, s0);
+            q1 = prim2ent(gas, s1);
+          }
+          const double dq[4] = {(q0.f + q1.f - E0.f - E1.f) / 2.0,
+                                (q0.vx + q1.vx - E0.vx - E1.vx) / 2.0,
+                                (q0.vy + q1.vy - E0.vy - E1.vy) / 2.0,
+                                (q0.l + q1.l - E0.l - E1.l) / 2.0};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            g[2 * q] += nx * dq[q];
+            g[2 * q + 1] += ny * dq[q];
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc[q] += g[q];
+    }
+  }
+  if (TG.tper[t]) {  // periodic sets of this tile: sum members in set order
+    if (own)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) SP[q * TB + tid] = acc[q];
+    __syncthreads();
+    if (own) {
+      const int pi = TG.pinfo[v];
+      if (pi >= 0) {
+        const int first = pi >> 8, cnt = pi & 255;
+        double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int m = 0; m < cnt; ++m)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) tot[q] += SP[q * TB + first + m] * 1.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] = tot[q];
+      }
+    }
+  }
+  if (own) {
+    const double fact = G.gfact[v];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) A.dE[(size_t)q * cs + off + v] = acc[q] * fact;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// flux + gather + update kernel
+// ---------------------------------------------------------------------------
+template <int FLUX, int REC>
+struct SlotLayout {
+  static constexpr bool TEC = FLUX == F_ROE || FLUX == F_RUS;
+  static constexpr bool GRAD = TEC && REC != R_FIRST;
+  static constexpr int QS = 6, QC = 7, QD = 6 + (TEC ? 1 : 0) + (FLUX == F_ROE ? 1 : 0);
+  static constexpr int QX = QD + 8;
+  static constexpr int NQ = QD + (GRAD ? 10 : 0);
+};
+
+template <int FLUX, int REC, int LAST>
+__global__ __launch_bounds__(256) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs A, StepParams P,
+                                                 const double* Wn, double* dtpart, int r) {
+  using L = SlotLayout<FLUX, REC>;
+  extern __shared__ double lds[];
+  const int t = blockIdx.x, s = blockIdx.y;
+  if (A.ctl[s].done) return;
+  const int MS = TG.max_slots, MF = TG.max_faces, TB = TG.tb;
+  double* Q = lds;                  // NQ x MS slot data
+  double* SF = lds + L::NQ * MS;    // 4 x MF face fluxes
+  double* SR = SF + 4 * MF;         // 4 x TB periodic exchange
+  double* red = SR + 4 * TB;        // 16 reduction scratch
+  const int b0 = TG.vbeg[t], nown = TG.vbeg[t + 1] - b0;
+  const int h0 = TG.hbeg[t], ns = nown + TG.hbeg[t + 1] - h0;
+  const int f0 = TG.fbeg[t], nfl = TG.fbeg[t + 1] - f0;
+  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  const Gas gas{A.gamma[s], A.gasR[s]};
+
+  // ---- phase 1: slot quantities ------------------------------------------
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    const int gid = (i < nown) ? b0 + i : TG.halo[h0 + i - nown];
+    const Prim w = ld_prim(A.W_in, cs, off + gid);
+    const Side q = make_side<L::TEC, FLUX == F_ROE>(gas, w);
+    Q[i] = w.T; Q[MS + i] = w.u; Q[2 * MS + i] = w.v; Q[3 * MS + i] = w.p;
+    Q[4 * MS + i] = q.rho; Q[5 * MS + i] = q.beta;
+    if (L::TEC) Q[L::QS * MS + i] = q.s;
+    if (FLUX == F_ROE) Q[L::QC * MS + i] = q.c;
+    if (L::GRAD) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) Q[(L::QD + k) * MS + i] = A.dE[(size_t)k * cs + off + gid];
+      Q[L::QX * MS + i] = G.x[gid];
+      Q[(L::QX + 1) * MS + i] = G.y[gid];
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2: local face fluxes ----------------------------------------
+  for (int j = threadIdx.x; j < nfl; j += blockDim.x) {
+    const unsigned lr = TG.f_lr[f0 + j];
+    const int l = lr & 0xffff, rr = lr >> 16;
+    const double nx = TG.f_n[2 * (size_t)(f0 + j)], ny = TG.f_n[2 * (size_t)(f0 + j) + 1];
+    const double area = sqrt(nx * nx + ny * ny);
+    const double ia = 1.0 / area;
+    Side Ls, Rs;
+    Ls.w = Prim{Q[l], Q[MS + l], Q[2 * MS + l], Q[3 * MS + l]};
+    Rs.w = Prim{Q[rr], Q[MS + rr], Q[2 * MS + rr], Q[3 * MS + rr]};
+    Ls.rho = Q[4 * MS + l]; Rs.rho = Q[4 * MS + rr];
+    Ls.beta = Q[5 * MS + l]; Rs.beta = Q[5 * MS + rr];
+    Ls.s = L::TEC ? Q[L::QS * MS + l] : 0.0;
+    Rs.s = L::TEC ? Q[L::QS * MS + rr] : 0.0;
+    Ls.c = (FLUX == F_ROE) ? Q[L::QC * MS + l] : 0.0;
+    Rs.c = (FLUX == F_ROE) ? Q[L::QC * MS + rr] : 0.0;
+    double gl[8], gr[8], rx = 0.0, ry = 0.0;
+    if (L::GRAD) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) { gl[k] = Q[(L::QD + k) * MS + l]; gr[k] = Q[(L::QD + k) * MS + rr]; }
+      rx = Q[L::QX * MS + rr] - Q[L::QX * MS + l];
+      ry = Q[(L::QX + 1) * MS + rr] - Q[(L::QX + 1) * MS + l];
+    }
+    const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gl, gr, nx * ia, ny * ia);
+    SF[j] = F.m * area;
+    SF[MF + j] = F.mx * area;
+    SF[2 * MF + j] = F.my * area;
+    SF[3 * MF + j] = F.e * area;
+  }
+  __syncthreads();
+
+  // ---- phase 3: gather, periodic sums, update ----------------------------
+  const int tid = threadIdx.x;
+  const bool own = tid < nown;
+  const int v = b0 + tid;
+  const int bb0 = TG.bbeg[t];
+  double acc[4] = {0, 0, 0, 0};
+  Prim w0{1, 0, 0, 1};
+  if (own) {
+    w0 = Prim{Q[tid], Q[MS + tid], Q[2 * MS + tid], Q[3 * MS + tid]};
+    for (int k = TG.gptr[v]; k < TG.gptr[v + 1]; ++k) {
+      const unsigned e = TG.gent[k];
+      if (e & 0x4000u) {
+        const int b = TG.tbf[bb0 + (e & 0x3fffu)];
+        const Cons Fb = bflux<FLUX, REC>(G, A.fo, gas, b, w0);
+        acc[0] += Fb.m * 0.5; acc[1] += Fb.mx * 0.5; acc[2] += Fb.my * 0.5; acc[3] += Fb.e * 0.5;
+      } else {
+        const int j = e & 0x3fffu;
+        const double q0 = SF[j], q1 = SF[MF + j], q2 = SF[2 * MF + j], q3 = SF[3 * MF + j];
+        if (e & 0x8000u) { acc[0] -= q0; acc[1] -= q1; acc[2] -= q2; acc[3] -= q3; }
+        else { acc[0] += q0; acc[1] += q1; acc[2] += q2; acc[3] += q3; }
+      }
+    }
+  }
+  const bool per = TG.tper[t] != 0;
+  if (per) {
+    if (own)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) SR[k * TB + tid] = acc[k];
+    __syncthreads();
+    if (own) {
+      const int pi = TG.pinfo[v];
+      if (pi >= 0) {
+        const int first = pi >> 8, cnt = pi & 255;
+        double tot[4] = {0, 0, 0, 0};
+        for (int m = 0; m < cnt; ++m)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) tot[k] += SR[k * TB + first + m] * 1.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[k] = tot[k];
+      }
+    }
+  }
+  double part[4] = {0, 0, 0, 0};
+  double lam = 0.0;
+  Prim w1{1, 0, 0, 1};
+  if (own) {
+    const double area = G.dcarea[v];
+    if (LAST) {
+      const double wt = G.weight[v];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double q = acc[k] / area;
+        part[k] = (q * q) * wt;
+      }
+    }
+    const double a_rk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
+    const double b_rk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+    const double factor = A.ctl[s].dt / (1.0 * area);
+    const double b = b_rk[r], a = a_rk[r], c = -factor * b_rk[r];
+    Cons U = prim2con(gas, w0);
+    U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
+    if (r == 0) {  // a = 0: U_old * 0 contributes an exact zero
+      U.m += 0.0 + acc[0] * c; U.mx += 0.0 + acc[1] * c;
+      U.my += 0.0 + acc[2] * c; U.e += 0.0 + acc[3] * c;
+    } else {
+      const Cons Uo = prim2con(gas, ld_prim(Wn, cs, off + v));
+      U.m += Uo.m * a + acc[0] * c; U.mx += Uo.mx * a + acc[1] * c;
+      U.my += Uo.my * a + acc[2] * c; U.e += Uo.e * a + acc[3] * c;
+    }
+    w1 = con2prim(gas, U);
+    double* Wo = A.W_out + off + v;
+    Wo[0] = w1.T; Wo[cs] = w1.u; Wo[2 * cs] = w1.v; Wo[3 * cs] = w1.p;
+    if (LAST) {  // spectral radius for the next step's dt, src/fv.cc:1511-1553
+      const double cm = sound_speed(gas, w1);
+      for (int k = TG.gptr[v]; k < TG.gptr[v + 1]; ++k) {
+        const unsigned e = TG.gent[k];
+        if (e & 0x4000u) {
+          const int bf = TG.tbf[bb0 + (e & 0x3fffu)];
+          const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
+          const double ar = sqrt(nx * nx + ny * ny);
+          const double vn = w1.u * nx + w1.v * ny;
+          lam += 0.5 * (fabs(vn) + cm * ar);
+        } else {
+          const size_t j = (size_t)f0 + (e & 0x3fffu);
+          const double nx = TG.f_n[2 * j], ny = TG.f_n[2 * j + 1];
+          const double ar = sqrt(nx * nx + ny * ny);
+          const double vn = w1.u * nx + w1.v * ny;
+          lam += (fabs(vn) + cm * ar) * 1.0;
+        }
+      }
+    }
+  }
+  if (LAST) {
+    if (per) {
+      __syncthreads();  // SR reads of the residual exchange are complete
+      if (own) SR[tid] = lam;
+      __syncthreads();
+      if (own) {
+        const int pi = TG.pinfo[v];
+        if (pi >= 0) {
+          const int first = pi >> 8, cnt = pi & 255;
+          double tot = 0.0;
+          for (int m = 0; m < cnt; ++m) tot = tot + SR[first + m] * 1.0;
+          lam = tot;
+        }
+      }
+    }
+    double dtv = 1.0e20;
+    if (own) dtv = (P.time_step > 0.0) ? P.time_step : P.cfl * G.dcarea[v] / lam;
+    bsum<4>(part, red);
+    if (threadIdx.x == 0) {
+      double* pp = A.partial + ((size_t)s * gridDim.x + t) * 4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) pp[k] = part[k];
+    }
+    __syncthreads();
+    const double m = bmin(dtv, red);
+    if (threadIdx.x == 0) dtpart[(size_t)s * gridDim.x + t] = m;
+  }
+}
+
+// Residual norm, time advance, loop test and the next step's global dt
+// (src/fv.cc:1591-1618, :1801-1879, :2273-2316).
+__global__ __launch_bounds__(256) void k_step_final_tiled(StageArgs A, StepParams P, int nblk,
+                                                          int nv, const double* dtpart,
+                                                          int log_slot) {
+  __shared__ double red[16];
+  const int s = blockIdx.x;
+  Ctl& c = A.ctl[s];
+  if (c.done) return;
+  double part[4] = {0, 0, 0, 0};
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+    const double* q = A.partial + ((size_t)s * nblk + i) * 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) part[k] += q[k];
+  }
+  bsum<4>(part, red);
+  __syncthreads();
+  double x = 1.0e20;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) x = fmin(x, dtpart[(size_t)s * nblk + i]);
+  x = bmin(x, red);
+  if (threadIdx.x == 0) {
+    double n[4];
+    for (int k = 0; k < 4; ++k) n[k] = sqrt(part[k]) / nv;
+    const double tot = sqrt((n[0] * n[0]) + (n[1] * n[1] + n[2] * n[2]) + (n[3] * n[3]));
+    if (c.iter == 0) {
+      c.res0 = tot;
+      if (c.res0 == 0.0) c.res0 = 1.0;
+    }
+    c.res_total = tot / c.res0;
+    c.iter += 1;
+    c.t += c.dt;
+    if (c.reached && P.unsteady) { c.counter += 1; c.reached = 0; }
+    if (A.log && log_slot >= 0) {
+      double* Lg = A.log + ((size_t)s * A.log_cap + log_slot) * 4;
+      Lg[0] = (double)c.iter; Lg[1] = c.dt; Lg[2] = c.t; Lg[3] = c.res_total;
+    }
+    const bool go = c.res_total > P.min_residue && (P.unsteady ? true : c.iter < P.max_iter) &&
+                    c.t < P.final_time;
+    if (!go) {
+      c.done = 1;
+    } else {
+      double dt = x;
+      if (P.unsteady && !(P.time_step > 0.0)) {
+        const double target = A.tstamps[min(c.counter, P.n_stamps)];
+        if (c.t + dt > target) { dt = target - c.t; c.reached = 1; }
+      }
+      c.dt = dt;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+size_t tile_grad_lds(const TileGeo& T) {
+  return sizeof(double) * (6 * (size_t)T.max_slots + 8 * (size_t)T.tb);
+}
+
+namespace {
+template <int FLUX, int REC>
+size_t fu_lds(const TileGeo& T) {
+  return sizeof(double) * ((size_t)SlotLayout<FLUX, REC>::NQ * T.max_slots + 4 * (size_t)T.max_faces +
+                           4 * (size_t)T.tb + 16);
+}
+template <int FLUX, int REC>
+void fu_launch(const DevGeo& G, const TileGeo& T, const StageArgs& A, const StepParams& P,
+               const double* Wn, double* dtpart, int S, int r, hipStream_t st) {
+  const size_t lds = fu_lds<FLUX, REC>(T);
+  dim3 grid(T.ntiles, S), blk(256);
+  if (r == 2) {
+    auto k = k_fu_tile<FLUX, REC, 1>;
+    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, grid, blk, lds, st, G, T, A, P, Wn, dtpart, r);
+  } else {
+    auto k = k_fu_tile<FLUX, REC, 0>;
+    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, grid, blk, lds, st, G, T, A, P, Wn, dtpart, r);
+  }
+}
+template <int FLUX>
+void fu_rec(int rec, const DevGeo& G, const TileGeo& T, const StageArgs& A, const StepParams& P,
+            const double* Wn, double* dtpart, int S, int r, hipStream_t st) {
+  switch (rec) {
+    case R_FIRST: fu_launch<FLUX, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, st); break;
+    case R_SECOND: fu_launch<FLUX, R_SECOND>(G, T, A, P, Wn, dtpart, S, r, st); break;
+    case R_MUSCL: fu_launch<FLUX, R_MUSCL>(G, T, A, P, Wn, dtpart, S, r, st); break;
+    case R_ALBADA: fu_launch<FLUX, R_ALBADA>(G, T, A, P, Wn, dtpart, S, r, st); break;
+    case R_MINMOD: fu_launch<FLUX, R_MINMOD>(G, T, A, P, Wn, dtpart, S, r, st); break;
+    case R_TVB: fu_launch<FLUX, R_TVB>(G, T, A, P, Wn, dtpart, S, r, st); break;
+    default: fu_launch<FLUX, R_ENO2>(G, T, A, P, Wn, dtpart, S, r, st); break;
+  }
+}
+}  // namespace
+
+size_t tile_fu_lds(const TileGeo& T, int flux, int rec) {
+  if (flux == F_ROE) return rec == R_FIRST ? fu_lds<F_ROE, R_FIRST>(T) : fu_lds<F_ROE, R_MINMOD>(T);
+  if (flux == F_RUS) return rec == R_FIRST ? fu_lds<F_RUS, R_FIRST>(T) : fu_lds<F_RUS, R_MINMOD>(T);
+  return fu_lds<F_KEPEC, R_FIRST>(T);
+}
+
+void launch_tile_gradient(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
+                          hipStream_t st) {
+  const size_t lds = tile_grad_lds(T);
+  (void)hipFuncSetAttribute((const void*)k_grad_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+  hipLaunchKernelGGL(k_grad_tile, dim3(T.ntiles, S), dim3(256), lds, st, G, T, A);
+}
+
+void launch_tile_stage(int flux, int rec, const DevGeo& G, const TileGeo& T, const StageArgs& A,
+                       const StepParams& P, const double* Wn, double* dtpart, int S, int r,
+                       hipStream_t st) {
+  switch (flux) {
+    case F_ROE: fu_rec<F_ROE>(rec, G, T, A, P, Wn, dtpart, S, r, st); break;
+    case F_RUS: fu_rec<F_RUS>(rec, G, T, A, P, Wn, dtpart, S, r, st); break;
+    case F_KEPEC: fu_launch<F_KEPEC, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, st); break;
+    case F_ROEEC: fu_launch<F_ROEEC, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, st); break;
+    case F_KEP: fu_launch<F_KEP, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, st); break;
+    default: fu_launch<F_AVG, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, st); break;
+  }
+}
+
+void launch_step_final_tiled(const DevGeo& G, const StageArgs& A, const StepParams& P, int S,
+                             int nblk, const double* dtpart, int log_slot, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_final_tiled, dim3(S), dim3(256), 0, st, A, P, nblk, G.nv, dtpart,
+                     log_slot);
+}
+
+}  // namespace tsm

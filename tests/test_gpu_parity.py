@@ -102,9 +102,11 @@ def test_dt():
     assert abs(a - b) <= 1e-14 * abs(b)
 
 
-def test_vortex_334_steps():
+@pytest.mark.parametrize("path", ["auto", "csr"])
+def test_vortex_334_steps(path):
     m = vortex_mesh(33)
-    prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=10.0)
+    prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=10.0,
+                   kernel_path=path)
     s, o = _pair(m, prm, vortex(m.x, m.y))
     done, log = s.run(334, want_log=True)
     n, olog = o.run(334, want_log=True)
@@ -114,9 +116,11 @@ def test_vortex_334_steps():
     assert np.allclose(log[:, 3], olog[:, 3], rtol=1e-9, atol=0)    # residual norm
 
 
-def test_riemann_334_steps():
+@pytest.mark.parametrize("path", ["auto", "csr"])
+def test_riemann_334_steps(path):
     m = riemann_mesh(33)
-    prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8)
+    prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8,
+                   kernel_path=path)
     s, o = _pair(m, prm, riemann(m.x, m.y))
     s.run(334)
     o.run(334)
@@ -146,10 +150,11 @@ def test_navier_stokes_tgv():
     assert abs(k1 - k2) <= 1e-12 * abs(k2) and abs(e1 - e2) <= 1e-12 * abs(e2)
 
 
-def test_final_time_and_error_norms():
+@pytest.mark.parametrize("path", ["auto", "csr"])
+def test_final_time_and_error_norms(path):
     m = vortex_mesh(17)
     prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=1.0,
-                   time_stamps=2)
+                   time_stamps=2, kernel_path=path)
     s, o = _pair(m, prm, vortex(m.x, m.y))
     done = s.run(100000)
     n = o.run(100000)
@@ -161,9 +166,11 @@ def test_final_time_and_error_norms():
     assert np.allclose(a, b, rtol=1e-9, atol=1e-15)
 
 
-def test_deterministic_rerun():
+@pytest.mark.parametrize("path", ["auto", "csr"])
+def test_deterministic_rerun(path):
     m = riemann_mesh(33)
-    prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8)
+    prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8,
+                   kernel_path=path)
     s = T.Solver(m, prm)
     s.set_state(riemann(m.x, m.y))
     s.run(50)
@@ -173,9 +180,11 @@ def test_deterministic_rerun():
     assert np.array_equal(a, s.get_state())
 
 
-def test_batched_samples_equal_single():
+@pytest.mark.parametrize("path", ["auto", "csr"])
+def test_batched_samples_equal_single(path):
     m = vortex_mesh(25)
-    prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=0.5)
+    prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=0.5,
+                   kernel_path=path)
     states = [vortex(m.x, m.y, beta=b) for b in (4.0, 5.0, 6.0)]
     sb = T.Solver(m, prm, n_samples=3)
     for k, w in enumerate(states):
@@ -186,3 +195,47 @@ def test_batched_samples_equal_single():
         s1.set_state(w)
         s1.run(10000)
         assert np.array_equal(sb.get_state(k), s1.get_state())
+
+
+TILED_CASES = [(f, "minmod") for f in FLUXES] + [("kepes_tecno_roe", r) for r in RECS] + \
+    [("kepes_tecno_rusanov", r) for r in ("first", "second", "van_albada", "tvb_minmod")]
+
+
+@pytest.mark.parametrize("flux,rec", TILED_CASES)
+def test_tiled_runs_each_scheme(flux, rec):
+    """The LDS-tiled stage kernels against the oracle, 20 steps, on a mesh with
+    outlet, slip, inlet and farfield sides."""
+    st = (1.0, 0.3, 0.1, 1.2)
+    m = T.jittered_mesh(23, seed=5, bc={1: ("slip",), 2: ("outlet",), 3: ("inlet", st),
+                                        4: ("farfield", st)})
+    prm = T.Params(flux=flux, reconstruct=rec, tvb_M=10.0, cfl=0.4, final_time=0.8)
+    s, o = _pair(m, prm, riemann(m.x, m.y))
+    s.run(20)
+    o.run(20)
+    assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-12
+
+
+@pytest.mark.parametrize("tile", [16, 64, 256])
+def test_tiled_tile_sizes_agree_bitwise(tile):
+    m = vortex_mesh(41)
+    base = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=10.0)
+    a = T.Solver(m, base)
+    a.set_state(vortex(m.x, m.y))
+    a.run(30)
+    b = T.Solver(m, T.Params(**{**base.__dict__, "tile_vertices": tile}))
+    b.set_state(vortex(m.x, m.y))
+    b.run(30)
+    assert np.array_equal(a.get_state(), b.get_state())
+
+
+def test_tiled_then_csr_api_calls_see_the_same_state():
+    m = vortex_mesh(25)
+    prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=10.0)
+    s, o = _pair(m, prm, vortex(m.x, m.y))
+    s.run(7)
+    o.run(7)
+    assert rel_l2(s.residual(), o.residual()) < 1e-11
+    assert rel_l2(s.gradients(), o.gradients()) < 1e-11
+    s.run(3)
+    o.run(3)
+    assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-12
