@@ -114,9 +114,14 @@ int tsm_compute_dt(tsm_ctx* ctx, int sample, double* dt_global);
 /* Advances every sample by up to nsteps time steps of the loop body of
    FiniteVolume::solve (src/fv.cc:2273-2368); a sample stops by itself when it
    reaches final_time.  done[s] receives the steps sample s performed; log
-   (optional, 4*nsteps for sample 0) receives (iter, dt_global, t, res_total). */
+   (optional, n_samples*nsteps*4, sample-major) receives per step
+   (iter, dt_global, t, res_total), zero rows after a sample stopped. */
 int tsm_run(tsm_ctx* ctx, long nsteps, long* done, double* log);
 int tsm_time(tsm_ctx* ctx, int sample, double* t, long* iter);
+/* Loop state of one sample: v[0] t, v[1] dt of the last/next step, v[2]
+   res_total; iter; counter = index of the next output time stamp
+   (src/fv.cc:2324-2343); done = the sample has left its time loop. */
+int tsm_status(tsm_ctx* ctx, int sample, double* v, long* iter, int* counter, int* done);
 
 /* FiniteVolume::compute_error_norm (src/fv.cc:1712-1795): exact primitive
    field in, out[24] = (L1, L2, Linf) for (T, p, u, v, w, rho, Mach, entropy). */

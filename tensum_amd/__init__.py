@@ -171,6 +171,8 @@ def lib():
         L.tsm_compute_dt.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.tsm_run.argtypes = [C.c_void_p, C.c_long, C.c_void_p, C.c_void_p]
         L.tsm_time.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
+        L.tsm_status.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_long),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.tsm_error_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.tsm_global.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tsm_geometry_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -297,9 +299,15 @@ class Solver:
 
     def run(self, nsteps, want_log=False):
         done = np.zeros(self.n_samples, dtype=np.int64)
-        log = np.zeros((nsteps, 4)) if want_log else None
+        log = np.zeros((self.n_samples, nsteps, 4)) if want_log else None
         _check(lib().tsm_run(self._h, nsteps, _dptr(done), _dptr(log) if want_log else None))
-        return (done, log) if want_log else done
+        return (done, log[0] if self.n_samples == 1 else log) if want_log else done
+
+    def status(self, sample=0):
+        v = np.zeros(3)
+        it, cnt, dn = C.c_long(), C.c_int(), C.c_int()
+        _check(lib().tsm_status(self._h, sample, _dptr(v), C.byref(it), C.byref(cnt), C.byref(dn)))
+        return dict(t=v[0], dt=v[1], res=v[2], iter=it.value, counter=cnt.value, done=bool(dn.value))
 
     def time(self, sample=0):
         t, it = C.c_double(), C.c_long()

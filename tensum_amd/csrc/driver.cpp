@@ -1,0 +1,458 @@
+// driver.cpp -- `tensum`: drop-in command-line driver over libtsm.
+//
+// Same command line as the reference (src/process_command_line.cc:23-131):
+//   tensum -i param.in -npart 1 -L <dir> [-r] [-p] [-b] [-v] [-d]
+// reads <dir>/PARTITION/part_file_0.dat (readme/README.md:155-201), runs every
+// sample of the sample list as one batch on the GPU, and writes the
+// reference's per-sample outputs in SAMPLE_<id>/: master_file.visit,
+// sol<stamp>_0.vtk, residual.dat, global.dat, error_norms.dat,
+// restart_<head>_0.dat + LAST_RESTART_HEAD.txt (src/writer.cc:72-409,
+// src/fv.cc:1939-2022, :2175-2225, :1712-1795), and with the online
+// statistics tool MEAN_ONLINE / VARIANCE_ONLINE / SECOND_MOMENT_ONLINE VTK
+// files (src/fv.cc:595-694, :2027-2094).  Errors are printed and give exit
+// status 1 (the reference exits with 0, src/mpi_utils.h:29-35).
+#include <sys/stat.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/tsm.h"
+#include "expr.h"
+#include "param.h"
+
+namespace {
+
+struct PartFile {
+  std::vector<double> x, y;
+  std::vector<int> btag, bv, cv, pptr, pv;
+};
+
+bool die(const std::string& m) {
+  std::cerr << "ERROR --> " << m << std::endl;
+  return false;
+}
+
+// readme/README.md:155-201, src/read_part_mesh_file.cc:43-214 (NPART = 1)
+bool read_part_file(const std::string& path, PartFile& P) {
+  std::ifstream f(path);
+  if (!f) return die("cannot open " + path);
+  std::string w;
+  int dim, npart, n, ntot;
+  f >> w >> dim;
+  if (w != "DIMENSION" || dim != 2) return die("part file: DIMENSION 2 expected");
+  f >> w >> npart;
+  if (w != "PARTITIONS" || npart != 1) return die("part file: one partition expected");
+  f >> w >> n >> w >> ntot;
+  if (n <= 0 || n != ntot) return die("part file: NODES");
+  P.x.resize(n); P.y.resize(n);
+  std::vector<int> gid(n);
+  std::map<int, int> g2l;
+  for (int i = 0; i < n; ++i) {
+    double z;
+    int ns, part, dbnd;
+    f >> gid[i] >> P.x[i] >> P.y[i] >> z >> ns;
+    for (int k = 0; k < ns; ++k) f >> part;
+    f >> dbnd;
+    g2l[gid[i]] = i;
+  }
+  int np;
+  f >> w >> np;
+  if (w != "PERIODIC") return die("part file: PERIODIC expected");
+  P.pptr.assign(1, 0);
+  for (int i = 0; i < np; ++i) {
+    int id, cnt, v, nl, part;
+    f >> id >> cnt;
+    for (int k = 0; k < cnt; ++k) { f >> v; P.pv.push_back(g2l.at(v)); }
+    f >> nl;
+    for (int k = 0; k < nl; ++k) f >> part;
+    P.pptr.push_back((int)P.pv.size());
+  }
+  int nb, nbt;
+  f >> w >> nb >> w >> nbt;
+  if (w != "OF") return die("part file: FACES");
+  for (int i = 0; i < nb; ++i) {
+    int tag, a, b, per;
+    f >> tag >> a >> b >> per;
+    P.btag.push_back(tag);
+    P.bv.push_back(g2l.at(a));
+    P.bv.push_back(g2l.at(b));
+  }
+  int nc, nct;
+  f >> w >> nc >> w >> nct;
+  for (int i = 0; i < nc; ++i) {
+    int a, b, c, ghost;
+    f >> a >> b >> c >> ghost;
+    if (ghost) return die("part file: ghost cells need NPART > 1");
+    P.cv.push_back(g2l.at(a)); P.cv.push_back(g2l.at(b)); P.cv.push_back(g2l.at(c));
+  }
+  if (!f) return die("part file truncated: " + path);
+  return true;
+}
+
+bool mkdirs(const std::string& d) {
+  if (mkdir(d.c_str(), 0755) == 0 || errno == EEXIST) return true;
+  return die("cannot create " + d);
+}
+
+std::string stamp_name(const std::string& base, int counter) {
+  std::ostringstream o;
+  o << base << std::setw(4) << std::setfill('0') << counter << "_0";
+  return o.str();
+}
+
+struct Mat {
+  double gamma, R;
+  double rho(const double* w) const { return w[3] / (R * w[0]); }
+  double mach(const double* w) const { return std::sqrt(w[1] * w[1] + w[2] * w[2]) / std::sqrt(gamma * R * w[0]); }
+  double entropy(const double* w) const { return std::log(w[3]) - gamma * std::log(rho(w)); }
+};
+
+// Legacy ASCII VTK in the layout of Writer::output_vtk (src/writer.cc:129-328)
+void write_vtk(const std::string& file, const PartFile& P, const std::vector<double>& W,
+               const std::vector<double>& dE, const std::vector<double>& dcarea, const Mat& M,
+               const tsm::Settings& S, double time) {
+  std::ofstream vtk(file);
+  const int nv = (int)P.x.size(), nc = (int)P.cv.size() / 3;
+  vtk << "# vtk DataFile Version 3.0" << std::endl << "tensum" << std::endl << "ASCII" << std::endl
+      << "DATASET UNSTRUCTURED_GRID" << std::endl;
+  if (S.time_mode == "unsteady")
+    vtk << "FIELD FieldData 1" << std::endl << "TIME 1 1 double" << std::endl << time << std::endl;
+  vtk << "POINTS  " << nv << "  double" << std::endl;
+  for (int i = 0; i < nv; ++i) vtk << P.x[i] << " " << P.y[i] << " " << 0.0 << std::endl;
+  vtk << "CELLS  " << nc << " " << 4 * nc << std::endl;
+  for (int i = 0; i < nc; ++i)
+    vtk << 3 << " " << P.cv[3 * i] << " " << P.cv[3 * i + 1] << " " << P.cv[3 * i + 2] << std::endl;
+  vtk << "CELL_TYPES  " << nc << std::endl;
+  for (int i = 0; i < nc; ++i) vtk << 5 << std::endl;
+  vtk << "POINT_DATA  " << nv << std::endl;
+  vtk << "SCALARS pressure double 1" << std::endl << "LOOKUP_TABLE default" << std::endl;
+  for (int i = 0; i < nv; ++i) vtk << std::setprecision(12) << W[4 * i + 3] << std::endl;
+  vtk << "SCALARS temperature double 1" << std::endl << "LOOKUP_TABLE default" << std::endl;
+  for (int i = 0; i < nv; ++i) vtk << std::setprecision(12) << W[4 * i] << std::endl;
+  vtk << "VECTORS velocity double" << std::endl;
+  for (int i = 0; i < nv; ++i)
+    vtk << std::setprecision(12) << W[4 * i + 1] << " " << W[4 * i + 2] << " " << 0.0 << std::endl;
+  auto has = [&](const char* n) {
+    for (auto& v : S.write_variables) if (v == n) return true;
+    return false;
+  };
+  if (has("mach")) {
+    vtk << "SCALARS mach double 1" << std::endl << "LOOKUP_TABLE default" << std::endl;
+    for (int i = 0; i < nv; ++i) vtk << std::setprecision(12) << M.mach(&W[4 * i]) << std::endl;
+  }
+  if (has("density")) {
+    vtk << "SCALARS density double 1" << std::endl << "LOOKUP_TABLE default" << std::endl;
+    for (int i = 0; i < nv; ++i) vtk << std::setprecision(12) << M.rho(&W[4 * i]) << std::endl;
+  }
+  if (has("entropy")) {
+    vtk << "SCALARS entropy double 1" << std::endl << "LOOKUP_TABLE default" << std::endl;
+    for (int i = 0; i < nv; ++i) vtk << std::setprecision(12) << M.entropy(&W[4 * i]) << std::endl;
+  }
+  if (has("vorticity")) {  // src/material.h:622-627 in (f, vx, vy, l) x (x, y) layout
+    vtk << "SCALARS vorticity double 1" << std::endl << "LOOKUP_TABLE default" << std::endl;
+    for (int i = 0; i < nv; ++i) {
+      const double* g = &dE[8 * i];
+      double om = M.R * W[4 * i] * (g[4] + W[4 * i + 2] * g[6] - g[3] - W[4 * i + 1] * g[7]);
+      vtk << std::setprecision(12) << om << std::endl;
+    }
+  }
+  vtk << "SCALARS  dual_area  double 1" << std::endl << "LOOKUP_TABLE default" << std::endl;
+  for (int i = 0; i < nv; ++i) vtk << std::setprecision(12) << dcarea[i] << std::endl;
+}
+
+struct Moments {  // src/fv.cc:595-694 (primitive fields + density/mach/entropy)
+  std::vector<std::vector<double>> fm, sm, var;  // per stamp: nv * 7
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  std::string ifile, loc;
+  int npart = -1;
+  bool restart = false, check_only = false;
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    if (a == "-i" && i + 1 < argc) ifile = argv[++i];
+    else if (a == "-npart" && i + 1 < argc) npart = std::atoi(argv[++i]);
+    else if (a == "-L" && i + 1 < argc) loc = argv[++i];
+    else if (a == "-r") restart = true;
+    else if (a == "--check") { check_only = true; npart = npart < 1 ? 1 : npart; loc = loc.empty() ? "." : loc; }
+    else if (a == "-p" || a == "-b" || a == "-v" || a == "-d") {}
+    else if (a == "-axi") { std::cerr << "ERROR --> -axi is not supported (2-D only)" << std::endl; return 1; }
+    else { std::cerr << "Unknown command line flag: " << a << std::endl; ifile.clear(); break; }
+  }
+  if (ifile.empty() || loc.empty() || npart < 1) {
+    std::cout << "Valid flags are:\n   -i filename   Specify input file name (required)\n"
+                 "   -npart N      Specify number of mesh partitions N (required, 1)\n"
+                 "   -L path       Location of mesh partition folder (required)\n"
+                 "   -r            Restart from SAMPLE_<id>/restart files\n";
+    return 1;
+  }
+  if (npart != 1) { std::cerr << "ERROR --> one GPU holds the whole mesh: use -npart 1" << std::endl; return 1; }
+  tsm::Settings S;
+  std::string err;
+  if (!tsm::read_settings(ifile, S, err)) { std::cerr << "ERROR --> " << err << std::endl; return 1; }
+  if (check_only) {  // parse report, no device use
+    const tsm_params& p = S.p;
+    std::cout << std::setprecision(17) << "cell_type " << p.cell_type << "\ntime_mode " << S.time_mode
+              << "\ntime_scheme " << S.time_scheme << "\ncfl " << p.cfl << "\ntime_step " << p.time_step
+              << "\nfinal_time " << p.final_time << "\nmax_iter " << p.max_iter << "\nreconstruct "
+              << p.reconstruct << "\ntvb_M " << p.tvb_M << "\nliou_fix " << p.liou_fix << "\ngamma "
+              << p.gamma << "\ngas_const " << p.gas_const << "\nmodel " << p.model << "\nmu_model "
+              << p.mu_model << "\nmu_ref " << p.mu_ref << "\nprandtl " << p.prandtl << "\nflux " << p.flux
+              << "\nev_alpha " << p.ev_alpha << "\nev_beta " << p.ev_beta << "\nnd_active " << p.nd_active
+              << "\nn_samples " << S.sample_list.size() << "\nrnd_per_sample " << S.rnd_per_sample
+              << "\ntime_stamps " << S.n_time_stamps << "\nonline_stat " << S.online_stat
+              << "\nwrite_restart " << S.write_restart << "\nhas_global " << S.has_global
+              << "\nfind_error " << S.find_error << "\nn_constants " << S.constants.size()
+              << "\nn_bc " << S.bc.size() << "\n";
+    for (auto& kv : S.bc) std::cout << "bc " << kv.first << " " << kv.second.type << "\n";
+    for (auto& kv : S.constants) std::cout << "const " << kv.first << " " << kv.second << "\n";
+    for (const char* v : {"temperature", "xvelocity", "yvelocity", "pressure"}) {
+      tsm::Expr e;
+      if (!e.parse(S.ic[v], S.constants, err)) { std::cerr << "ERROR --> " << err << std::endl; return 1; }
+      e.set_pert([](int, double, double, double k) { return 0.1 * k; });
+      std::cout << "ic " << v << " " << e.eval(0.25, -0.5, 0.0) << "\n";
+    }
+    return 0;
+  }
+  if (S.time_mode != "unsteady") { std::cerr << "ERROR --> steady time_mode is not supported" << std::endl; return 1; }
+  if (S.time_scheme == "heuns") { std::cerr << "ERROR --> heuns is not supported (it is broken in the reference)" << std::endl; return 1; }
+  PartFile P;
+  if (!read_part_file(loc + "/PARTITION/part_file_0.dat", P)) return 1;
+  const int nv = (int)P.x.size();
+
+  // boundary table: constant boundary states from the expressions
+  std::vector<int> bc_tag, bc_type;
+  std::vector<double> bc_state;
+  const std::map<std::string, int> bcode = {{"slip", 1}, {"noslip", 2}, {"farfield", 3}, {"inlet", 4},
+                                            {"outlet", 5}, {"pressure", 6}, {"periodic", 7}};
+  for (auto& kv : S.bc) {
+    bc_tag.push_back(kv.first);
+    bc_type.push_back(bcode.at(kv.second.type));
+    double st[4] = {0, 0, 0, 0};
+    const char* names[4] = {"temperature", "xvelocity", "yvelocity", "pressure"};
+    for (int k = 0; k < 4; ++k) {
+      auto it = kv.second.fn.find(names[k]);
+      if (it == kv.second.fn.end()) continue;
+      tsm::Expr e;
+      if (!e.parse(it->second, S.constants, err)) { std::cerr << "ERROR --> " << err << std::endl; return 1; }
+      double a = e.eval(0.0, 0.0, 0.0), b = e.eval(0.37, -1.3, 0.0);
+      if (e.uses_time() || a != b) {
+        std::cerr << "ERROR --> boundary " << kv.first << ": only constant boundary values are supported" << std::endl;
+        return 1;
+      }
+      st[k] = a;
+    }
+    bc_state.insert(bc_state.end(), st, st + 4);
+  }
+  tsm_mesh m{};
+  m.n_vertex = nv; m.x = P.x.data(); m.y = P.y.data();
+  m.n_bface = (int)P.btag.size(); m.bface_tag = P.btag.data(); m.bface_v = P.bv.data();
+  m.n_cell = (int)P.cv.size() / 3; m.cell_v = P.cv.data();
+  m.n_periodic = (int)P.pptr.size() - 1; m.periodic_ptr = P.pptr.data(); m.periodic_v = P.pv.data();
+  m.n_bc = (int)bc_tag.size(); m.bc_tag = bc_tag.data(); m.bc_type = bc_type.data(); m.bc_state = bc_state.data();
+
+  // samples and their random inputs (src/fv.cc:502-551)
+  const int ns = (int)S.sample_list.size();
+  std::vector<std::vector<double>> rnd(ns);
+  if (S.rnd_per_sample > 0) {
+    std::ifstream rf(S.rnd_file_loc);
+    if (!rf) { std::cerr << "ERROR --> the random parameter file " << S.rnd_file_loc << " doesn't exist!!" << std::endl; return 1; }
+    int total;
+    rf >> total;
+    std::map<int, std::vector<double>> rows;
+    for (int r = 0; r < total; ++r) {
+      int id;
+      rf >> id;
+      std::vector<double> v(S.rnd_per_sample);
+      for (auto& q : v) rf >> q;
+      rows[id] = v;
+    }
+    for (int k = 0; k < ns; ++k) {
+      auto it = rows.find(S.sample_list[k]);
+      if (it == rows.end()) { std::cerr << "ERROR --> Sample " << S.sample_list[k] << " not present in " << S.rnd_file_loc << std::endl; return 1; }
+      rnd[k] = it->second;
+    }
+  }
+  tsm_ctx* ctx = nullptr;
+  if (tsm_create(&m, &S.p, ns, &ctx) != TSM_OK) {
+    std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
+    return 1;
+  }
+  const Mat M{S.p.gamma, S.p.gas_const};
+  std::vector<double> dcarea(nv), mcarea(nv);
+  tsm_geometry(ctx, nullptr, nullptr, dcarea.data(), mcarea.data());
+  std::vector<std::string> dirs(ns);
+  std::vector<std::ofstream> res_f(ns), glob_f(ns);
+  std::vector<int> head(ns, 0);
+  std::vector<double> W(4 * (size_t)nv), dE(8 * (size_t)nv);
+  for (int k = 0; k < ns; ++k) {
+    dirs[k] = "SAMPLE_" + std::to_string(S.sample_list[k]);
+    if (!mkdirs(dirs[k])) return 1;
+    // initial state: restart file or expressions (src/fv.cc:300-402)
+    bool from_restart = false;
+    if (restart) {
+      std::ifstream hf(dirs[k] + "/LAST_RESTART_HEAD.txt");
+      int h;
+      if (hf >> h) {
+        std::ifstream rf(dirs[k] + "/restart_" + std::to_string(h) + "_0.dat");
+        double wz;
+        for (int i = 0; i < nv; ++i) rf >> W[4 * i] >> W[4 * i + 1] >> W[4 * i + 2] >> wz >> W[4 * i + 3];
+        if (!rf) { std::cerr << "ERROR --> bad restart file in " << dirs[k] << std::endl; return 1; }
+        from_restart = true;
+        std::cout << "    Restart state of " << dirs[k] << " read (the run restarts its clock at t = 0)" << std::endl;
+      }
+    }
+    if (!from_restart) {
+      tsm::Expr e[4];
+      const char* names[4] = {"temperature", "xvelocity", "yvelocity", "pressure"};
+      const std::vector<double>& r = rnd[k];
+      for (int q = 0; q < 4; ++q) {
+        if (!e[q].parse(S.ic[names[q]], S.constants, err)) { std::cerr << "ERROR --> " << err << std::endl; return 1; }
+        e[q].set_pert([&r](int type, double, double, double kk) {
+          int id = (int)kk;  // PERT4, src/perturb.cc:194-209
+          if (type != 4 || id < 1 || id > (int)r.size()) return (double)NAN;
+          return r[id - 1];
+        });
+      }
+      for (int i = 0; i < nv; ++i)
+        for (int q = 0; q < 4; ++q) W[4 * i + q] = e[q].eval(P.x[i], P.y[i], 0.0);
+    }
+    if (tsm_set_state(ctx, k, W.data()) != TSM_OK) { std::cerr << "ERROR --> " << tsm_last_error() << std::endl; return 1; }
+    if (!restart) std::ofstream(dirs[k] + "/master_file.visit") << "!NBLOCKS 1" << std::endl;
+    if (S.write_log) res_f[k].open(dirs[k] + "/residual.dat", std::ios::app);
+    if (S.has_global) glob_f[k].open(dirs[k] + "/global.dat", std::ios::app);
+  }
+  auto output = [&](int k, int counter, double t) {
+    tsm_get_state(ctx, k, W.data());
+    tsm_gradients(ctx, k, dE.data());
+    std::string name = stamp_name("sol", counter);
+    write_vtk(dirs[k] + "/" + name + ".vtk", P, W, dE, dcarea, M, S, t);
+    std::ofstream(dirs[k] + "/master_file.visit", std::ios::app) << name << ".vtk" << std::endl;
+  };
+  // online statistics across samples at each stamp (Welford in sample order)
+  const int nst = S.n_time_stamps + 1;
+  std::vector<std::vector<std::vector<double>>> snap;
+  if (S.online_stat) snap.assign(nst, std::vector<std::vector<double>>(ns));
+  auto keep = [&](int k, int counter) {
+    if (!S.online_stat) return;
+    tsm_get_state(ctx, k, W.data());
+    snap[counter][k] = W;
+  };
+  for (int k = 0; k < ns; ++k) {
+    if (S.write_soln) output(k, 0, 0.0);
+    keep(k, 0);
+  }
+  // time loop, all samples together; outputs as each reaches its stamps
+  std::vector<double> logbuf(4 * (size_t)ns);
+  std::vector<long> done(ns);
+  std::vector<int> cnt_before(ns, 1), finished(ns, 0);
+  int live = ns;
+  while (live > 0) {
+    if (tsm_run(ctx, 1, done.data(), logbuf.data()) != TSM_OK) {
+      std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
+      return 1;
+    }
+    for (int k = 0; k < ns; ++k) {
+      if (finished[k]) continue;
+      double v[3];
+      long iter;
+      int counter, dn;
+      tsm_status(ctx, k, v, &iter, &counter, &dn);
+      if (done[k] > 0) {
+        const double* L = &logbuf[4 * (size_t)k];
+        if (S.write_log)
+          res_f[k] << std::setw(10) << (long)L[0] << "  " << std::scientific << std::setprecision(12)
+                   << L[1] << "  " << L[2] << "  " << L[3] << std::endl;
+        if (S.has_global) {
+          double ke, ent;
+          tsm_global(ctx, k, &ke, &ent);
+          glob_f[k] << std::defaultfloat << std::setprecision(6) << (long)L[0] << "  " << L[2]
+                    << "  " << ke << "  " << ent << std::endl;
+        }
+        if (counter > cnt_before[k]) {
+          if (S.write_soln) output(k, cnt_before[k], v[0]);
+          keep(k, cnt_before[k]);
+          cnt_before[k] = counter;
+        }
+        if (S.write_restart && (iter % S.restart_frequency == 0 || v[0] == S.p.final_time)) {
+          tsm_get_state(ctx, k, W.data());
+          std::ofstream fo(dirs[k] + "/restart_" + std::to_string(head[k]) + "_0.dat");
+          for (int i = 0; i < nv; ++i)
+            fo << std::scientific << std::setprecision(15) << W[4 * i] << "  " << W[4 * i + 1] << "  "
+               << W[4 * i + 2] << "  " << 0.0 << "  " << W[4 * i + 3] << std::endl;
+          fo << iter << std::endl << counter << std::endl << v[0] << std::endl << 1.0 << std::endl;
+          std::ofstream(dirs[k] + "/LAST_RESTART_HEAD.txt") << head[k] << std::endl;
+          head[k] = (head[k] + 1) % 2;
+        }
+      }
+      if (dn || done[k] == 0) {
+        finished[k] = 1;
+        --live;
+        if (!std::isfinite(v[2]))
+          std::cerr << "    " << dirs[k] << ": non-finite residual at t = " << v[0] << ", time loop left" << std::endl;
+      }
+    }
+  }
+  // error norms (src/fv.cc:1712-1795)
+  if (S.find_error) {
+    for (int k = 0; k < ns; ++k) {
+      tsm::Expr e[4];
+      const char* names[4] = {"temperature", "xvelocity", "yvelocity", "pressure"};
+      for (int q = 0; q < 4; ++q)
+        if (!e[q].parse(S.exact[names[q]], S.constants, err)) { std::cerr << "ERROR --> " << err << std::endl; return 1; }
+      std::vector<double> ex(4 * (size_t)nv);
+      for (int i = 0; i < nv; ++i)
+        for (int q = 0; q < 4; ++q) ex[4 * i + q] = e[q].eval(P.x[i], P.y[i], S.p.final_time);
+      double o[24];
+      tsm_error_norms(ctx, k, ex.data(), o);
+      std::ofstream ef(dirs[k] + "/error_norms.dat");
+      const char* lab[8] = {"TEMPERATURE", "PRESSURE", "VEL_X", "VEL_Y", "VEL_Z", "DENSITY", "MACH", "ENTROPY"};
+      ef << "N = " << nv << "\n" << std::scientific << std::setprecision(8);
+      for (int q = 0; q < 8; ++q)
+        ef << lab[q] << " (L1, L2, Linf): " << o[3 * q] << "   " << o[3 * q + 1] << "   " << o[3 * q + 2] << "\n";
+    }
+  }
+  // moments across samples, in sample order (src/fv.cc:595-694, src/reduce_moments.cc:115-141)
+  if (S.online_stat) {
+    mkdirs("MEAN_ONLINE"); mkdirs("VARIANCE_ONLINE"); mkdirs("SECOND_MOMENT_ONLINE");
+    for (int st = 0; st < nst; ++st) {
+      std::vector<double> mean(4 * (size_t)nv, 0.0), m2(4 * (size_t)nv, 0.0), s2(4 * (size_t)nv, 0.0);
+      int kk = 0;
+      for (int k = 0; k < ns; ++k) {
+        const auto& x = snap[st][k];
+        if (x.empty()) continue;
+        ++kk;
+        for (size_t i = 0; i < x.size(); ++i) {
+          if (kk == 1) { mean[i] = x[i]; s2[i] = x[i] * x[i]; m2[i] = 0.0; continue; }
+          double Mk1 = mean[i];
+          double Mk = x[i] * (1.0 / kk) + Mk1 * ((kk - 1.0) / kk);
+          mean[i] = Mk;
+          s2[i] += x[i] * x[i];
+          m2[i] += (x[i] - Mk1) * (x[i] - Mk);
+        }
+      }
+      if (kk < 2) continue;
+      std::vector<double> var(m2.size()), sm(s2.size());
+      for (size_t i = 0; i < m2.size(); ++i) { var[i] = m2[i] / (kk - 1); sm[i] = s2[i] / kk; }
+      std::vector<double> zero(8 * (size_t)nv, 0.0);
+      tsm::Settings S0 = S;
+      S0.write_variables.clear();
+      write_vtk(stamp_name("MEAN_ONLINE/mean", st) + ".vtk", P, mean, zero, dcarea, M, S0, st * S.p.final_time / S.n_time_stamps);
+      write_vtk(stamp_name("VARIANCE_ONLINE/var", st) + ".vtk", P, var, zero, dcarea, M, S0, st * S.p.final_time / S.n_time_stamps);
+      write_vtk(stamp_name("SECOND_MOMENT_ONLINE/sec_mom", st) + ".vtk", P, sm, zero, dcarea, M, S0, st * S.p.final_time / S.n_time_stamps);
+    }
+  }
+  tsm_destroy(ctx);
+  return 0;
+}

@@ -480,10 +480,14 @@ extern "C" int tsm_run(tsm_ctx* c, long nsteps, long* done, double* log) {
   if (done)
     for (int s = 0; s < S; ++s) done[s] = (long)(after[s].iter - before[s].iter);
   if (log && nsteps > 0) {
-    long n0 = (long)(after[0].iter - before[0].iter);
-    std::vector<double> buf((size_t)nsteps * 4);
+    const size_t cap = (size_t)c->A.log_cap;
+    std::vector<double> buf((size_t)S * cap * 4);
     HIP_TRY(hipMemcpy(buf.data(), c->A.log, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (long k = 0; k < nsteps * 4; ++k) log[k] = (k < n0 * 4) ? buf[k] : 0.0;
+    for (int s = 0; s < S; ++s) {
+      const long ns = (long)(after[s].iter - before[s].iter);
+      for (long k = 0; k < nsteps * 4; ++k)
+        log[(size_t)s * nsteps * 4 + k] = (k < ns * 4) ? buf[(size_t)s * cap * 4 + k] : 0.0;
+    }
   }
   c->A.log = (log ? c->A.log : c->A.log);
   return TSM_OK;
@@ -496,6 +500,18 @@ extern "C" int tsm_time(tsm_ctx* c, int s, double* t, long* iter) {
   HIP_TRY(hipMemcpy(&q, c->A.ctl + s, sizeof(q), hipMemcpyDeviceToHost));
   if (t) *t = q.t;
   if (iter) *iter = (long)q.iter;
+  return TSM_OK;
+}
+
+extern "C" int tsm_status(tsm_ctx* c, int s, double* v, long* iter, int* counter, int* done) {
+  if (!c || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  tsm::Ctl q;
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(&q, c->A.ctl + s, sizeof(q), hipMemcpyDeviceToHost));
+  if (v) { v[0] = q.t; v[1] = q.dt; v[2] = q.res_total; }
+  if (iter) *iter = (long)q.iter;
+  if (counter) *counter = q.counter;
+  if (done) *done = q.done;
   return TSM_OK;
 }
 

@@ -83,3 +83,144 @@ def conserved(prim, gam=1.4, R=1.0):
     Tm, u, v, p = prim.T
     rho = p / (R * Tm)
     return np.stack([rho, rho * u, rho * v, p / (gam - 1) + 0.5 * rho * (u * u + v * v)], 1)
+
+
+def write_part_file(mesh, directory, periodic_tags=()):
+    """Writes <directory>/PARTITION/part_file_0.dat for NPART = 1 in the layout of
+    readme/README.md:155-201 (gids 0-based, coordinates at 17 digits)."""
+    import os
+    d = os.path.join(directory, "PARTITION")
+    os.makedirs(d, exist_ok=True)
+    n = mesh.n_vertex
+    on_b = np.zeros(n, dtype=int)
+    on_b[mesh.bface_v] = 1
+    lines = ["DIMENSION 2", "PARTITIONS 1", f"NODES {n} OF {n}"]
+    lines += [f"{i} {mesh.x[i]:.17g} {mesh.y[i]:.17g} 0 1 0 {on_b[i]}" for i in range(n)]
+    nper = mesh.periodic_ptr.size - 1 if mesh.periodic_ptr.size else 0
+    lines.append(f"PERIODIC {nper}")
+    for k in range(nper):
+        mem = mesh.periodic_v[mesh.periodic_ptr[k]:mesh.periodic_ptr[k + 1]]
+        lines.append(f"{k} {mem.size} " + " ".join(map(str, mem)) + " 1 0")
+    nb = mesh.bface_tag.size
+    lines.append(f"FACES {nb} OF {nb}")
+    bv = mesh.bface_v.reshape(-1, 2)
+    for k in range(nb):
+        per = 1 if mesh.bface_tag[k] in periodic_tags else 0
+        lines.append(f"{mesh.bface_tag[k]} {bv[k, 0]} {bv[k, 1]} {per}")
+    cv = mesh.cell_v.reshape(-1, 3)
+    lines.append(f"CELLS {cv.shape[0]} OF {cv.shape[0]}")
+    lines += [f"{a} {b} {c} 0" for a, b, c in cv]
+    with open(os.path.join(d, "part_file_0.dat"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+VORTEX_PARAM = """grid
+{{
+   cell      median
+}}
+numeric
+{{
+   time_mode    unsteady
+   time_scheme  ssprk3
+   time_step    0.0
+   cfl          0.8
+   max_iter     5000
+   final_time   {final_time}
+   min_residue  0.0
+   reconstruct
+   {{
+      method   minmod
+   }}
+   bc_scheme    weak
+   liou_fix     no
+   sample_list
+   {{
+      groups
+      {{
+      }}
+      free_list
+      {{
+      }}
+   }}
+   rnd_file_loc    rnd_nos.dat
+   rnd_per_sample  0
+}}
+material
+{{
+   gamma       1.4
+   gas_const   1
+   model       euler
+   viscosity
+   {{
+      model    constant
+      mu_ref   0.0
+   }}
+   prandtl     0.72
+   flux        kepes_tecno_roe
+   flux_par
+   {{
+      alpha      0
+      beta       0
+   }}
+   balance
+   {{
+       switch_active  no
+       ND             yes
+       HEAT           yes
+       VISC           yes
+   }}
+}}
+constants
+{{
+   gam   1.4
+   beta  5
+}}
+initial_condition
+{{
+   temperature  1.0 - (gam-1)*(pow(beta,2)/(8*gam*pi*pi))*exp((1-pow(x,2)-pow(y,2)))
+   xvelocity    1 - (beta/(2*pi))*(y)*exp(0.5*(1-pow(x,2)-pow(y,2)))
+   yvelocity    1 + (beta/(2*pi))*(x)*exp(0.5*(1-pow(x,2)-pow(y,2)))
+   zvelocity    0.0
+   pressure     pow(1.0 - (gam-1)*(pow(beta,2)/(8*gam*pi*pi))*exp((1-pow(x,2)-pow(y,2))),gam/(gam-1))
+}}
+boundary
+{{
+   1 2 3 4
+   {{
+      type         periodic
+   }}
+}}
+exact_soln
+{{
+   available    yes
+   temperature  1.0 - (gam-1)*(pow(beta,2)/(8*gam*pi*pi))*exp((1-pow(x-t,2)-pow(y-t,2)))
+   xvelocity    1 - (beta/(2*pi))*(y-t)*exp(0.5*(1-pow(x-t,2)-pow(y-t,2)))
+   yvelocity    1 + (beta/(2*pi))*(x-t)*exp(0.5*(1-pow(x-t,2)-pow(y-t,2)))
+   zvelocity    0.0
+   pressure     pow(1.0 - (gam-1)*(pow(beta,2)/(8*gam*pi*pi))*exp((1-pow(x-t,2)-pow(y-t,2))),gam/(gam-1))
+}}
+integrals
+{{
+}}
+output
+{{
+   format                vtk
+   frequency             100
+   time_stamps           {stamps}
+   use_online_stat_tool  no
+   variables
+   {{
+       density
+       vorticity
+   }}
+   surfaces
+   {{
+   }}
+   restart               yes
+   restart_frequency     10
+   log_output            yes
+   soln_output           yes
+   save_global           yes
+   find_error            yes
+}}
+"""
