@@ -39,6 +39,12 @@ def kernel_bytes(nv, nf, nc, tiles):
         # W, dE, coords, U_old source W^n, dcarea, weight, offsets, W write;
         # local faces (slot pair 4 B + normal 16 B), 16-bit gather entries
         "fu_tile": (32 + 64 + 16 + 32 + 8 + 8 + 4 + 32) * nv + 20 * fl + 4 * nf + 4 * halo,
+        # fused two-ring stage kernel: W, coords, 1/(6 mcarea), U_old source
+        # W^n, dcarea, weight, gather offsets, W write; gradient cell entries
+        # of the owned and first-ring slots (4 B + 4 B offsets per slot);
+        # local faces, gather entries, halo ids of both rings
+        "stage_fused": (32 + 16 + 8 + 32 + 8 + 8 + 4 + 32) * nv + 4 * tiles.get("grad_cells", 0)
+                       + 4 * (nv + halo) + 20 * fl + 4 * nf + 4 * (halo + tiles.get("halo2", 0)),
     }
 
 
@@ -106,9 +112,11 @@ def main():
 
     # per-kernel durations (HIP events on the library's stream)
     ku = s.bench(20)
-    tiles = T.host_tiling(mesh, prm.tile_vertices or 128)
+    tiles = T.host_tiling(mesh, prm.tile_vertices or 128, two_rings=True)
     kb = kernel_bytes(nv, nf, nc, tiles)
-    if ku["dt_us"] == 0.0:  # tiled kernels
+    if ku["dt_us"] == 0.0 and tiles["two_rings"]:  # fused stage kernel
+        per_launch = {"stage_fused": ku["flux_us"]}
+    elif ku["dt_us"] == 0.0:  # tiled kernels with a gradient pass
         per_launch = {"grad_tile": ku["gradient_us"], "fu_tile": ku["flux_us"]}
     else:
         per_launch = {"dt": ku["dt_us"], "gradient": ku["gradient_us"], "flux": ku["flux_us"],

@@ -66,6 +66,10 @@ typedef struct tsm_params {
      give the reference's summation order. */
   int kernel_path;
   int tile_vertices;  /* owned vertices per tile of the tiled kernels (0: 128) */
+  int block_grad;     /* threads per tile of the tiled gradient kernel (0: auto) */
+  int block_fu;       /* threads per tile of the tiled flux/update kernel (0: auto) */
+  int tile_separate_gradient; /* 1: gradients in their own pass even where the
+                                 mesh allows the fused two-ring stage kernel */
 } tsm_params;
 
 /* NPART = 1 part file content (readme/README.md:155-201), 0-based gids. */
@@ -117,6 +121,13 @@ int tsm_compute_dt(tsm_ctx* ctx, int sample, double* dt_global);
    (optional, n_samples*nsteps*4, sample-major) receives per step
    (iter, dt_global, t, res_total), zero rows after a sample stopped. */
 int tsm_run(tsm_ctx* ctx, long nsteps, long* done, double* log);
+/* tsm_run over several contexts at once (meshes of different resolution, e.g.
+   the levels of a multilevel Monte Carlo run): each context's steps go to its
+   own stream, interleaved, so the levels share the GPU.  done[i] receives the
+   steps summed over the samples of context i (0 once all of them stopped).
+   No counterpart in the reference, which runs one sample at a time
+   (src/fv.cc:2440-2459). */
+int tsm_run_group(tsm_ctx** ctxs, int n, long nsteps, long* done);
 int tsm_time(tsm_ctx* ctx, int sample, double* t, long* iter);
 /* Loop state of one sample: v[0] t, v[1] dt of the last/next step, v[2]
    res_total; iter; counter = index of the next output time stamp
@@ -129,6 +140,29 @@ int tsm_error_norms(tsm_ctx* ctx, int sample, const double* exact_prim, double* 
 /* FiniteVolume::compute_global (src/fv.cc:2175-2225): kinetic energy and entropy. */
 int tsm_global(tsm_ctx* ctx, int sample, double* ke, double* entropy);
 
+/* Monte Carlo reductions over all samples of the context (current states).
+   tsm_moments: per vertex, in part-file order, 5 fields (T, u, v, p, rho):
+   mean, M2 = sum of squared deviations and S2 = sum of squares, accumulated in
+   sample order with the update of FiniteVolume::increment_moments
+   (src/fv.cc:634-690); variance = M2/(n-1), second moment = S2/n
+   (src/reduce_moments.cc:115-141).  Arrays of 5*n_vertex each (may be NULL). */
+int tsm_moments(tsm_ctx* ctx, double* mean, double* m2, double* s2);
+/* Density of every sample at n_probe points (px, py), linearly interpolated in
+   the containing triangle: out[n_samples*n_probe] (sample-major).  Returns
+   TSM_E_ARG if a point lies outside the mesh. */
+int tsm_probe_density(tsm_ctx* ctx, int n_probe, const double* px, const double* py, double* out);
+/* Kinetic energy and entropy integrals (src/fv.cc:2175-2225) of every sample:
+   ke[n_samples], entropy[n_samples] (either may be NULL). */
+int tsm_global_all(tsm_ctx* ctx, double* ke, double* entropy);
+
+/* Sample inputs.  No counterpart in the reference, which reads its random
+   numbers from the rnd_file_loc file (src/fv.cc:502-551); these points feed that
+   role.  Sobol points in Gray-code order with the Joe-Kuo direction numbers,
+   optionally nested-uniform (Owen) scrambled with a per-dimension seed derived
+   from `seed`: out[n*dim], row i = point skip+i in [0,1)^dim.  dim <= 16. */
+int tsm_sobol(long n, int dim, long skip, unsigned long long seed, int scramble, double* out);
+int tsm_sobol_max_dim(void);
+
 /* Geometry export in reference order, for parity checks. */
 int tsm_geometry_sizes(tsm_ctx* ctx, int* n_face, int* n_bface);
 int tsm_geometry(tsm_ctx* ctx, int* face_v, double* face_n, double* dcarea, double* mcarea);
@@ -140,10 +174,13 @@ int tsm_host_geometry(const tsm_mesh* mesh, int cell_type, int* face_v, double* 
                       int* bface_v, double* bface_n, double* dcarea, double* mcarea,
                       int* n_face, int* n_bface);
 
-/* Host-only: builds and bounds-checks the vertex tiling of the tiled kernels.
-   stats[6] = tiles, periodic tiles, max slots, max local faces, halo entries,
-   local face entries. */
-int tsm_host_tiling(const tsm_mesh* mesh, int cell_type, int tile_vertices, int* stats);
+/* Host-only: builds and bounds-checks the vertex tiling of the tiled kernels
+   (two_rings: with the second halo ring of the fused stage kernel, on meshes
+   without periodic sets).  stats[10] = tiles, periodic tiles, max slots, max
+   local faces, first-ring halo entries, local face entries, two rings built,
+   second-ring halo entries, gradient cell entries, max gradient slots. */
+int tsm_host_tiling(const tsm_mesh* mesh, int cell_type, int tile_vertices, int two_rings,
+                    int* stats);
 
 /* Times nsteps full steps on the context stream with HIP events and reports
    average durations in microseconds.  CSR kernels: us[0] dt pass, us[1]
@@ -161,6 +198,35 @@ int tsm_mesh_jittered(int n, double x0, double x1, double y0, double y1,
                       double jitter, unsigned long long seed, int periodic,
                       double* x, double* y, int* bface_tag, int* bface_v,
                       int* cell_v, int* periodic_ptr, int* periodic_v);
+
+
+/* ---------------------------------------------------------------------------
+   Surrogate network (no counterpart in the reference): an MLP
+   widths[0] -> ... -> widths[n_layers] with exact GELU between layers, BF16
+   weights and activations on MFMA with FP32 accumulation, FP32 master weights
+   and Adam (beta1 0.9, beta2 0.999, eps 1e-8).  Loss: mean over rows and
+   outputs of (y - t)^2.  Parameters are flat fp32: W_0 (widths[1] x widths[0],
+   row-major), b_0, W_1, b_1, ...  Inputs and targets are row-major fp32 host
+   arrays.  One context owns one HIP stream.
+   --------------------------------------------------------------------------- */
+typedef struct tsm_mlp tsm_mlp;
+const char* tsm_mlp_last_error(void);
+int tsm_mlp_create(int n_layers, const int* widths, int max_batch, unsigned long long seed,
+                   tsm_mlp** out);
+void tsm_mlp_destroy(tsm_mlp* m);
+long tsm_mlp_n_params(const tsm_mlp* m);
+int tsm_mlp_get_params(tsm_mlp* m, float* params);
+int tsm_mlp_set_params(tsm_mlp* m, const float* params);
+/* y[n x widths[L]] = net(x[n x widths[0]]), in chunks of max_batch rows */
+int tsm_mlp_forward(tsm_mlp* m, long n, const float* x, float* y);
+/* loss and gradient (flat, like the parameters) of one batch, n <= max_batch */
+int tsm_mlp_gradient(tsm_mlp* m, int n, const float* x, const float* t, double* loss, float* grad);
+/* epochs of minibatch Adam over a seeded shuffle; epoch_loss[epochs] optional */
+int tsm_mlp_train(tsm_mlp* m, long n, const float* x, const float* t, int epochs, int batch,
+                  float lr, unsigned long long seed, double* epoch_loss);
+/* C[M x N] (fp32) = A B through the MLP's MFMA GEMM, bf16 bit patterns in */
+int tsm_mlp_gemm_test(int M, int N, int K, const unsigned short* A, int a_kmajor,
+                      const unsigned short* B, int b_kmajor, float* C);
 
 #ifdef __cplusplus
 }

@@ -1202,6 +1202,36 @@ void orc_global(const orc_solver* s, double* ke, double* ent) {
   *ke = K; *ent = S;
 }
 
+// src/fv.cc:595-694 (increment_moments): first sample sets mean = x, S2 = x*x,
+// M2 = 0; sample k >= 2 (1-based) updates primitives with PrimVar::equ
+// (src/primvar.h:213-221): M = x*(1/k) + M*((k-1)/k), and derived fields with
+// M = M + (x - M)/k; S2 += x*x and M2 += (x - M_old)*(x - M_new) for both.
+void orc_moments(int nv, int S, const double* prim, const double* gamma, const double* gas_const,
+                 double* mean, double* m2, double* s2) {
+  (void)gamma;
+  for (int s = 0; s < S; ++s) {
+    const double k = s + 1.0;
+    for (int i = 0; i < nv; ++i) {
+      const double* w = prim + ((size_t)s * nv + i) * 4;
+      const double x[5] = {w[0], w[1], w[2], w[3], w[3] / (gas_const[s] * w[0])};
+      double* M = mean + 5 * (size_t)i;
+      double* Q = m2 + 5 * (size_t)i;
+      double* Z = s2 + 5 * (size_t)i;
+      for (int q = 0; q < 5; ++q) {
+        if (s == 0) {
+          M[q] = x[q]; Z[q] = x[q] * x[q]; Q[q] = 0.0;
+          continue;
+        }
+        const double mo = M[q];
+        const double mn = q < 4 ? x[q] * (1.0 / k) + mo * ((k - 1.0) / k) : mo + (x[q] - mo) / k;
+        M[q] = mn;
+        Z[q] += x[q] * x[q];
+        Q[q] += (x[q] - mo) * (x[q] - mn);
+      }
+    }
+  }
+}
+
 double orc_logavg(double a, double b) { return logavg(a, b); }
 
 void orc_num_flux(const orc_params* p, const double* xl, const double* xr, const double* Wl,

@@ -92,6 +92,11 @@ void orc_error_norms(const orc_solver* s, const double* exact_prim, double* out)
 /* global KE and entropy integrals */
 void orc_global(const orc_solver* s, double* ke, double* ent);
 
+/* Monte Carlo moments of S sample states (prim: S x nv x 4, sample-major),
+   fields (T,u,v,p,rho), in sample order: mean, M2, S2 each nv x 5 */
+void orc_moments(int nv, int S, const double* prim, const double* gamma, const double* gas_const,
+                 double* mean, double* m2, double* s2);
+
 /* pointwise pieces */
 double orc_logavg(double a, double b);
 /* primitive (T,u,v,p) left/right, gradients 8 each, coords, normal, ssw */

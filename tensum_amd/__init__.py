@@ -38,7 +38,9 @@ class TsmParams(C.Structure):
                 ("cfl", C.c_double), ("time_step", C.c_double), ("final_time", C.c_double),
                 ("min_residue", C.c_double), ("max_iter", C.c_long),
                 ("n_time_stamps", C.c_int), ("cell_type", C.c_int), ("liou_fix", C.c_int),
-                ("kernel_path", C.c_int), ("tile_vertices", C.c_int)]
+                ("kernel_path", C.c_int), ("tile_vertices", C.c_int),
+                ("block_grad", C.c_int), ("block_fu", C.c_int),
+                ("tile_separate_gradient", C.c_int)]
 
 
 class TsmMesh(C.Structure):
@@ -84,6 +86,9 @@ class Params:
     liou_fix: bool = False
     kernel_path: str = "auto"      # "auto" (tiled where supported) or "csr"
     tile_vertices: int = 0
+    block_grad: int = 0            # threads per tile, gradient kernel (0: auto)
+    block_fu: int = 0              # threads per tile, flux/update kernel (0: auto)
+    separate_gradient: bool = False  # gradient pass of its own even where it could be fused
 
     def to_c(self):
         p = TsmParams()
@@ -103,6 +108,8 @@ class Params:
         p.liou_fix = int(self.liou_fix)
         p.kernel_path = {"auto": 0, "csr": 1}[self.kernel_path]
         p.tile_vertices = self.tile_vertices
+        p.block_grad, p.block_fu = self.block_grad, self.block_fu
+        p.tile_separate_gradient = int(self.separate_gradient)
         return p
 
 
@@ -178,6 +185,11 @@ def lib():
         L.tsm_geometry_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.tsm_geometry.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         L.tsm_bench.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.tsm_sobol.argtypes = [C.c_long, C.c_int, C.c_long, C.c_ulonglong, C.c_int, C.c_void_p]
+        L.tsm_run_group.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_void_p]
+        L.tsm_moments.argtypes = [C.c_void_p] + [C.c_void_p] * 3
+        L.tsm_probe_density.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
+        L.tsm_global_all.argtypes = [C.c_void_p] + [C.c_void_p] * 2
         L.tsm_mesh_jittered_sizes.argtypes = [C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 5
         L.tsm_mesh_jittered.argtypes = ([C.c_int] + [C.c_double] * 5 + [C.c_ulonglong, C.c_int]
                                         + [C.c_void_p] * 7)
@@ -236,14 +248,30 @@ def host_geometry(mesh: Mesh, cell="median"):
                 bface_n=bn.reshape(-1, 2), dcarea=dc, mcarea=mc)
 
 
-def host_tiling(mesh: Mesh, tile_vertices=128, cell="median"):
+def host_tiling(mesh: Mesh, tile_vertices=128, cell="median", two_rings=False):
     """Builds and bounds-checks the tiled kernels' vertex tiling on the host."""
     m, keep = mesh.to_c()
-    st = np.zeros(6, dtype=np.int32)
+    st = np.zeros(10, dtype=np.int32)
     _check(lib().tsm_host_tiling(C.byref(m), {"median": 0, "voronoi": 1}[cell], tile_vertices,
-                                 _dptr(st)))
-    return dict(zip(("tiles", "periodic_tiles", "max_slots", "max_faces", "halo", "faces"),
-                    map(int, st)))
+                                 int(two_rings), _dptr(st)))
+    return dict(zip(("tiles", "periodic_tiles", "max_slots", "max_faces", "halo", "faces",
+                     "two_rings", "halo2", "grad_cells", "max_grad_slots"), map(int, st)))
+
+
+def sobol(n, dim, seed=0, scramble=True, skip=0):
+    """(n, dim) Sobol points in [0,1)^dim, Gray-code order, optionally Owen-scrambled."""
+    out = np.empty((n, dim))
+    _check(lib().tsm_sobol(n, dim, skip, seed, int(scramble), _dptr(out)))
+    return out
+
+
+def run_group(solvers, nsteps):
+    """Advances several Solvers together (one stream each); returns the steps
+    each performed, summed over its samples."""
+    hs = (C.c_void_p * len(solvers))(*[s._h for s in solvers])
+    done = np.zeros(len(solvers), dtype=np.int64)
+    _check(lib().tsm_run_group(hs, len(solvers), nsteps, _dptr(done)))
+    return done
 
 
 class Solver:
@@ -324,6 +352,26 @@ class Solver:
         k, e = C.c_double(), C.c_double()
         _check(lib().tsm_global(self._h, sample, C.byref(k), C.byref(e)))
         return k.value, e.value
+
+    def moments(self):
+        """Per-vertex (mean, M2, S2) of (T, u, v, p, rho) over all samples, each (n_vertex, 5)."""
+        m, q, z = (np.empty((self.nv, 5)) for _ in range(3))
+        _check(lib().tsm_moments(self._h, _dptr(m), _dptr(q), _dptr(z)))
+        return m, q, z
+
+    def probe_density(self, px, py):
+        """Density of every sample at the probe points: (n_samples, n_probe)."""
+        px = np.ascontiguousarray(px, dtype=np.float64)
+        py = np.ascontiguousarray(py, dtype=np.float64)
+        out = np.empty((self.n_samples, px.size))
+        _check(lib().tsm_probe_density(self._h, px.size, _dptr(px), _dptr(py), _dptr(out)))
+        return out
+
+    def global_all(self):
+        """Kinetic energy and entropy integrals of every sample: two (n_samples,) arrays."""
+        k, e = np.empty(self.n_samples), np.empty(self.n_samples)
+        _check(lib().tsm_global_all(self._h, _dptr(k), _dptr(e)))
+        return k, e
 
     def geometry(self):
         nf, nb = C.c_int(), C.c_int()

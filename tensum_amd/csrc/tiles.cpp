@@ -36,10 +36,11 @@ inline uint32_t spread16(uint32_t v) {
 }
 }  // namespace
 
-bool build_tiles(const Geometry& g, int tb, Tiling& T, std::string& msg) {
+bool build_tiles(const Geometry& g, int tb, Tiling& T, std::string& msg, bool ring2) {
   const int nv = g.nv;
   T = Tiling{};
   T.tb = tb;
+  T.fused = ring2 && g.periodic.empty();
   // ---- ownership and numbering -------------------------------------------
   std::vector<int> order;
   order.reserve(nv);
@@ -137,9 +138,33 @@ bool build_tiles(const Geometry& g, int tb, Tiling& T, std::string& msg) {
         for (int j = 0; j < 3; ++j) add_slot(T.perm[g.cell[3 * (size_t)c + j]]);
     for (int v = b0; v < b1; ++v)
       for (int f : vfaces[v]) { add_slot(T.perm[g.face[f].v[0]]); add_slot(T.perm[g.face[f].v[1]]); }
+    T.hbeg.push_back((int)T.halo.size());
+    const int n1 = (int)slots.size();
+    if (T.fused) {
+      // second ring: every cell corner of every first-ring vertex
+      if (t == 0) { T.h2beg.assign(1, 0); T.g1beg.assign(1, 0); }
+      for (int k = nown; k < n1; ++k)
+        for (int c : vcells[slots[k]])
+          for (int j = 0; j < 3; ++j) {
+            const int v = T.perm[g.cell[3 * (size_t)c + j]];
+            if (slot[v] < 0) { slot[v] = (int)slots.size(); slots.push_back(v); T.halo2.push_back(v); }
+          }
+      T.h2beg.push_back((int)T.halo2.size());
+      T.max_slots1 = std::max(T.max_slots1, n1);
+      for (int k = 0; k < n1; ++k) {
+        T.g1ptr.push_back((int)T.g1ent.size());
+        for (int c : vcells[slots[k]]) {
+          uint32_t a = slot[T.perm[g.cell[3 * (size_t)c]]], bb = slot[T.perm[g.cell[3 * (size_t)c + 1]]],
+                   d = slot[T.perm[g.cell[3 * (size_t)c + 2]]];
+          T.g1ent.push_back(a | (bb << 10) | (d << 20) | ((uint32_t)ccorr[c] << 30));
+          T.g1cgid.push_back(c);
+        }
+      }
+      T.g1ptr.push_back((int)T.g1ent.size());
+      T.g1beg.push_back((int)T.g1ptr.size());
+    }
     if ((int)slots.size() > 1023) { msg = "tile exceeds 1023 slots"; return false; }
     T.max_slots = std::max(T.max_slots, (int)slots.size());
-    T.hbeg.push_back((int)T.halo.size());
     // local faces in reference order
     std::vector<int> lf;
     for (int v = b0; v < b1; ++v)
@@ -245,19 +270,43 @@ bool validate_tiles(const Geometry& g, const Tiling& T, std::string& msg) {
         if (!T.tper[t] || first + cnt > nown) { msg = "periodic slots"; return false; }
       }
     }
+    if (T.fused) {
+      const int n1 = ns;  // owned + first ring
+      const int ns2 = n1 + T.h2beg[t + 1] - T.h2beg[t];
+      if (n1 > T.max_slots1 || ns2 > T.max_slots || ns2 > 1023) { msg = "ring-2 slot count"; return false; }
+      for (int h = T.h2beg[t]; h < T.h2beg[t + 1]; ++h)
+        if (T.halo2[h] < 0 || T.halo2[h] >= nv) { msg = "ring-2 id"; return false; }
+      if (T.g1beg[t + 1] - T.g1beg[t] != n1 + 1) { msg = "gradient slot pointers"; return false; }
+      for (int k = T.g1beg[t]; k < T.g1beg[t + 1] - 1; ++k) {
+        if (T.g1ptr[k] > T.g1ptr[k + 1] || T.g1ptr[k + 1] > (int)T.g1ent.size()) {
+          msg = "gradient cell range"; return false;
+        }
+        for (int e = T.g1ptr[k]; e < T.g1ptr[k + 1]; ++e) {
+          const unsigned c = T.g1ent[e];
+          if ((int)(c & 1023) >= ns2 || (int)((c >> 10) & 1023) >= ns2 || (int)((c >> 20) & 1023) >= ns2) {
+            msg = "gradient cell slot"; return false;
+          }
+          if (T.g1cgid[e] < 0 || T.g1cgid[e] >= g.nc) { msg = "gradient cell id"; return false; }
+        }
+      }
+    }
+  }
+  if (T.fused && ((int)T.h2beg.size() != T.ntiles + 1 || (int)T.g1beg.size() != T.ntiles + 1)) {
+    msg = "ring-2 tables"; return false;
   }
   return true;
 }
 
 }  // namespace tsm
 
-extern "C" int tsm_host_tiling(const tsm_mesh* mesh, int cell_type, int tile_vertices, int* stats) {
+extern "C" int tsm_host_tiling(const tsm_mesh* mesh, int cell_type, int tile_vertices, int two_rings,
+                               int* stats) {
   if (!mesh || !stats) return TSM_E_ARG;
   tsm::Geometry g;
   std::string msg;
   if (!tsm::build_geometry(*mesh, cell_type, g, msg)) return TSM_E_MESH;
   tsm::Tiling T;
-  if (!tsm::build_tiles(g, tile_vertices, T, msg)) return TSM_E_MESH;
+  if (!tsm::build_tiles(g, tile_vertices, T, msg, two_rings != 0)) return TSM_E_MESH;
   if (!tsm::validate_tiles(g, T, msg)) return TSM_E_MESH;
   stats[0] = T.ntiles;
   stats[1] = T.n_periodic_tiles;
@@ -265,5 +314,9 @@ extern "C" int tsm_host_tiling(const tsm_mesh* mesh, int cell_type, int tile_ver
   stats[3] = T.max_faces;
   stats[4] = (int)T.halo.size();
   stats[5] = (int)T.f_lr.size();
+  stats[6] = T.fused ? 1 : 0;
+  stats[7] = (int)T.halo2.size();
+  stats[8] = (int)T.g1ent.size();
+  stats[9] = T.max_slots1;
   return TSM_OK;
 }

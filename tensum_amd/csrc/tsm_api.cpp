@@ -155,7 +155,9 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   if (c->tiled) {
     int tb = params->tile_vertices > 0 ? params->tile_vertices : 128;
     if (tb > 256) { delete c; return fail(TSM_E_ARG, "tile_vertices must be <= 256"); }
-    if (!tsm::build_tiles(g, tb, c->tiling, msg) || !tsm::validate_tiles(g, c->tiling, msg)) {
+    const bool grad = (params->flux == 0 || params->flux == 1) && params->reconstruct != 0;
+    const bool ring2 = grad && !params->tile_separate_gradient;
+    if (!tsm::build_tiles(g, tb, c->tiling, msg, ring2) || !tsm::validate_tiles(g, c->tiling, msg)) {
       delete c;
       return fail(TSM_E_MESH, "tiling: " + msg);
     }
@@ -252,10 +254,28 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     const tsm::Tiling& T = c->tiling;
     tsm::TileGeo& TG = c->TG;
     TG.ntiles = T.ntiles; TG.tb = T.tb; TG.max_slots = T.max_slots; TG.max_faces = T.max_faces;
+    TG.fused = T.fused ? 1 : 0;
+    TG.max_slots1 = T.max_slots1;
+    TG.h2beg = TG.halo2 = TG.g1beg = TG.g1ptr = TG.g1cgid = nullptr;
+    TG.g1ent = nullptr;
+    TG.grad_block = std::max(0, params->block_grad);
+    TG.fu_block = std::max(0, params->block_fu);
+    const int bf = tsm::tile_fu_block(TG), bg = tsm::tile_grad_block(TG);
+    if (bf > tsm::tile_fu_block_max() || bf % 64 || bg > 512 || bg % 64 ||
+        tsm::tile_fu_lds(TG, params->flux, params->reconstruct) > 160 * 1024 ||
+        tsm::tile_grad_lds(TG) > 160 * 1024) {
+      delete c;
+      return fail(TSM_E_ARG, "tiled kernels: block size or LDS out of range (blocks " +
+                                 std::to_string(bg) + "/" + std::to_string(bf) + ")");
+    }
 #define UT(field, vec) do { if ((rc = c->upload(&TG.field, vec))) { delete c; return rc; } } while (0)
     UT(vbeg, T.vbeg); UT(hbeg, T.hbeg); UT(halo, T.halo); UT(fbeg, T.fbeg); UT(bbeg, T.bbeg);
     UT(tbf, T.tbf); UT(gptr, T.gptr); UT(cptr, T.cptr); UT(cgid, T.cgid); UT(tper, T.tper);
     UT(pinfo, T.pinfo); UT(f_lr, T.f_lr); UT(cent, T.cent); UT(f_n, T.f_n); UT(gent, T.gent);
+    if (T.fused) {
+      UT(h2beg, T.h2beg); UT(halo2, T.halo2); UT(g1beg, T.g1beg); UT(g1ptr, T.g1ptr);
+      UT(g1cgid, T.g1cgid); UT(g1ent, T.g1ent);
+    }
 #undef UT
   }
   tsm::StageArgs& A = c->A;
@@ -323,15 +343,14 @@ extern "C" int tsm_set_state(tsm_ctx* c, int s, const double* prim) {
     if (!(prim[4 * i] > 0.0) || !(prim[4 * i + 3] > 0.0))
       return fail(TSM_E_STATE, "initial temperature and pressure must be positive (vertex " +
                                    std::to_string(i) + ")");
-  std::vector<double> soa(4 * (size_t)nv);
+  std::vector<double> aos(4 * (size_t)nv);
   for (int i = 0; i < nv; ++i)
-    for (int k = 0; k < 4; ++k) soa[(size_t)k * nv + c->perm[i]] = prim[4 * i + k];
-  const size_t cs = c->A.cs, off = (size_t)s * nv;
+    for (int k = 0; k < 4; ++k) aos[4 * (size_t)c->perm[i] + k] = prim[4 * i + k];
+  const size_t off = (size_t)s * nv;
   HIP_TRY(hipStreamSynchronize(c->st));
   c->csr_ready();
-  for (int k = 0; k < 4; ++k)
-    HIP_TRY(hipMemcpy(c->W[c->cur] + k * cs + off, soa.data() + (size_t)k * nv,
-                      nv * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->W[c->cur] + 4 * off, aos.data(), aos.size() * sizeof(double),
+                    hipMemcpyHostToDevice));
   c->bind_state();
   tsm::launch_prim2ent(c->G, c->A, s, c->st);
   HIP_TRY(hipGetLastError());
@@ -343,16 +362,16 @@ extern "C" int tsm_set_state(tsm_ctx* c, int s, const double* prim) {
 }
 
 namespace {
+// Vertex arrays are array-of-structs with ncomp doubles per vertex.
 int download_vec(tsm_ctx* c, const double* base, int ncomp, int s, double* out) {
   const int nv = c->G.nv;
-  const size_t cs = c->A.cs, off = (size_t)s * nv;
-  std::vector<double> soa((size_t)ncomp * nv);
+  const size_t off = (size_t)s * nv;
+  std::vector<double> aos((size_t)ncomp * nv);
   HIP_TRY(hipStreamSynchronize(c->st));
-  for (int k = 0; k < ncomp; ++k)
-    HIP_TRY(hipMemcpy(soa.data() + (size_t)k * nv, base + k * cs + off, nv * sizeof(double),
-                      hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(aos.data(), base + (size_t)ncomp * off, aos.size() * sizeof(double),
+                    hipMemcpyDeviceToHost));
   for (int i = 0; i < nv; ++i)
-    for (int k = 0; k < ncomp; ++k) out[(size_t)ncomp * i + k] = soa[(size_t)k * nv + c->perm[i]];
+    for (int k = 0; k < ncomp; ++k) out[(size_t)ncomp * i + k] = aos[(size_t)ncomp * c->perm[i] + k];
   return TSM_OK;
 }
 }  // namespace
@@ -432,7 +451,8 @@ void enqueue_tiled_step(tsm_ctx* c, int log_slot, hipEvent_t* ev = nullptr) {
     c->A.W_in = ins[r];
     c->A.W_out = outs[r];
     if (ev) (void)hipEventRecord(ev[e++], c->st);
-    tsm::launch_tile_gradient(c->G, c->TG, c->A, c->S, c->st);
+    if (tsm::tile_needs_gradient_pass(c->TG, c->P.flux, c->P.reconstruct))
+      tsm::launch_tile_gradient(c->G, c->TG, c->A, c->S, c->st);
     if (ev) (void)hipEventRecord(ev[e++], c->st);
     tsm::launch_tile_stage(c->P.flux, c->P.reconstruct, c->G, c->TG, c->A, c->SP, Wn, c->dtpart,
                            c->S, r, c->st);
@@ -490,6 +510,46 @@ extern "C" int tsm_run(tsm_ctx* c, long nsteps, long* done, double* log) {
     }
   }
   c->A.log = (log ? c->A.log : c->A.log);
+  return TSM_OK;
+}
+
+extern "C" int tsm_run_group(tsm_ctx** cs, int n, long nsteps, long* done) {
+  if (!cs || n < 1 || nsteps < 0) return fail(TSM_E_ARG, "bad argument");
+  std::vector<std::vector<tsm::Ctl>> before(n);
+  for (int i = 0; i < n; ++i) {
+    tsm_ctx* c = cs[i];
+    if (!c) return fail(TSM_E_ARG, "null context in group");
+    before[i].resize(c->S);
+    HIP_TRY(hipStreamSynchronize(c->st));
+    HIP_TRY(hipMemcpy(before[i].data(), c->A.ctl, c->S * sizeof(tsm::Ctl), hipMemcpyDeviceToHost));
+  }
+  // Interleave the contexts' launches in chunks so every stream has work queued
+  // while the others run: levels of different sizes share the GPU.
+  const long chunk = 8;
+  for (long k0 = 0; k0 < nsteps; k0 += chunk) {
+    const long m = std::min(chunk, nsteps - k0);
+    for (int i = 0; i < n; ++i) {
+      tsm_ctx* c = cs[i];
+      if (c->tiled) {
+        c->bind_state();
+        if (k0 == 0) tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
+        for (long k = 0; k < m; ++k) enqueue_tiled_step(c, -1);
+      } else {
+        c->csr_ready();
+        for (long k = 0; k < m; ++k) enqueue_step(c, -1);
+      }
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  for (int i = 0; i < n; ++i) {
+    tsm_ctx* c = cs[i];
+    std::vector<tsm::Ctl> after(c->S);
+    HIP_TRY(hipStreamSynchronize(c->st));
+    HIP_TRY(hipMemcpy(after.data(), c->A.ctl, c->S * sizeof(tsm::Ctl), hipMemcpyDeviceToHost));
+    long tot = 0;
+    for (int s = 0; s < c->S; ++s) tot += (long)(after[s].iter - before[i][s].iter);
+    if (done) done[i] = tot;
+  }
   return TSM_OK;
 }
 
@@ -666,5 +726,97 @@ extern "C" int tsm_bench(tsm_ctx* c, int nsteps, double* us) {
   HIP_TRY(hipEventElapsedTime(&ms, ev[14], ev[15]));
   us[4] = 1e3 * ms / nsteps;
   for (auto& e : ev) (void)hipEventDestroy(e);
+  return TSM_OK;
+}
+
+extern "C" int tsm_moments(tsm_ctx* c, double* mean, double* m2, double* s2) {
+  if (!c) return fail(TSM_E_ARG, "bad argument");
+  const int nv = c->G.nv;
+  double *dm = nullptr, *dq = nullptr, *dz = nullptr;
+  const size_t n = 5 * (size_t)nv;
+  HIP_TRY(hipMalloc(&dm, 3 * n * sizeof(double)));
+  dq = dm + n;
+  dz = dq + n;
+  c->bind_state();
+  tsm::launch_moments(c->G, c->A, c->S, dm, dq, dz, c->st);
+  hipError_t e = hipGetLastError();
+  std::vector<double> h(3 * n);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), dm, h.size() * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(dm);
+  if (e != hipSuccess) return fail(TSM_E_HIP, hipGetErrorString(e));
+  double* outs[3] = {mean, m2, s2};
+  for (int a = 0; a < 3; ++a) {
+    if (!outs[a]) continue;
+    for (int i = 0; i < nv; ++i)
+      for (int q = 0; q < 5; ++q) outs[a][5 * (size_t)i + q] = h[a * n + 5 * (size_t)c->perm[i] + q];
+  }
+  return TSM_OK;
+}
+
+extern "C" int tsm_probe_density(tsm_ctx* c, int np, const double* px, const double* py,
+                                 double* out) {
+  if (!c || np < 1 || !px || !py || !out) return fail(TSM_E_ARG, "bad argument");
+  const auto& g = c->geo;
+  std::vector<int> pc(3 * (size_t)np);
+  std::vector<double> pw(3 * (size_t)np);
+  for (int p = 0; p < np; ++p) {  // brute-force point location (once per call)
+    int found = -1;
+    double best = -1e300, wb[3] = {0, 0, 0};
+    for (int k = 0; k < g.nc && found < 0; ++k) {
+      const int* v = &g.cell[3 * (size_t)k];
+      double x0 = g.x[v[0]], y0 = g.y[v[0]], x1 = g.x[v[1]], y1 = g.y[v[1]], x2 = g.x[v[2]], y2 = g.y[v[2]];
+      double det = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0);
+      double l1 = ((px[p] - x0) * (y2 - y0) - (x2 - x0) * (py[p] - y0)) / det;
+      double l2 = ((x1 - x0) * (py[p] - y0) - (px[p] - x0) * (y1 - y0)) / det;
+      double l0 = 1.0 - l1 - l2;
+      double mn = std::fmin(l0, std::fmin(l1, l2));
+      if (mn >= -1e-12) { found = k; wb[0] = l0; wb[1] = l1; wb[2] = l2; }
+      else if (mn > best) best = mn;
+    }
+    if (found < 0) return fail(TSM_E_ARG, "probe point " + std::to_string(p) + " outside the mesh");
+    for (int j = 0; j < 3; ++j) {
+      pc[3 * (size_t)p + j] = c->perm[g.cell[3 * (size_t)found + j]];
+      pw[3 * (size_t)p + j] = wb[j];
+    }
+  }
+  int* dpc = nullptr;
+  double *dpw = nullptr, *dout = nullptr;
+  HIP_TRY(hipMalloc(&dpc, pc.size() * sizeof(int)));
+  hipError_t e = hipMalloc(&dpw, pw.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dout, (size_t)np * c->S * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(dpc, pc.data(), pc.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dpw, pw.data(), pw.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    c->bind_state();
+    tsm::launch_probes(c->A, c->G.nv, np, dpc, dpw, c->S, dout, c->st);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)np * c->S * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(dpc); (void)hipFree(dpw); (void)hipFree(dout);
+  if (e != hipSuccess) return fail(TSM_E_HIP, hipGetErrorString(e));
+  return TSM_OK;
+}
+
+extern "C" int tsm_global_all(tsm_ctx* c, double* ke, double* ent) {
+  if (!c) return fail(TSM_E_ARG, "bad argument");
+  const int nb = c->nblk_v, S = c->S;
+  double* d = nullptr;
+  HIP_TRY(hipMalloc(&d, (size_t)nb * S * 2 * sizeof(double)));
+  c->bind_state();
+  tsm::launch_global_all(c->G, c->A, S, d, c->st);
+  hipError_t e = hipGetLastError();
+  std::vector<double> h((size_t)nb * S * 2);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(TSM_E_HIP, hipGetErrorString(e));
+  for (int s = 0; s < S; ++s) {
+    double K = 0, E = 0;
+    for (int b = 0; b < nb; ++b) { K += h[((size_t)s * nb + b) * 2]; E += h[((size_t)s * nb + b) * 2 + 1]; }
+    if (ke) ke[s] = K;
+    if (ent) ent[s] = E;
+  }
   return TSM_OK;
 }

@@ -43,6 +43,10 @@ struct DevGeo {
 // Tile arrays of the LDS-staged kernels (device pointers, new vertex ids).
 struct TileGeo {
   int ntiles, tb, max_slots, max_faces;
+  int grad_block, fu_block;  // requested threads per tile (0: smallest that fits)
+  int fused, max_slots1;     // two-ring tiles: gradients formed inside the stage kernel
+  const int *h2beg, *halo2, *g1beg, *g1ptr, *g1cgid;
+  const unsigned* g1ent;
   const int *vbeg, *hbeg, *halo, *fbeg, *bbeg, *tbf, *gptr, *cptr, *cgid, *tper, *pinfo;
   const unsigned *f_lr, *cent;
   const double* f_n;
@@ -61,8 +65,9 @@ struct StepParams {
   int n_stamps;
 };
 
-// Mutable state and scratch (device pointers).  Components are strided by
-// cs = n_samples*nv (vertex arrays) and fs = n_samples*nf (face arrays).
+// Mutable state and scratch (device pointers).  Vertex arrays are array-of-
+// structs (4 or 8 doubles per vertex, sample-major, cs = n_samples*nv
+// vertices); face arrays are structure-of-arrays strided by fs = n_samples*nf.
 struct StageArgs {
   size_t cs, fs, off;
   const double* W_in; double* W_out;
@@ -87,6 +92,10 @@ struct StageArgs {
 
 // LDS-staged tile kernels (Euler, SSP-RK3)
 size_t tile_grad_lds(const TileGeo& T);
+int tile_grad_block(const TileGeo& T);
+int tile_fu_block(const TileGeo& T);
+int tile_fu_block_max();
+bool tile_needs_gradient_pass(const TileGeo& T, int flux, int rec);
 size_t tile_fu_lds(const TileGeo& T, int flux, int rec);
 void launch_tile_gradient(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
                           hipStream_t st);
@@ -107,5 +116,11 @@ void launch_prim2ent(const DevGeo& G, const StageArgs& A, int s, hipStream_t st)
 void launch_global(const DevGeo& G, const StageArgs& A, int s, double* out, hipStream_t st);
 void launch_errnorm(const DevGeo& G, const StageArgs& A, int s, const double* ex, double* out,
                     hipStream_t st);
+
+void launch_moments(const DevGeo& G, const StageArgs& A, int S, double* mean, double* m2,
+                    double* s2, hipStream_t st);
+void launch_probes(const StageArgs& A, int nv, int np, const int* pc, const double* pw, int S,
+                   double* out, hipStream_t st);
+void launch_global_all(const DevGeo& G, const StageArgs& A, int S, double* out, hipStream_t st);
 
 }  // namespace tsm

@@ -63,9 +63,20 @@ struct Tiling {
   std::vector<int> cgid;             // global cell id per cell entry
   std::vector<int> tper;             // tile holds periodic sets
   std::vector<int> pinfo;            // per new vertex: first member slot << 8 | set size, or -1
+  // Second halo ring (meshes without periodic sets): the stage kernel then
+  // forms the gradients of the owned and first-ring slots itself.  Slots are
+  // owned, first ring (halo), second ring (halo2); the gradient slots are the
+  // first n1 = n_own + first-ring slots.
+  bool fused = false;
+  int max_slots1 = 0;                // largest n1
+  std::vector<int> h2beg, halo2;     // second-ring slots
+  std::vector<int> g1beg;            // ntiles+1: start of the tile's n1+1 entries in g1ptr
+  std::vector<int> g1ptr;            // per gradient slot: cell entries [g1ptr[k], g1ptr[k+1])
+  std::vector<uint32_t> g1ent;       // 3 x 10-bit slots | correction flag << 30
+  std::vector<int> g1cgid;           // global cell id per entry
 };
 
-bool build_tiles(const Geometry& g, int tb, Tiling& T, std::string& msg);
+bool build_tiles(const Geometry& g, int tb, Tiling& T, std::string& msg, bool ring2 = false);
 bool validate_tiles(const Geometry& g, const Tiling& T, std::string& msg);
 
 // Builds faces, normals and areas exactly as Grid::preproc does for one

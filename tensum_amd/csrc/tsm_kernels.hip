@@ -18,17 +18,21 @@ namespace tsm {
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ Prim load_prim(const double* W, size_t cs, size_t i) {
-  return Prim{W[i], W[cs + i], W[2 * cs + i], W[3 * cs + i]};
+// Vertex arrays are array-of-structs: 4 doubles (W, E, U_old, R) or 8 doubles
+// (dE) per vertex, so one halo gather touches one or two cache lines.
+__device__ __forceinline__ Prim load_prim(const double* W, size_t i) {
+  const double4 q = reinterpret_cast<const double4*>(W)[i];
+  return Prim{q.x, q.y, q.z, q.w};
 }
-__device__ __forceinline__ void store_prim(double* W, size_t cs, size_t i, const Prim& w) {
-  W[i] = w.T; W[cs + i] = w.u; W[2 * cs + i] = w.v; W[3 * cs + i] = w.p;
+__device__ __forceinline__ void store_prim(double* W, size_t i, const Prim& w) {
+  reinterpret_cast<double4*>(W)[i] = make_double4(w.T, w.u, w.v, w.p);
 }
-__device__ __forceinline__ Ent load_ent(const double* E, size_t cs, size_t i) {
-  return Ent{E[i], E[cs + i], E[2 * cs + i], E[3 * cs + i]};
+__device__ __forceinline__ Ent load_ent(const double* E, size_t i) {
+  const double4 q = reinterpret_cast<const double4*>(E)[i];
+  return Ent{q.x, q.y, q.z, q.w};
 }
-__device__ __forceinline__ void store_ent(double* E, size_t cs, size_t i, const Ent& e) {
-  E[i] = e.f; E[cs + i] = e.vx; E[2 * cs + i] = e.vy; E[3 * cs + i] = e.l;
+__device__ __forceinline__ void store_ent(double* E, size_t i, const Ent& e) {
+  reinterpret_cast<double4*>(E)[i] = make_double4(e.f, e.vx, e.vy, e.l);
 }
 
 // Block-wide deterministic sum / min of NV values per thread (blockDim 256).
@@ -122,15 +126,15 @@ __device__ __forceinline__ void bc_apply2(int type, const double* bs, double nx,
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void cell_gradient(const DevGeo& G, const StageArgs& A, const Gas& gas,
                                               int c, bool corr, double* g) {
-  const size_t cs = A.cs, off = A.off;
+  const size_t off = A.off;
   int a = G.cv[3 * c], b = G.cv[3 * c + 1], d = G.cv[3 * c + 2];
   double xa = G.x[a], ya = G.y[a], xb = G.x[b], yb = G.y[b], xd = G.x[d], yd = G.y[d];
   // cell normals, src/grid_preproc.cc:336-362: N0 = dr12, N1 = dr20, N2 = dr01
   double n0x = (yb - yd), n0y = -(xb - xd);
   double n1x = (yd - ya), n1y = -(xd - xa);
   double n2x = (ya - yb), n2y = -(xa - xb);
-  Ent ea = load_ent(A.E_in, cs, off + a), eb = load_ent(A.E_in, cs, off + b),
-      ed = load_ent(A.E_in, cs, off + d);
+  Ent ea = load_ent(A.E_in, off + a), eb = load_ent(A.E_in, off + b),
+      ed = load_ent(A.E_in, off + d);
   const double qa[4] = {ea.f, ea.vx, ea.vy, ea.l};
   const double qb[4] = {eb.f, eb.vx, eb.vy, eb.l};
   const double qd[4] = {ed.f, ed.vx, ed.vy, ed.l};
@@ -146,11 +150,11 @@ __device__ __forceinline__ void cell_gradient(const DevGeo& G, const StageArgs& 
       int v0 = G.bv[2 * bf], v1 = G.bv[2 * bf + 1];
       double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
       int type = G.btype[bf];
-      Ent E0 = load_ent(A.E_in, cs, off + v0), E1 = load_ent(A.E_in, cs, off + v1);
+      Ent E0 = load_ent(A.E_in, off + v0), E1 = load_ent(A.E_in, off + v1);
       Ent q0 = E0, q1 = E1;
       if (type == TSM_BC_SLIP || type == TSM_BC_NOSLIP || type == TSM_BC_PRESSURE ||
           type == TSM_BC_INLET) {
-        Prim s0 = load_prim(A.W_in, cs, off + v0), s1 = load_prim(A.W_in, cs, off + v1);
+        Prim s0 = load_prim(A.W_in, off + v0), s1 = load_prim(A.W_in, off + v1);
         bc_apply1(type, G.bstate + 4 * bf, nx, ny, s0);
         bc_apply1(type, G.bstate + 4 * bf, nx, ny, s1);
         q0 = prim2ent(gas, s0);
@@ -175,7 +179,6 @@ __global__ __launch_bounds__(256) void k_gradient(DevGeo G, StageArgs A) {
   if (!A.ignore_done && A.ctl[s].done) return;
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= G.nv) return;
-  const size_t cs = A.cs;
   StageArgs As = A;
   As.off = (size_t)s * G.nv;
   Gas gas{A.gamma[s], A.gasR[s]};
@@ -203,9 +206,9 @@ __global__ __launch_bounds__(256) void k_gradient(DevGeo G, StageArgs A) {
     }
   }
   double fact = G.gfact[v];
-  double* dE = A.dE + As.off + v;
+  double* dE = A.dE + 8 * (As.off + v);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) dE[(size_t)k * cs] = tot[k] * fact;
+  for (int k = 0; k < 8; ++k) dE[k] = tot[k] * fact;
 }
 
 // ---------------------------------------------------------------------------
@@ -217,22 +220,22 @@ __global__ __launch_bounds__(256) void k_flux(DevGeo G, StageArgs A) {
   if (!A.ignore_done && A.ctl[s].done) return;
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= G.nf) return;
-  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  const size_t off = (size_t)s * G.nv;
   Gas gas{A.gamma[s], A.gasR[s]};
   int l = G.fv[2 * f], r = G.fv[2 * f + 1];
   double nx = G.fn[2 * f], ny = G.fn[2 * f + 1];
   double area = sqrt(nx * nx + ny * ny);
   double ia = 1.0 / area;
   double unx = nx * ia, uny = ny * ia;
-  Prim Wl = load_prim(A.W_in, cs, off + l), Wr = load_prim(A.W_in, cs, off + r);
+  Prim Wl = load_prim(A.W_in, off + l), Wr = load_prim(A.W_in, off + r);
   Cons F;
   if (FLUX == F_ROE || FLUX == F_RUS) {
     double gl[8], gr[8];
     if (REC != R_FIRST) {
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        gl[k] = A.dE[(size_t)k * cs + off + l];
-        gr[k] = A.dE[(size_t)k * cs + off + r];
+        gl[k] = A.dE[8 * (off + l) + k];
+        gr[k] = A.dE[8 * (off + r) + k];
       }
     }
     double rx = G.x[r] - G.x[l], ry = G.y[r] - G.y[l];
@@ -292,10 +295,10 @@ __device__ __forceinline__ Cons bface_flux(const DevGeo& G, const StageArgs& A, 
 // (src/fv.cc:1197-1232, src/material.cc:158-230).
 __device__ __forceinline__ Cons viscous_part(const DevGeo& G, const StageArgs& A, const Gas& gas,
                                              int c, int j) {
-  const size_t cs = A.cs, off = A.off;
+  const size_t off = A.off;
   int a = G.cv[3 * c], b = G.cv[3 * c + 1], d = G.cv[3 * c + 2];
-  Ent ea = load_ent(A.E_in, cs, off + a), eb = load_ent(A.E_in, cs, off + b),
-      ed = load_ent(A.E_in, cs, off + d);
+  Ent ea = load_ent(A.E_in, off + a), eb = load_ent(A.E_in, off + b),
+      ed = load_ent(A.E_in, off + d);
   const double fact = 1.0 / 3.0;
   Ent s;
   s.f = ea.f * fact + eb.f * fact + ed.f * fact;
@@ -357,15 +360,12 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
   __shared__ double lds[4 * 4];
   const int s = blockIdx.y;
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  const size_t off = (size_t)s * G.nv;
   if (!A.ignore_done && A.ctl[s].done) {
     // a finished sample keeps its state across the buffer swap
     if (MODE != 0 && v < G.nv) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        A.W_out[(size_t)k * cs + off + v] = A.W_in[(size_t)k * cs + off + v];
-        A.E_out[(size_t)k * cs + off + v] = A.E_in[(size_t)k * cs + off + v];
-      }
+      store_prim(A.W_out, off + v, load_prim(A.W_in, off + v));
+      store_ent(A.E_out, off + v, load_ent(A.E_in, off + v));
     }
     return;
   }
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
           if (kind == 0) { acc[0] += f0; acc[1] += f1; acc[2] += f2; acc[3] += f3; }
           else { acc[0] -= f0; acc[1] -= f1; acc[2] -= f2; acc[3] -= f3; }
         } else {
-          if (!have_wm) { wm = load_prim(A.W_in, cs, off + m); have_wm = true; }
+          if (!have_wm) { wm = load_prim(A.W_in, off + m); have_wm = true; }
           Cons Fb = bface_flux<FLUX, REC>(G, A, gas, (int)idx, wm, G.x[m], G.y[m]);
           acc[0] += Fb.m * 0.5; acc[1] += Fb.mx * 0.5; acc[2] += Fb.my * 0.5; acc[3] += Fb.e * 0.5;
         }
@@ -416,9 +416,9 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
       }
     }
     if (MODE == 0) {
-      double* R = A.R + off + v;
+      double* R = A.R + 4 * (off + v);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) R[(size_t)k * cs] = tot[k];
+      for (int k = 0; k < 4; ++k) R[k] = tot[k];
     } else {
       double area = G.dcarea[v];
       if (last) {
@@ -431,9 +431,9 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
       }
       double dt = A.ctl[s].dt;
       double factor = dt / (1.0 * area);
-      Prim w0 = load_prim(A.W_in, cs, off + v);
+      Prim w0 = load_prim(A.W_in, off + v);
       Cons U;
-      const double* Uo = A.Uold + off + v;
+      const double* Uo = A.Uold + 4 * (off + v);
       if (MODE == 1) {
         const double a_rk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
         const double b_rk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
@@ -441,29 +441,29 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
         double b = b_rk[r], a = a_rk[r], c = -factor * b_rk[r];
         U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
         U.m += Uo[0] * a + tot[0] * c;
-        U.mx += Uo[cs] * a + tot[1] * c;
-        U.my += Uo[2 * cs] * a + tot[2] * c;
-        U.e += Uo[3 * cs] * a + tot[3] * c;
+        U.mx += Uo[1] * a + tot[1] * c;
+        U.my += Uo[2] * a + tot[2] * c;
+        U.e += Uo[3] * a + tot[3] * c;
       } else {
         const double rks[4] = {1.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
-        double* R2 = A.R2 + off + v;
+        double* R2 = A.R2 + 4 * (off + v);
         double ir = 1.0 / rks[r];
         double q[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          double r2 = (r == 0 ? 0.0 : R2[(size_t)k * cs]) + tot[k] * ir;
-          R2[(size_t)k * cs] = r2;
+          double r2 = (r == 0 ? 0.0 : R2[k]) + tot[k] * ir;
+          R2[k] = r2;
           q[k] = (r < 3) ? tot[k] : r2;
         }
         double c = (r < 3) ? -factor * rks[r < 3 ? r + 1 : 3] : -factor / 6.0;
         U.m = Uo[0] * 1.0 + q[0] * c;
-        U.mx = Uo[cs] * 1.0 + q[1] * c;
-        U.my = Uo[2 * cs] * 1.0 + q[2] * c;
-        U.e = Uo[3 * cs] * 1.0 + q[3] * c;
+        U.mx = Uo[1] * 1.0 + q[1] * c;
+        U.my = Uo[2] * 1.0 + q[2] * c;
+        U.e = Uo[3] * 1.0 + q[3] * c;
       }
       Prim w1 = con2prim(gas, U);
-      store_prim(A.W_out, cs, off + v, w1);
-      store_ent(A.E_out, cs, off + v, prim2ent(gas, w1));
+      store_prim(A.W_out, off + v, w1);
+      store_ent(A.E_out, off + v, prim2ent(gas, w1));
     }
   }
   if (MODE != 0 && last) {
@@ -492,14 +492,13 @@ __global__ __launch_bounds__(256) void k_dt(DevGeo G, StageArgs A, StepParams P)
     return;
   }
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  const size_t off = (size_t)s * G.nv;
   Gas gas{A.gamma[s], A.gasR[s]};
   double dtv = 1.0e20;
   if (v < G.nv) {
-    Prim w = load_prim(A.W_in, cs, off + v);
+    Prim w = load_prim(A.W_in, off + v);
     Cons U = prim2con(gas, w);
-    double* Uo = A.Uold + off + v;
-    Uo[0] = U.m; Uo[cs] = U.mx; Uo[2 * cs] = U.my; Uo[3 * cs] = U.e;
+    reinterpret_cast<double4*>(A.Uold)[off + v] = make_double4(U.m, U.mx, U.my, U.e);
     if (P.time_step > 0.0) {
       dtv = P.time_step;
     } else {
@@ -509,7 +508,7 @@ __global__ __launch_bounds__(256) void k_dt(DevGeo G, StageArgs A, StepParams P)
       double tot = 0.0;
       for (int mi = m0; mi < m1; ++mi) {
         int m = (mi < 0) ? v : G.pv[mi];
-        Prim wm = (m == v) ? w : load_prim(A.W_in, cs, off + m);
+        Prim wm = (m == v) ? w : load_prim(A.W_in, off + m);
         double cm = sound_speed(gas, wm);
         double lam = 0.0;
         for (int e = G.vfe_ptr[m]; e < G.vfe_ptr[m + 1]; ++e) {
@@ -623,11 +622,11 @@ __global__ __launch_bounds__(256) void k_ssw(DevGeo G, StageArgs A) {
   if (!A.ignore_done && A.ctl[s].done) return;
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= G.nf) return;
-  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  const size_t off = (size_t)s * G.nv;
   Gas gas{A.gamma[s], A.gasR[s]};
   int l = G.fv[2 * f], r = G.fv[2 * f + 1];
   double nx = G.fn[2 * f], ny = G.fn[2 * f + 1];
-  Prim wl = load_prim(A.W_in, cs, off + l), wr = load_prim(A.W_in, cs, off + r);
+  Prim wl = load_prim(A.W_in, off + l), wr = load_prim(A.W_in, off + r);
   double ui = wl.u * nx + wl.v * ny, uj = wr.u * nx + wr.v * ny;
   double ai = sound_speed(gas, wl), aj = sound_speed(gas, wr);
   double area = sqrt(nx * nx + ny * ny);
@@ -643,21 +642,21 @@ __global__ __launch_bounds__(256) void k_ssw(DevGeo G, StageArgs A) {
 __global__ void k_prim2ent(DevGeo G, StageArgs A, int s) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= G.nv) return;
-  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  const size_t off = (size_t)s * G.nv;
   Gas gas{A.gamma[s], A.gasR[s]};
-  Prim w = load_prim(A.W_in, cs, off + v);
-  store_ent(A.E_in, cs, off + v, prim2ent(gas, w));
+  Prim w = load_prim(A.W_in, off + v);
+  store_ent(A.E_in, off + v, prim2ent(gas, w));
 }
 
 // Global integrals (src/fv.cc:2175-2225) and error norms (src/fv.cc:1712-1795)
 __global__ __launch_bounds__(256) void k_global(DevGeo G, StageArgs A, int s, double* out) {
   __shared__ double lds[4 * 2];
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  const size_t off = (size_t)s * G.nv;
   Gas gas{A.gamma[s], A.gasR[s]};
   double q[2] = {0, 0};
   if (v < G.nv) {
-    Prim w = load_prim(A.W_in, cs, off + v);
+    Prim w = load_prim(A.W_in, off + v);
     double wt = G.weight[v], da = G.dcarea[v];
     double rho = density(gas, w);
     q[0] = (0.5 * rho * (w.u * w.u + w.v * w.v) * da) * wt;
@@ -671,14 +670,14 @@ __global__ __launch_bounds__(256) void k_errnorm(DevGeo G, StageArgs A, int s, c
                                                  double* out) {
   __shared__ double lds[4 * 24];
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  const size_t off = (size_t)s * G.nv;
   Gas gas{A.gamma[s], A.gasR[s]};
   double q[24];
 #pragma unroll
   for (int k = 0; k < 24; ++k) q[k] = 0.0;
   double mx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (v < G.nv) {
-    Prim w = load_prim(A.W_in, cs, off + v);
+    Prim w = load_prim(A.W_in, off + v);
     Prim pe{ex[4 * v], ex[4 * v + 1], ex[4 * v + 2], ex[4 * v + 3]};
     double aw = G.dcarea[v], wt = G.weight[v];
     double e[8];
@@ -789,6 +788,100 @@ void launch_global(const DevGeo& G, const StageArgs& A, int s, double* out, hipS
 void launch_errnorm(const DevGeo& G, const StageArgs& A, int s, const double* ex, double* out,
                     hipStream_t st) {
   hipLaunchKernelGGL(k_errnorm, dim3((G.nv + 255) / 256), dim3(256), 0, st, G, A, s, ex, out);
+}
+
+}  // namespace tsm
+
+namespace tsm {
+
+// ---------------------------------------------------------------------------
+// Monte Carlo reductions over the sample dimension
+// ---------------------------------------------------------------------------
+// Per-vertex mean, M2 (sum of squared deviations) and sum of squares of
+// (T, u, v, p, rho) over the samples in sample order, with the update of
+// FiniteVolume::increment_moments (src/fv.cc:634-690): M_k = x/k + M_{k-1}(k-1)/k,
+// M2 += (x - M_{k-1})(x - M_k), S2 += x^2; the density mean uses
+// M_k = M_{k-1} + (x - M_{k-1})/k as the reference does for derived fields.
+__global__ __launch_bounds__(256) void k_moments(DevGeo G, StageArgs A, int S, double* mean,
+                                                 double* m2, double* s2) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= G.nv) return;
+  double M[5], Q[5], Z[5];
+  for (int s = 0; s < S; ++s) {
+    const Gas gas{A.gamma[s], A.gasR[s]};
+    const Prim w = load_prim(A.W_in, (size_t)s * G.nv + v);
+    const double x[5] = {w.T, w.u, w.v, w.p, density(gas, w)};
+    if (s == 0) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) { M[q] = x[q]; Z[q] = x[q] * x[q]; Q[q] = 0.0; }
+    } else {
+      const double k = s + 1.0;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        const double mk1 = M[q];
+        // primitives: PrimVar::equ (src/primvar.h:213-221); density: scalar form (src/fv.cc:659-661)
+        const double mk = q < 4 ? x[q] * (1.0 / k) + mk1 * ((k - 1.0) / k) : mk1 + (x[q] - mk1) / k;
+        M[q] = mk;
+        Z[q] += x[q] * x[q];
+        Q[q] += (x[q] - mk1) * (x[q] - mk);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    mean[5 * (size_t)v + q] = M[q];
+    m2[5 * (size_t)v + q] = Q[q];
+    s2[5 * (size_t)v + q] = Z[q];
+  }
+}
+
+// Density at probe points: linear interpolation in the containing triangle
+// (corner ids pc, barycentric weights pw, both from the host) for every sample.
+__global__ void k_probes(StageArgs A, int nv, int np, const int* pc, const double* pw, int S,
+                         double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np * S) return;
+  const int s = i / np, p = i % np;
+  const Gas gas{A.gamma[s], A.gasR[s]};
+  double r = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const Prim w = load_prim(A.W_in, (size_t)s * nv + pc[3 * p + k]);
+    r += pw[3 * p + k] * density(gas, w);
+  }
+  out[i] = r;
+}
+
+// Kinetic energy and entropy integrals of every sample (block partials).
+__global__ __launch_bounds__(256) void k_global_all(DevGeo G, StageArgs A, double* out) {
+  __shared__ double lds[4 * 2];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+  const Gas gas{A.gamma[s], A.gasR[s]};
+  double q[2] = {0, 0};
+  if (v < G.nv) {
+    const Prim w = load_prim(A.W_in, (size_t)s * G.nv + v);
+    const double wt = G.weight[v], da = G.dcarea[v];
+    const double rho = density(gas, w);
+    q[0] = (0.5 * rho * (w.u * w.u + w.v * w.v) * da) * wt;
+    q[1] = (-rho * entropy(gas, w) / (gas.gamma - 1.0)) * da * wt;
+  }
+  block_sum<2>(q, lds);
+  if (threadIdx.x == 0) {
+    double* o = out + ((size_t)s * gridDim.x + blockIdx.x) * 2;
+    o[0] = q[0];
+    o[1] = q[1];
+  }
+}
+
+void launch_moments(const DevGeo& G, const StageArgs& A, int S, double* mean, double* m2,
+                    double* s2, hipStream_t st) {
+  hipLaunchKernelGGL(k_moments, dim3((G.nv + 255) / 256), dim3(256), 0, st, G, A, S, mean, m2, s2);
+}
+void launch_probes(const StageArgs& A, int nv, int np, const int* pc, const double* pw, int S,
+                   double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_probes, dim3((np * S + 255) / 256), dim3(256), 0, st, A, nv, np, pc, pw, S, out);
+}
+void launch_global_all(const DevGeo& G, const StageArgs& A, int S, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_global_all, dim3((G.nv + 255) / 256, S), dim3(256), 0, st, G, A, out);
 }
 
 }  // namespace tsm

@@ -34,20 +34,25 @@ __device__ __forceinline__ double entropy(const Gas& g, const Prim& w) {
   return log(w.p) - g.gamma * log(density(g, w));
 }
 
-// src/material.h:24-42
+// Logarithmic mean (src/material.h:24-42): F(f) = atanh(f)/f with
+// f = (b - a)/(b + a), by its series 1 + u/3 + u^2/5 + u^3/7 (u = f^2 < 1e-2)
+// or by log(b/a)/(2f).  The reference forms f as (xi - 1)/(xi + 1) with
+// xi = b/a and divides the series terms; forming f with one division and
+// multiplying by the reciprocals gives the same value to within an ulp with
+// two divisions instead of five on the series branch.
 __device__ __forceinline__ double logavg(double a, double b) {
-  double xi = b / a;
-  double f = (xi - 1.0) / (xi + 1.0);
-  double u = f * f;
+  const double sum = a + b;
+  const double f = (b - a) / sum;
+  const double u = f * f;
   double F;
   if (u < 1.0e-2) {
-    double u2 = u * u;
-    double u3 = u2 * u;
-    F = 1.0 + u / 3.0 + u2 / 5.0 + u3 / 7.0;
+    const double u2 = u * u;
+    const double u3 = u2 * u;
+    F = 1.0 + u * (1.0 / 3.0) + u2 * (1.0 / 5.0) + u3 * (1.0 / 7.0);
   } else {
-    F = log(xi) / 2.0 / f;
+    F = log(b / a) / 2.0 / f;
   }
-  return 0.5 * (a + b) / F;
+  return 0.5 * sum / F;
 }
 
 // src/material.h:501-513
@@ -144,17 +149,28 @@ struct FluxOpts {
 
 // Per-vertex quantities the two-point fluxes use, computed once per vertex with
 // the reference's own expressions (Density, 0.5/(R T), Entropy, sound_speed).
+// V: entropy variables of the TeCNO dissipation (src/kepes_tecno_roe.cc:
+// 98-113), (gamma - s)/(gamma - 1) - beta |u|^2, 2 beta u, 2 beta v, -2 beta.
 struct Side {
   Prim w;
-  double rho, beta, s, c;
+  double rho, beta, c;
+  double V[4];
 };
-template <bool NEED_S, bool NEED_C>
+template <bool NEED_V, bool NEED_C>
 __device__ __forceinline__ Side make_side(const Gas& g, const Prim& w) {
   Side q;
   q.w = w;
   q.rho = density(g, w);
   q.beta = 0.5 / (g.R * w.T);
-  q.s = NEED_S ? (log(w.p) - g.gamma * log(q.rho)) : 0.0;
+  if (NEED_V) {
+    const double s = log(w.p) - g.gamma * log(q.rho);
+    q.V[0] = (g.gamma - s) / (g.gamma - 1.0) - q.beta * (w.u * w.u + w.v * w.v);
+    q.V[1] = 2 * w.u * q.beta;
+    q.V[2] = 2 * w.v * q.beta;
+    q.V[3] = -2 * q.beta;
+  } else {
+    q.V[0] = q.V[1] = q.V[2] = q.V[3] = 0.0;
+  }
   q.c = NEED_C ? sound_speed(g, w) : 0.0;
   return q;
 }
@@ -315,11 +331,8 @@ __device__ __forceinline__ Cons tecno_flux(const Gas& g, const FluxOpts& o, doub
     double lr = fabs(vn) + fabs(a);
     Lam[0] = Lam[1] = Lam[2] = Lam[3] = lr;
   }
-  double sl = L.s, sr = Rr.s;
-  double Vl[4] = {(gam - sl) / (gam - 1.0) - betal * (l.u * l.u + l.v * l.v), 2 * l.u * betal,
-                  2 * l.v * betal, -2 * betal};
-  double Vr[4] = {(gam - sr) / (gam - 1.0) - betar * (r.u * r.u + r.v * r.v), 2 * r.u * betar,
-                  2 * r.v * betar, -2 * betar};
+  const double* Vl = L.V;
+  const double* Vr = Rr.V;
   double h = 0.0;
   if (REC == R_TVB) h = sqrt(rx * rx + ry * ry);
   double gdl[4], gdr[4];

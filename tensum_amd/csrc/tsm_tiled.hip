@@ -15,6 +15,8 @@
 // Every sum is taken in the same order as in the reference; no atomics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "tsm_device.h"
 #include "tsm_physics.h"
 
@@ -22,8 +24,9 @@ namespace tsm {
 
 namespace {
 
-__device__ __forceinline__ Prim ld_prim(const double* W, size_t cs, size_t i) {
-  return Prim{W[i], W[cs + i], W[2 * cs + i], W[3 * cs + i]};
+__device__ __forceinline__ Prim ld_prim(const double* W, size_t i) {
+  const double4 q = reinterpret_cast<const double4*>(W)[i];
+  return Prim{q.x, q.y, q.z, q.w};
 }
 
 template <int NV>
@@ -145,7 +148,7 @@ __device__ __forceinline__ Cons bflux(const DevGeo& G, const FluxOpts& fo, const
 // ---------------------------------------------------------------------------
 // gradient kernel
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_grad_tile(DevGeo G, TileGeo TG, StageArgs A) {
+__global__ __launch_bounds__(512, 4) void k_grad_tile(DevGeo G, TileGeo TG, StageArgs A) {
   extern __shared__ double lds[];
   const int t = blockIdx.x, s = blockIdx.y;
   if (A.ctl[s].done) return;
@@ -155,11 +158,11 @@ __global__ __launch_bounds__(256) void k_grad_tile(DevGeo G, TileGeo TG, StageAr
   double* SP = lds + 6 * MS;   // 8 x TB periodic exchange
   const int b0 = TG.vbeg[t], nown = TG.vbeg[t + 1] - b0;
   const int h0 = TG.hbeg[t], ns = nown + TG.hbeg[t + 1] - h0;
-  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  const size_t off = (size_t)s * G.nv;
   const Gas gas{A.gamma[s], A.gasR[s]};
   for (int i = threadIdx.x; i < ns; i += blockDim.x) {
     const int gid = (i < nown) ? b0 + i : TG.halo[h0 + i - nown];
-    const Ent e = prim2ent(gas, ld_prim(A.W_in, cs, off + gid));
+    const Ent e = prim2ent(gas, ld_prim(A.W_in, off + gid));
     SE[i] = e.f; SE[MS + i] = e.vx; SE[2 * MS + i] = e.vy; SE[3 * MS + i] = e.l;
     SX[i] = G.x[gid]; SX[MS + i] = G.y[gid];
   }
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(256) void k_grad_tile(DevGeo G, TileGeo TG, StageAr
           const int v0 = G.bv[2 * bf], v1 = G.bv[2 * bf + 1];
           const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
           const int type = G.btype[bf];
-          const Prim w0 = ld_prim(A.W_in, cs, off + v0), w1 = ld_prim(A.W_in, cs, off + v1);
+          const Prim w0 = ld_prim(A.W_in, off + v0), w1 = ld_prim(A.W_in, off + v1);
           const Ent E0 = prim2ent(gas, w0), E1 = prim2ent(gas, w1);
           Ent q0 = E0, q1 = E1;
           if (type == TSM_BC_SLIP || type == TSM_BC_NOSLIP || type == TSM_BC_PRESSURE ||
@@ -238,8 +241,9 @@ __global__ __launch_bounds__(256) void k_grad_tile(DevGeo G, TileGeo TG, StageAr
   }
   if (own) {
     const double fact = G.gfact[v];
+    double* dEv = A.dE + 8 * (off + v);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) A.dE[(size_t)q * cs + off + v] = acc[q] * fact;
+    for (int q = 0; q < 8; ++q) dEv[q] = acc[q] * fact;
   }
 }
 
@@ -250,49 +254,72 @@ template <int FLUX, int REC>
 struct SlotLayout {
   static constexpr bool TEC = FLUX == F_ROE || FLUX == F_RUS;
   static constexpr bool GRAD = TEC && REC != R_FIRST;
-  static constexpr int QS = 6, QC = 7, QD = 6 + (TEC ? 1 : 0) + (FLUX == F_ROE ? 1 : 0);
+  // rows: T u v p | rho beta | V0..V3 (TeCNO) | c (Roe) | dE x8, x, y (gradients)
+  static constexpr int QS = 6, QC = 6 + (TEC ? 4 : 0), QD = QC + (FLUX == F_ROE ? 1 : 0);
   static constexpr int QX = QD + 8;
   static constexpr int NQ = QD + (GRAD ? 10 : 0);
 };
 
+// Faces per thread of the flux phase: the host launches at least
+// ceil(max_faces / FPT) threads per tile.
+constexpr int FPT = 2;
+
+// Waves per SIMD the flux kernel is compiled for: 3 keeps every value in
+// registers (at most 168 VGPRs); 4 caps it at 128 VGPRs with a few spills.
+#ifndef TSM_FU_WAVES
+#define TSM_FU_WAVES 3
+#endif
+
 template <int FLUX, int REC, int LAST>
-__global__ __launch_bounds__(256) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs A, StepParams P,
+__global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs A, StepParams P,
                                                  const double* Wn, double* dtpart, int r) {
   using L = SlotLayout<FLUX, REC>;
   extern __shared__ double lds[];
   const int t = blockIdx.x, s = blockIdx.y;
   if (A.ctl[s].done) return;
   const int MS = TG.max_slots, MF = TG.max_faces, TB = TG.tb;
-  double* Q = lds;                  // NQ x MS slot data
-  double* SF = lds + L::NQ * MS;    // 4 x MF face fluxes
-  double* SR = SF + 4 * MF;         // 4 x TB periodic exchange
-  double* red = SR + 4 * TB;        // 16 reduction scratch
+  // LDS: slot rows Q (NQ x MS).  Once the face fluxes are in registers the
+  // rows past the primitive state are dead and hold the face fluxes SF
+  // (4 x MF) and, in the last stage, the face normals SN (2 x MF); the
+  // periodic exchange SR (4 x TB) reuses the primitive rows once every owned
+  // state has been read.
+  double* Q = lds;
+  double* SF = lds + 4 * MS;
+  double* SN = SF + 4 * MF;
+  double* SR = lds;
+  double* red = lds + 4 * MS + max((L::NQ - 4) * MS, 6 * MF);  // 16 reduction scratch
   const int b0 = TG.vbeg[t], nown = TG.vbeg[t + 1] - b0;
   const int h0 = TG.hbeg[t], ns = nown + TG.hbeg[t + 1] - h0;
   const int f0 = TG.fbeg[t], nfl = TG.fbeg[t + 1] - f0;
-  const size_t cs = A.cs, off = (size_t)s * G.nv;
+  const size_t off = (size_t)s * G.nv;
   const Gas gas{A.gamma[s], A.gasR[s]};
 
   // ---- phase 1: slot quantities ------------------------------------------
   for (int i = threadIdx.x; i < ns; i += blockDim.x) {
     const int gid = (i < nown) ? b0 + i : TG.halo[h0 + i - nown];
-    const Prim w = ld_prim(A.W_in, cs, off + gid);
+    const Prim w = ld_prim(A.W_in, off + gid);
     const Side q = make_side<L::TEC, FLUX == F_ROE>(gas, w);
     Q[i] = w.T; Q[MS + i] = w.u; Q[2 * MS + i] = w.v; Q[3 * MS + i] = w.p;
     Q[4 * MS + i] = q.rho; Q[5 * MS + i] = q.beta;
-    if (L::TEC) Q[L::QS * MS + i] = q.s;
+    if (L::TEC)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Q[(L::QS + k) * MS + i] = q.V[k];
     if (FLUX == F_ROE) Q[L::QC * MS + i] = q.c;
     if (L::GRAD) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) Q[(L::QD + k) * MS + i] = A.dE[(size_t)k * cs + off + gid];
+      for (int k = 0; k < 8; ++k) Q[(L::QD + k) * MS + i] = A.dE[8 * (off + gid) + k];
       Q[L::QX * MS + i] = G.x[gid];
       Q[(L::QX + 1) * MS + i] = G.y[gid];
     }
   }
   __syncthreads();
 
-  // ---- phase 2: local face fluxes ----------------------------------------
-  for (int j = threadIdx.x; j < nfl; j += blockDim.x) {
+  // ---- phase 2: local face fluxes (registers, then LDS) ------------------
+  double fl[FPT][4], fn[FPT][2];
+#pragma unroll
+  for (int q = 0; q < FPT; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    if (j >= nfl) continue;
     const unsigned lr = TG.f_lr[f0 + j];
     const int l = lr & 0xffff, rr = lr >> 16;
     const double nx = TG.f_n[2 * (size_t)(f0 + j)], ny = TG.f_n[2 * (size_t)(f0 + j) + 1];
@@ -303,8 +330,11 @@ __global__ __launch_bounds__(256) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs
     Rs.w = Prim{Q[rr], Q[MS + rr], Q[2 * MS + rr], Q[3 * MS + rr]};
     Ls.rho = Q[4 * MS + l]; Rs.rho = Q[4 * MS + rr];
     Ls.beta = Q[5 * MS + l]; Rs.beta = Q[5 * MS + rr];
-    Ls.s = L::TEC ? Q[L::QS * MS + l] : 0.0;
-    Rs.s = L::TEC ? Q[L::QS * MS + rr] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      Ls.V[k] = L::TEC ? Q[(L::QS + k) * MS + l] : 0.0;
+      Rs.V[k] = L::TEC ? Q[(L::QS + k) * MS + rr] : 0.0;
+    }
     Ls.c = (FLUX == F_ROE) ? Q[L::QC * MS + l] : 0.0;
     Rs.c = (FLUX == F_ROE) ? Q[L::QC * MS + rr] : 0.0;
     double gl[8], gr[8], rx = 0.0, ry = 0.0;
@@ -315,10 +345,16 @@ __global__ __launch_bounds__(256) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs
       ry = Q[(L::QX + 1) * MS + rr] - Q[(L::QX + 1) * MS + l];
     }
     const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gl, gr, nx * ia, ny * ia);
-    SF[j] = F.m * area;
-    SF[MF + j] = F.mx * area;
-    SF[2 * MF + j] = F.my * area;
-    SF[3 * MF + j] = F.e * area;
+    fl[q][0] = F.m * area; fl[q][1] = F.mx * area; fl[q][2] = F.my * area; fl[q][3] = F.e * area;
+    fn[q][0] = nx; fn[q][1] = ny;
+  }
+  __syncthreads();  // every slot row read: rows 4.. become SF / SN
+#pragma unroll
+  for (int q = 0; q < FPT; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    if (j >= nfl) continue;
+    SF[j] = fl[q][0]; SF[MF + j] = fl[q][1]; SF[2 * MF + j] = fl[q][2]; SF[3 * MF + j] = fl[q][3];
+    if (LAST) { SN[j] = fn[q][0]; SN[MF + j] = fn[q][1]; }
   }
   __syncthreads();
 
@@ -329,8 +365,10 @@ __global__ __launch_bounds__(256) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs
   const int bb0 = TG.bbeg[t];
   double acc[4] = {0, 0, 0, 0};
   Prim w0{1, 0, 0, 1};
+  if (own) w0 = Prim{Q[tid], Q[MS + tid], Q[2 * MS + tid], Q[3 * MS + tid]};
+  const bool per = TG.tper[t] != 0;
+  if (per) __syncthreads();  // owned states read: SR may overwrite the primitive rows
   if (own) {
-    w0 = Prim{Q[tid], Q[MS + tid], Q[2 * MS + tid], Q[3 * MS + tid]};
     for (int k = TG.gptr[v]; k < TG.gptr[v + 1]; ++k) {
       const unsigned e = TG.gent[k];
       if (e & 0x4000u) {
@@ -345,7 +383,6 @@ __global__ __launch_bounds__(256) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs
       }
     }
   }
-  const bool per = TG.tper[t] != 0;
   if (per) {
     if (own)
 #pragma unroll
@@ -387,13 +424,12 @@ __global__ __launch_bounds__(256) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs
       U.m += 0.0 + acc[0] * c; U.mx += 0.0 + acc[1] * c;
       U.my += 0.0 + acc[2] * c; U.e += 0.0 + acc[3] * c;
     } else {
-      const Cons Uo = prim2con(gas, ld_prim(Wn, cs, off + v));
+      const Cons Uo = prim2con(gas, ld_prim(Wn, off + v));
       U.m += Uo.m * a + acc[0] * c; U.mx += Uo.mx * a + acc[1] * c;
       U.my += Uo.my * a + acc[2] * c; U.e += Uo.e * a + acc[3] * c;
     }
     w1 = con2prim(gas, U);
-    double* Wo = A.W_out + off + v;
-    Wo[0] = w1.T; Wo[cs] = w1.u; Wo[2 * cs] = w1.v; Wo[3 * cs] = w1.p;
+    reinterpret_cast<double4*>(A.W_out)[off + v] = make_double4(w1.T, w1.u, w1.v, w1.p);
     if (LAST) {  // spectral radius for the next step's dt, src/fv.cc:1511-1553
       const double cm = sound_speed(gas, w1);
       for (int k = TG.gptr[v]; k < TG.gptr[v + 1]; ++k) {
@@ -405,9 +441,8 @@ __global__ __launch_bounds__(256) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs
           const double vn = w1.u * nx + w1.v * ny;
           lam += 0.5 * (fabs(vn) + cm * ar);
         } else {
-          const size_t j = (size_t)f0 + (e & 0x3fffu);
-          const double nx = TG.f_n[2 * j], ny = TG.f_n[2 * j + 1];
-          const double ar = sqrt(nx * nx + ny * ny);
+          const int j = e & 0x3fffu;
+          const double nx = SN[j], ny = SN[MF + j], ar = sqrt(nx * nx + ny * ny);
           const double vn = w1.u * nx + w1.v * ny;
           lam += (fabs(vn) + cm * ar) * 1.0;
         }
@@ -443,12 +478,252 @@ __global__ __launch_bounds__(256) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs
   }
 }
 
+// ---------------------------------------------------------------------------
+// fused stage kernel (meshes without periodic sets, schemes with gradients):
+// gradient + flux + gather + update in one pass over a tile with two halo
+// rings.  The owned and first-ring slots ("gradient slots", the first n1)
+// get their Green-Gauss gradients from the cells around them, whose corners
+// are the second ring; each first-ring gradient is the same sum, in the same
+// order, as the one its owning tile forms, so the result equals the
+// two-kernel path bit for bit and dE never goes through memory.
+// ---------------------------------------------------------------------------
+template <int FLUX, int REC>
+struct FusedLayout {  // LDS rows: MS1 = gradient slots, MS2 = all slots
+  static constexpr bool ROE = FLUX == F_ROE;
+  // per gradient slot: T u v p | rho beta | V0..V3 | c (Roe) | dE x8
+  static constexpr int R1 = 4 + 2 + 4 + (ROE ? 1 : 0) + 8;
+  // per slot: entropy variables of the gradient (prim2ent) x4 | x y
+  static constexpr int R2 = 6;
+};
+
+template <int FLUX, int REC, int LAST>
+__global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, TileGeo TG,
+                                                                  StageArgs A, StepParams P,
+                                                                  const double* Wn,
+                                                                  double* dtpart, int r) {
+  using FL = FusedLayout<FLUX, REC>;
+  extern __shared__ double lds[];
+  const int t = blockIdx.x, s = blockIdx.y;
+  if (A.ctl[s].done) return;
+  const int MS1 = TG.max_slots1, MS2 = TG.max_slots, MF = TG.max_faces;
+  double* Q = lds;                          // R1 x MS1
+  double* QV = Q + 6 * MS1;                 // flux entropy variables
+  double* QC = QV + 4 * MS1;                // Roe sound speed
+  double* QD = QC + (FL::ROE ? MS1 : 0);    // dE
+  double* SE = Q + FL::R1 * MS1;            // 4 x MS2 gradient entropy variables
+  double* SX = SE + 4 * MS2;                // 2 x MS2 coordinates
+  double* SF = lds + 4 * MS1;               // after the flux phase: 4 x MF fluxes
+  double* SN = SF + 4 * MF;                 // last stage: 2 x MF normals
+  double* red = lds + 4 * MS1 + max((FL::R1 - 4) * MS1 + FL::R2 * MS2, 6 * MF);
+  const int b0 = TG.vbeg[t], nown = TG.vbeg[t + 1] - b0;
+  const int h0 = TG.hbeg[t], n1 = nown + TG.hbeg[t + 1] - h0;
+  const int k0 = TG.h2beg[t], ns = n1 + TG.h2beg[t + 1] - k0;
+  const int f0 = TG.fbeg[t], nfl = TG.fbeg[t + 1] - f0;
+  const size_t off = (size_t)s * G.nv;
+  const Gas gas{A.gamma[s], A.gasR[s]};
+
+  // ---- phase 1: slot quantities (two rings) ------------------------------
+  for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+    const int gid = (i < nown) ? b0 + i : (i < n1 ? TG.halo[h0 + i - nown] : TG.halo2[k0 + i - n1]);
+    const Prim w = ld_prim(A.W_in, off + gid);
+    const Ent e = prim2ent(gas, w);
+    SE[i] = e.f; SE[MS2 + i] = e.vx; SE[2 * MS2 + i] = e.vy; SE[3 * MS2 + i] = e.l;
+    SX[i] = G.x[gid]; SX[MS2 + i] = G.y[gid];
+    if (i < n1) {
+      const Side q = make_side<true, FL::ROE>(gas, w);
+      Q[i] = w.T; Q[MS1 + i] = w.u; Q[2 * MS1 + i] = w.v; Q[3 * MS1 + i] = w.p;
+      Q[4 * MS1 + i] = q.rho; Q[5 * MS1 + i] = q.beta;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) QV[k * MS1 + i] = q.V[k];
+      if (FL::ROE) QC[i] = q.c;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2: gradients of the gradient slots (src/fv.cc:887-996) -------
+  const int gb = TG.g1beg[t];
+  for (int k = threadIdx.x; k < n1; k += blockDim.x) {
+    const int gid = (k < nown) ? b0 + k : TG.halo[h0 + k - nown];
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = TG.g1ptr[gb + k]; c < TG.g1ptr[gb + k + 1]; ++c) {
+      const unsigned e = TG.g1ent[c];
+      const int a = e & 1023, b = (e >> 10) & 1023, d = (e >> 20) & 1023;
+      const double xa = SX[a], ya = SX[MS2 + a], xb = SX[b], yb = SX[MS2 + b], xd = SX[d], yd = SX[MS2 + d];
+      const double n0x = (yb - yd), n0y = -(xb - xd);
+      const double n1x = (yd - ya), n1y = -(xd - xa);
+      const double n2x = (ya - yb), n2y = -(xa - xb);
+      double g[8];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double ea = SE[q * MS2 + a], eb = SE[q * MS2 + b], ed = SE[q * MS2 + d];
+        g[2 * q] = n0x * ea + n1x * eb + n2x * ed;
+        g[2 * q + 1] = n0y * ea + n1y * eb + n2y * ed;
+      }
+      if (e >> 30) {  // boundary correction, src/fv.cc:926-966
+        const int cg = TG.g1cgid[c];
+        for (int z = 0; z < 2; ++z) {
+          const int bf = G.cbf[2 * cg + z];
+          if (bf < 0) break;
+          const int v0 = G.bv[2 * bf], v1 = G.bv[2 * bf + 1];
+          const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
+          const int type = G.btype[bf];
+          const Prim w0 = ld_prim(A.W_in, off + v0), w1 = ld_prim(A.W_in, off + v1);
+          const Ent E0 = prim2ent(gas, w0), E1 = prim2ent(gas, w1);
+          Ent q0 = E0, q1 = E1;
+          if (type == TSM_BC_SLIP || type == TSM_BC_NOSLIP || type == TSM_BC_PRESSURE ||
+              type == TSM_BC_INLET) {
+            Prim s0 = w0, s1 = w1;
+            bc1(type, G.bstate + 4 * bf, nx, ny, s0);
+            bc1(type, G.bstate + 4 * bf, nx, ny, s1);
+            q0 = prim2ent(gas, s0);
+            q1 = prim2ent(gas, s1);
+          }
+          const double dq[4] = {(q0.f + q1.f - E0.f - E1.f) / 2.0,
+                                (q0.vx + q1.vx - E0.vx - E1.vx) / 2.0,
+                                (q0.vy + q1.vy - E0.vy - E1.vy) / 2.0,
+                                (q0.l + q1.l - E0.l - E1.l) / 2.0};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            g[2 * q] += nx * dq[q];
+            g[2 * q + 1] += ny * dq[q];
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc[q] += g[q];
+    }
+    const double fact = G.gfact[gid];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) QD[q * MS1 + k] = acc[q] * fact;
+  }
+  __syncthreads();
+
+  // ---- phase 3: local face fluxes (registers, then LDS) ------------------
+  double fl[FPT][4], fn[FPT][2];
+#pragma unroll
+  for (int q = 0; q < FPT; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    if (j >= nfl) continue;
+    const unsigned lr = TG.f_lr[f0 + j];
+    const int l = lr & 0xffff, rr = lr >> 16;
+    const double nx = TG.f_n[2 * (size_t)(f0 + j)], ny = TG.f_n[2 * (size_t)(f0 + j) + 1];
+    const double area = sqrt(nx * nx + ny * ny);
+    const double ia = 1.0 / area;
+    Side Ls, Rs;
+    Ls.w = Prim{Q[l], Q[MS1 + l], Q[2 * MS1 + l], Q[3 * MS1 + l]};
+    Rs.w = Prim{Q[rr], Q[MS1 + rr], Q[2 * MS1 + rr], Q[3 * MS1 + rr]};
+    Ls.rho = Q[4 * MS1 + l]; Rs.rho = Q[4 * MS1 + rr];
+    Ls.beta = Q[5 * MS1 + l]; Rs.beta = Q[5 * MS1 + rr];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { Ls.V[k] = QV[k * MS1 + l]; Rs.V[k] = QV[k * MS1 + rr]; }
+    Ls.c = FL::ROE ? QC[l] : 0.0;
+    Rs.c = FL::ROE ? QC[rr] : 0.0;
+    double gl[8], gr[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) { gl[k] = QD[k * MS1 + l]; gr[k] = QD[k * MS1 + rr]; }
+    const double rx = SX[rr] - SX[l], ry = SX[MS2 + rr] - SX[MS2 + l];
+    const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gl, gr, nx * ia, ny * ia);
+    fl[q][0] = F.m * area; fl[q][1] = F.mx * area; fl[q][2] = F.my * area; fl[q][3] = F.e * area;
+    fn[q][0] = nx; fn[q][1] = ny;
+  }
+  __syncthreads();  // every slot row read: rows past the primitive state become SF / SN
+#pragma unroll
+  for (int q = 0; q < FPT; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    if (j >= nfl) continue;
+    SF[j] = fl[q][0]; SF[MF + j] = fl[q][1]; SF[2 * MF + j] = fl[q][2]; SF[3 * MF + j] = fl[q][3];
+    if (LAST) { SN[j] = fn[q][0]; SN[MF + j] = fn[q][1]; }
+  }
+  __syncthreads();
+
+  // ---- phase 4: gather and update (no periodic sets on these meshes) -----
+  const int tid = threadIdx.x;
+  const bool own = tid < nown;
+  const int v = b0 + tid;
+  const int bb0 = TG.bbeg[t];
+  double acc[4] = {0, 0, 0, 0};
+  double part[4] = {0, 0, 0, 0};
+  double lam = 0.0;
+  if (own) {
+    const Prim w0 = Prim{Q[tid], Q[MS1 + tid], Q[2 * MS1 + tid], Q[3 * MS1 + tid]};
+    for (int k = TG.gptr[v]; k < TG.gptr[v + 1]; ++k) {
+      const unsigned e = TG.gent[k];
+      if (e & 0x4000u) {
+        const int b = TG.tbf[bb0 + (e & 0x3fffu)];
+        const Cons Fb = bflux<FLUX, REC>(G, A.fo, gas, b, w0);
+        acc[0] += Fb.m * 0.5; acc[1] += Fb.mx * 0.5; acc[2] += Fb.my * 0.5; acc[3] += Fb.e * 0.5;
+      } else {
+        const int j = e & 0x3fffu;
+        const double q0 = SF[j], q1 = SF[MF + j], q2 = SF[2 * MF + j], q3 = SF[3 * MF + j];
+        if (e & 0x8000u) { acc[0] -= q0; acc[1] -= q1; acc[2] -= q2; acc[3] -= q3; }
+        else { acc[0] += q0; acc[1] += q1; acc[2] += q2; acc[3] += q3; }
+      }
+    }
+    const double area = G.dcarea[v];
+    if (LAST) {
+      const double wt = G.weight[v];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double q = acc[k] / area;
+        part[k] = (q * q) * wt;
+      }
+    }
+    const double a_rk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
+    const double b_rk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+    const double factor = A.ctl[s].dt / (1.0 * area);
+    const double b = b_rk[r], a = a_rk[r], c = -factor * b_rk[r];
+    Cons U = prim2con(gas, w0);
+    U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
+    if (r == 0) {
+      U.m += 0.0 + acc[0] * c; U.mx += 0.0 + acc[1] * c;
+      U.my += 0.0 + acc[2] * c; U.e += 0.0 + acc[3] * c;
+    } else {
+      const Cons Uo = prim2con(gas, ld_prim(Wn, off + v));
+      U.m += Uo.m * a + acc[0] * c; U.mx += Uo.mx * a + acc[1] * c;
+      U.my += Uo.my * a + acc[2] * c; U.e += Uo.e * a + acc[3] * c;
+    }
+    const Prim w1 = con2prim(gas, U);
+    reinterpret_cast<double4*>(A.W_out)[off + v] = make_double4(w1.T, w1.u, w1.v, w1.p);
+    if (LAST) {  // spectral radius for the next step's dt, src/fv.cc:1511-1553
+      const double cm = sound_speed(gas, w1);
+      for (int k = TG.gptr[v]; k < TG.gptr[v + 1]; ++k) {
+        const unsigned e = TG.gent[k];
+        if (e & 0x4000u) {
+          const int bf = TG.tbf[bb0 + (e & 0x3fffu)];
+          const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
+          const double ar = sqrt(nx * nx + ny * ny);
+          const double vn = w1.u * nx + w1.v * ny;
+          lam += 0.5 * (fabs(vn) + cm * ar);
+        } else {
+          const int j = e & 0x3fffu;
+          const double nx = SN[j], ny = SN[MF + j], ar = sqrt(nx * nx + ny * ny);
+          const double vn = w1.u * nx + w1.v * ny;
+          lam += (fabs(vn) + cm * ar) * 1.0;
+        }
+      }
+    }
+  }
+  if (LAST) {
+    double dtv = 1.0e20;
+    if (own) dtv = (P.time_step > 0.0) ? P.time_step : P.cfl * G.dcarea[v] / lam;
+    bsum<4>(part, red);
+    if (threadIdx.x == 0) {
+      double* pp = A.partial + ((size_t)s * gridDim.x + t) * 4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) pp[k] = part[k];
+    }
+    __syncthreads();
+    const double m = bmin(dtv, red);
+    if (threadIdx.x == 0) dtpart[(size_t)s * gridDim.x + t] = m;
+  }
+}
+
 // Residual norm, time advance, loop test and the next step's global dt
 // (src/fv.cc:1591-1618, :1801-1879, :2273-2316).
-__global__ __launch_bounds__(256) void k_step_final_tiled(StageArgs A, StepParams P, int nblk,
+__global__ __launch_bounds__(1024) void k_step_final_tiled(StageArgs A, StepParams P, int nblk,
                                                           int nv, const double* dtpart,
                                                           int log_slot) {
-  __shared__ double red[16];
+  __shared__ double red[64];
   const int s = blockIdx.x;
   Ctl& c = A.ctl[s];
   if (c.done) return;
@@ -497,21 +772,49 @@ __global__ __launch_bounds__(256) void k_step_final_tiled(StageArgs A, StepParam
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+int tile_grad_block(const TileGeo& T) {
+  const int b = ((T.tb + 63) / 64) * 64;
+  return std::max(b, T.grad_block);
+}
+
 size_t tile_grad_lds(const TileGeo& T) {
   return sizeof(double) * (6 * (size_t)T.max_slots + 8 * (size_t)T.tb);
 }
 
 namespace {
 template <int FLUX, int REC>
-size_t fu_lds(const TileGeo& T) {
-  return sizeof(double) * ((size_t)SlotLayout<FLUX, REC>::NQ * T.max_slots + 4 * (size_t)T.max_faces +
-                           4 * (size_t)T.tb + 16);
+size_t fu_lds(const TileGeo& T, bool) {
+  const size_t MS = T.max_slots, MF = T.max_faces;
+  return sizeof(double) *
+         (4 * MS + std::max((size_t)(SlotLayout<FLUX, REC>::NQ - 4) * MS, 6 * MF) + 16);
+}
+template <int FLUX, int REC>
+size_t fused_lds(const TileGeo& T) {
+  using FL = FusedLayout<FLUX, REC>;
+  const size_t MS1 = T.max_slots1, MS2 = T.max_slots, MF = T.max_faces;
+  return sizeof(double) *
+         (4 * MS1 + std::max((size_t)(FL::R1 - 4) * MS1 + (size_t)FL::R2 * MS2, 6 * MF) + 16);
 }
 template <int FLUX, int REC>
 void fu_launch(const DevGeo& G, const TileGeo& T, const StageArgs& A, const StepParams& P,
                const double* Wn, double* dtpart, int S, int r, hipStream_t st) {
-  const size_t lds = fu_lds<FLUX, REC>(T);
-  dim3 grid(T.ntiles, S), blk(256);
+  constexpr bool GRAD = SlotLayout<FLUX, REC>::GRAD;
+  if (GRAD && T.fused) {
+    const size_t lds = fused_lds<FLUX, REC>(T);
+    dim3 grid(T.ntiles, S), blk(tile_fu_block(T));
+    if (r == 2) {
+      auto k = k_stage_fused<FLUX, REC, 1>;
+      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(k, grid, blk, lds, st, G, T, A, P, Wn, dtpart, r);
+    } else {
+      auto k = k_stage_fused<FLUX, REC, 0>;
+      (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(k, grid, blk, lds, st, G, T, A, P, Wn, dtpart, r);
+    }
+    return;
+  }
+  const size_t lds = fu_lds<FLUX, REC>(T, r == 2);
+  dim3 grid(T.ntiles, S), blk(tile_fu_block(T));
   if (r == 2) {
     auto k = k_fu_tile<FLUX, REC, 1>;
     (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -537,10 +840,28 @@ void fu_rec(int rec, const DevGeo& G, const TileGeo& T, const StageArgs& A, cons
 }
 }  // namespace
 
+int tile_fu_block(const TileGeo& T) {
+  int b = std::max(T.tb, (T.max_faces + FPT - 1) / FPT);
+  b = ((b + 63) / 64) * 64;
+  return std::max(b, T.fu_block);
+}
+int tile_fu_block_max() { return 512; }
+
 size_t tile_fu_lds(const TileGeo& T, int flux, int rec) {
-  if (flux == F_ROE) return rec == R_FIRST ? fu_lds<F_ROE, R_FIRST>(T) : fu_lds<F_ROE, R_MINMOD>(T);
-  if (flux == F_RUS) return rec == R_FIRST ? fu_lds<F_RUS, R_FIRST>(T) : fu_lds<F_RUS, R_MINMOD>(T);
-  return fu_lds<F_KEPEC, R_FIRST>(T);
+  if (flux == F_ROE) {
+    if (rec == R_FIRST) return fu_lds<F_ROE, R_FIRST>(T, true);
+    return T.fused ? fused_lds<F_ROE, R_MINMOD>(T) : fu_lds<F_ROE, R_MINMOD>(T, true);
+  }
+  if (flux == F_RUS) {
+    if (rec == R_FIRST) return fu_lds<F_RUS, R_FIRST>(T, true);
+    return T.fused ? fused_lds<F_RUS, R_MINMOD>(T) : fu_lds<F_RUS, R_MINMOD>(T, true);
+  }
+  return fu_lds<F_KEPEC, R_FIRST>(T, true);
+}
+
+bool tile_needs_gradient_pass(const TileGeo& T, int flux, int rec) {
+  const bool grad = (flux == F_ROE || flux == F_RUS) && rec != R_FIRST;
+  return grad && !T.fused;
 }
 
 void launch_tile_gradient(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
@@ -548,7 +869,7 @@ void launch_tile_gradient(const DevGeo& G, const TileGeo& T, const StageArgs& A,
   const size_t lds = tile_grad_lds(T);
   (void)hipFuncSetAttribute((const void*)k_grad_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds);
-  hipLaunchKernelGGL(k_grad_tile, dim3(T.ntiles, S), dim3(256), lds, st, G, T, A);
+  hipLaunchKernelGGL(k_grad_tile, dim3(T.ntiles, S), dim3(tile_grad_block(T)), lds, st, G, T, A);
 }
 
 void launch_tile_stage(int flux, int rec, const DevGeo& G, const TileGeo& T, const StageArgs& A,
@@ -566,7 +887,7 @@ void launch_tile_stage(int flux, int rec, const DevGeo& G, const TileGeo& T, con
 
 void launch_step_final_tiled(const DevGeo& G, const StageArgs& A, const StepParams& P, int S,
                              int nblk, const double* dtpart, int log_slot, hipStream_t st) {
-  hipLaunchKernelGGL(k_step_final_tiled, dim3(S), dim3(256), 0, st, A, P, nblk, G.nv, dtpart,
+  hipLaunchKernelGGL(k_step_final_tiled, dim3(S), dim3(1024), 0, st, A, P, nblk, G.nv, dtpart,
                      log_slot);
 }
 

@@ -16,9 +16,14 @@ import tensum_amd as T
 
 
 def vortex(x, y, t=0.0, gam=1.4, beta=5.0, box=10.0, R=1.0):
-    """(T,u,v,p) of the isentropic vortex advected with (1,1) for time t."""
-    xc = ((x - t) + 5.0) % box - 5.0
-    yc = ((y - t) + 5.0) % box - 5.0
+    """(T,u,v,p) of the isentropic vortex advected with (1,1) for time t;
+    centred in the periodic box of side `box`, or not wrapped (box=None, the
+    param.in expressions as written)."""
+    if box is None:
+        xc, yc = x - t, y - t
+    else:
+        xc = ((x - t) + 0.5 * box) % box - 0.5 * box
+        yc = ((y - t) + 0.5 * box) % box - 0.5 * box
     r2 = xc * xc + yc * yc
     Tm = 1.0 - (gam - 1) * (beta ** 2 / (8 * gam * np.pi ** 2)) * np.exp(1 - r2)
     u = 1.0 - beta / (2 * np.pi) * yc * np.exp(0.5 * (1 - r2))

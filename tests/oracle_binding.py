@@ -64,6 +64,7 @@ def lib():
         L.orc_limited_slope.restype = C.c_double
         L.orc_limited_slope.argtypes = [C.POINTER(OrcParams), C.c_double, C.c_double, C.c_double]
         L.orc_reconstruct.argtypes = [C.POINTER(OrcParams), C.c_double] + [C.c_void_p] * 6
+        L.orc_moments.argtypes = [C.c_int, C.c_int] + [C.c_void_p] * 6
         _lib = L
     return _lib
 
@@ -176,3 +177,14 @@ def reconstruct(prm, h, zl, zr, gl, gr):
     rl, rr = np.empty(4), np.empty(4)
     lib().orc_reconstruct(C.byref(params(prm)), float(h), *[_p(a) for a in arrs], _p(rl), _p(rr))
     return rl, rr
+
+
+def moments(prims, gammas, gas_consts):
+    """(mean, M2, S2) of (T,u,v,p,rho) over the sample states prims (S, nv, 4)."""
+    a = np.ascontiguousarray(prims, dtype=np.float64)
+    S, nv = a.shape[0], a.shape[1]
+    g = np.ascontiguousarray(gammas, dtype=np.float64)
+    r = np.ascontiguousarray(gas_consts, dtype=np.float64)
+    m, q, z = (np.empty((nv, 5)) for _ in range(3))
+    lib().orc_moments(nv, S, _p(a), _p(g), _p(r), _p(m), _p(q), _p(z))
+    return m, q, z

@@ -235,3 +235,26 @@ def test_first_order_keeps_states_and_minmod_clips_extrema():
     assert np.array_equal(rl, zl) and np.array_equal(rr, zr)
     rl, rr = O.reconstruct(prm(reconstruct="minmod"), 1.0, zl, zr, g, -g)
     assert rl[0] == 1.5 and rr[0] == 2.0   # right side: minmod(-11, 1) = 0
+
+
+def test_moments_closed_form_streams():
+    """Welford moments of the reference's increment_moments on streams whose
+    moments are known: x_s = a + b*s gives mean a + b(S-1)/2,
+    M2 = b^2 S(S^2-1)/12, S2 = sum of squares; a constant stream gives M2 = 0."""
+    S, nv = 37, 3
+    a = np.array([1.0, 0.5, -0.25, 2.0])
+    b = np.array([0.01, -0.02, 0.03, 0.005])
+    prims = np.array([[a + b * s] * nv for s in range(S)])
+    R = np.full(S, 1.0)
+    m, q, z = O.moments(prims, np.full(S, 1.4), R)
+    s = np.arange(S)
+    for k in range(4):
+        xs = a[k] + b[k] * s
+        assert np.allclose(m[:, k], a[k] + b[k] * (S - 1) / 2, rtol=1e-13, atol=1e-15)
+        assert np.allclose(q[:, k], b[k] ** 2 * S * (S * S - 1) / 12, rtol=1e-10)
+        assert np.allclose(z[:, k], np.sum(xs * xs), rtol=1e-13)
+    rho = prims[:, 0, 3] / (R * prims[:, 0, 0])
+    assert np.allclose(m[:, 4], rho.mean(), rtol=1e-13)
+    assert np.allclose(q[:, 4], np.sum((rho - rho.mean()) ** 2), rtol=1e-9)
+    mc, qc, _ = O.moments(np.ones((5, 2, 4)), np.full(5, 1.4), np.ones(5))
+    assert np.all(qc == 0.0) and np.all(mc == 1.0)
