@@ -1,0 +1,228 @@
+"""Monte Carlo and multilevel Monte Carlo runs on the batched solver.
+
+The reference runs one sample after another (src/fv.cc:2440-2459), reads its
+random inputs from a file (src/fv.cc:502-551) and accumulates per-vertex
+moments with Welford updates (src/fv.cc:595-694) that a final pass merges
+across ranks (src/reduce_moments.cc:14-220).  Here every sample of a level is
+one slot of a single libtsm context (sample index = grid dimension of every
+kernel, per-sample dt and stop time), the levels of a multilevel run advance
+together on their own streams, inputs come from the scrambled Sobol sampler
+and the quantities of interest (probe densities, integrated entropy) are
+reduced on the device.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Params, Solver, jittered_mesh, run_group, sobol
+
+# ---------------------------------------------------------------------------
+# uncertain inputs
+# ---------------------------------------------------------------------------
+VORTEX_BOUNDS = (("beta", 4.0, 6.0), ("xc", -0.5, 0.5), ("yc", -0.5, 0.5),
+                 ("gamma", 1.3, 1.5), ("mach", 0.4, 0.6))
+RIEMANN_BOUNDS = (("sx", 0.75, 0.85), ("sy", 0.75, 0.85), ("p_ne", 0.9, 1.1),
+                  ("p_nw", 0.9, 1.1), ("p_sw", 0.9, 1.1), ("p_se", 0.9, 1.1))
+
+
+def map_inputs(u, bounds):
+    """Unit-cube points (n, d) -> dict of uniformly distributed inputs."""
+    u = np.asarray(u, dtype=np.float64)
+    return {name: lo + (hi - lo) * u[:, k] for k, (name, lo, hi) in enumerate(bounds)}
+
+
+def sample_inputs(n, bounds, seed=0, skip=0):
+    return map_inputs(sobol(n, len(bounds), seed=seed, skip=skip), bounds)
+
+
+# ---------------------------------------------------------------------------
+# initial states
+# ---------------------------------------------------------------------------
+def vortex_state(x, y, beta, xc, yc, gamma, mach, t=0.0, box=10.0):
+    """(T,u,v,p) of the isentropic vortex on the periodic box [-box/2, box/2]^2:
+    freestream rho = p = T = 1 (R = 1), velocity of Mach `mach` along (1,1),
+    strength beta, centre (xc, yc) at t = 0; the same expressions as the
+    reference's examples/Euler/isentropic_vortex/param.in:72-96 with a shifted
+    centre and a scaled freestream velocity."""
+    speed = mach * np.sqrt(gamma)
+    ux = uy = speed / np.sqrt(2.0)
+    xr = ((x - xc - ux * t) + 0.5 * box) % box - 0.5 * box
+    yr = ((y - yc - uy * t) + 0.5 * box) % box - 0.5 * box
+    r2 = xr * xr + yr * yr
+    Tm = 1.0 - (gamma - 1.0) * (beta * beta / (8.0 * gamma * np.pi ** 2)) * np.exp(1.0 - r2)
+    e = beta / (2.0 * np.pi) * np.exp(0.5 * (1.0 - r2))
+    return np.stack([Tm, ux - e * yr, uy + e * xr, Tm ** (gamma / (gamma - 1.0))], 1)
+
+
+RIEMANN_QUADRANTS = {  # (rho, u, v, p) NE, NW, SW, SE
+    "ne": (1.5, 0.0, 0.0, 1.5), "nw": (0.5323, 1.206, 0.0, 0.3),
+    "sw": (0.138, 1.206, 1.206, 0.029), "se": (0.5323, 0.0, 1.206, 0.3)}
+
+
+def riemann_state(x, y, sx, sy, p_ne, p_nw, p_sw, p_se):
+    """(T,u,v,p) of the four-quadrant Riemann problem with the split at (sx, sy)
+    and each quadrant's pressure scaled (R = 1)."""
+    east, north = x >= sx, y >= sy
+    out = np.empty((x.size, 4))
+    for key, mask, sc in (("ne", east & north, p_ne), ("nw", ~east & north, p_nw),
+                          ("sw", ~east & ~north, p_sw), ("se", east & ~north, p_se)):
+        rho, u, v, p = RIEMANN_QUADRANTS[key]
+        out[mask] = (p * sc / rho, u, v, p * sc)
+    return out
+
+
+def probe_grid(lo, hi, n=8):
+    """n x n probe points at the centres of an n x n grid of [lo, hi]^2."""
+    c = lo + (np.arange(n) + 0.5) * (hi - lo) / n
+    px, py = np.meshgrid(c, c, indexing="xy")
+    return px.ravel(), py.ravel()
+
+
+# ---------------------------------------------------------------------------
+# problems
+# ---------------------------------------------------------------------------
+@dataclass
+class Problem:
+    """A parametrised workload: mesh builder, solver settings, inputs -> states."""
+    name: str
+    bounds: tuple
+    params: Params
+    mesh: callable
+    state: callable
+    gamma: callable
+    probes: tuple
+    n_qoi: int = field(default=0)
+
+    def __post_init__(self):
+        self.n_qoi = self.probes[0].size + 2
+
+
+def vortex_problem(final_time=10.0):
+    prm = Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=final_time)
+    return Problem(
+        "vortex", VORTEX_BOUNDS, prm,
+        lambda n: jittered_mesh(n, box=(-5.0, 5.0, -5.0, 5.0), jitter=0.2, seed=0, periodic=True),
+        lambda m, p: vortex_state(m.x, m.y, p["beta"], p["xc"], p["yc"], p["gamma"], p["mach"]),
+        lambda p: p["gamma"], probe_grid(-4.375, 4.375))
+
+
+def riemann_problem(final_time=0.2):
+    prm = Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=final_time)
+    return Problem(
+        "riemann", RIEMANN_BOUNDS, prm,
+        lambda n: jittered_mesh(n, box=(0.0, 1.0, 0.0, 1.0), jitter=0.2, seed=1, periodic=False,
+                                bc={t: ("outlet",) for t in (1, 2, 3, 4)}),
+        lambda m, p: riemann_state(m.x, m.y, p["sx"], p["sy"], p["p_ne"], p["p_nw"], p["p_sw"],
+                                   p["p_se"]),
+        lambda p: np.full(np.size(p["sx"]), 1.4), probe_grid(0.05, 0.95))
+
+
+# ---------------------------------------------------------------------------
+# batched solves
+# ---------------------------------------------------------------------------
+def _one(inputs, k):
+    return {name: float(v[k]) for name, v in inputs.items()}
+
+
+def make_batch(problem: Problem, mesh, inputs):
+    """One Solver holding every sample of `inputs` (dict of arrays)."""
+    n = len(next(iter(inputs.values())))
+    s = Solver(mesh, problem.params, n_samples=n)
+    gam = np.broadcast_to(problem.gamma(inputs), (n,))
+    for k in range(n):
+        s.set_material(k, float(gam[k]), 1.0)
+        s.set_state(problem.state(mesh, _one(inputs, k)), k)
+    return s
+
+
+def qoi(problem: Problem, solver: Solver):
+    """(n_samples, n_probe + 2): probe densities, kinetic energy, entropy integral."""
+    rho = solver.probe_density(*problem.probes)
+    ke, ent = solver.global_all()
+    return np.hstack([rho, ke[:, None], ent[:, None]])
+
+
+def advance(solvers, chunk=64, max_chunks=100000):
+    """Runs every solver until all its samples reached final_time."""
+    for _ in range(max_chunks):
+        if not np.any(run_group(solvers, chunk)):
+            return
+    raise RuntimeError("samples did not reach final_time")
+
+
+def plain_mc(problem: Problem, n_mesh, n_samples, seed=0, skip=0, batch=None):
+    """Plain Monte Carlo on one mesh: QoI per sample, in sample order."""
+    mesh = problem.mesh(n_mesh)
+    inputs = sample_inputs(n_samples, problem.bounds, seed=seed, skip=skip)
+    batch = batch or n_samples
+    out = []
+    for b0 in range(0, n_samples, batch):
+        part = {k: v[b0:b0 + batch] for k, v in inputs.items()}
+        s = make_batch(problem, mesh, part)
+        advance([s])
+        out.append(qoi(problem, s))
+        s.close()
+    return inputs, np.vstack(out)
+
+
+@dataclass
+class Estimate:
+    mean: np.ndarray
+    var: np.ndarray
+    mean_se: np.ndarray        # standard error of the mean estimate
+    seconds: float
+    detail: dict
+
+
+def moments(Q):
+    """Sample mean and unbiased variance per column (Welford order, as the
+    reference's increment_moments + reduce_moments)."""
+    Q = np.asarray(Q, dtype=np.float64)
+    M = Q[0].copy()
+    M2 = np.zeros_like(M)
+    for k in range(1, Q.shape[0]):
+        d = Q[k] - M
+        M += d / (k + 1)
+        M2 += d * (Q[k] - M)
+    return M, M2 / max(Q.shape[0] - 1, 1)
+
+
+def mlmc(problem: Problem, levels, samples, seed=0):
+    """Multilevel Monte Carlo: E[Q_L] = E[Q_0] + sum_l E[Q_l - Q_{l-1}] with the
+    same inputs on both meshes of a level pair; E[Q_L^2] the same way, so
+    Var = E[Q^2] - E[Q]^2.  All level pairs advance together (one context per
+    mesh and level, one stream each)."""
+    t0 = time.perf_counter()
+    meshes = {n: problem.mesh(n) for n in levels}
+    groups, ins = [], []
+    skip = 0
+    for li, (n, N) in enumerate(zip(levels, samples)):
+        inp = sample_inputs(N, problem.bounds, seed=seed, skip=skip)
+        skip += N
+        fine = make_batch(problem, meshes[n], inp)
+        coarse = make_batch(problem, meshes[levels[li - 1]], inp) if li > 0 else None
+        groups.append((fine, coarse))
+        ins.append(inp)
+    advance([s for g in groups for s in g if s is not None])
+    mean = 0.0
+    sec = 0.0
+    se2 = 0.0
+    detail = {"levels": []}
+    for li, (fine, coarse) in enumerate(groups):
+        qf = qoi(problem, fine)
+        qc = qoi(problem, coarse) if coarse is not None else np.zeros_like(qf)
+        y1, y2 = qf - qc, qf * qf - qc * qc
+        m1, v1 = moments(y1)
+        m2, _ = moments(y2)
+        mean = mean + m1
+        sec = sec + m2
+        se2 = se2 + v1 / y1.shape[0]
+        detail["levels"].append({"n": levels[li], "samples": y1.shape[0],
+                                 "var_y": float(np.mean(v1))})
+        fine.close()
+        if coarse is not None:
+            coarse.close()
+    return Estimate(mean, sec - mean * mean, np.sqrt(se2), time.perf_counter() - t0, detail)

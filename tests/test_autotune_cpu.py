@@ -1,0 +1,30 @@
+"""Autotuner host logic (no GPU): candidate launch shapes and the cache."""
+import json
+
+import tensum_amd as T
+from tensum_amd import autotune as AT
+
+
+def test_candidates_respect_tile_and_block():
+    c = AT.candidates(tiles=(64, 256), fu_blocks=(0, 128, 512))
+    assert {"tile_vertices": 256, "block_fu": 128, "block_grad": 0} not in c
+    assert {"tile_vertices": 64, "block_fu": 128, "block_grad": 0} in c
+    assert all(x["block_grad"] == 0 for x in c)
+    sep = AT.candidates(tiles=(128,), fu_blocks=(0,), grad_blocks=(0, 256), separate=True)
+    assert len(sep) == 2
+
+
+def test_cache_stores_and_reuses(tmp_path):
+    path = str(tmp_path / "tune.json")
+    calls = []
+
+    def tune():
+        calls.append(1)
+        return {"tile_vertices": 96, "block_fu": 0, "block_grad": 0}
+
+    a = AT.cached(path, "k", tune)
+    b = AT.cached(path, "k", tune)
+    assert a == b and len(calls) == 1
+    assert json.load(open(path))["k"]["tile_vertices"] == 96
+    p = AT.apply(T.Params(), a)
+    assert AT.shape_of(p) == a
