@@ -13,6 +13,7 @@ Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -48,6 +49,24 @@ def kernel_bytes(nv, nf, nc, tiles):
     }
 
 
+PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                       "riemann1025_tiled_kernels.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration): the line 'traffic <kernel>: <bytes> B'.  None if absent."""
+    if not os.path.exists(PROFILE):
+        return None
+    with open(PROFILE) as f:
+        for line in f:
+            m = re.match(r"^traffic (\w+): (\d+) B", line.strip())
+            if m and m.group(1) == kernel:
+                return int(m.group(2))
+    return None
+
+
 # Whole-step algorithmic bytes per vertex-stage of an SSP-RK3 step at the
 # 1025^2 size (3 (256 V + 24 E + 12 T) + (40 V + 24 E) divided by 3 V).
 STEP_BYTES_PER_VERTEX_STAGE = 389.2
@@ -61,8 +80,27 @@ def main():
     ap.add_argument("--n", type=int, default=1025)
     ap.add_argument("--tile", type=int, default=0, help="owned vertices per tile (0: 128)")
     ap.add_argument("--path", default="auto", choices=["auto", "csr"])
+    ap.add_argument("--fused-gradient", action="store_true",
+                    help="gradients inside the stage kernel (two halo rings) instead of a pass "
+                         "of their own")
+    ap.add_argument("--graph", type=int, default=0, help="steps per captured HIP graph (0: none)")
+    ap.add_argument("--block", type=int, default=0, help="threads per tile of the stage kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--mc", action="store_true",
+                    help="second metric: surrogate control-variate MC against plain MC on the "
+                         "uncertain isentropic vortex (prints its own JSON line)")
+    ap.add_argument("--mc-fine", type=int, default=129)
+    ap.add_argument("--mc-coarse", type=int, default=65)
+    ap.add_argument("--mc-time", type=float, default=10.0)
+    ap.add_argument("--mc-train", type=int, default=1024)
+    ap.add_argument("--mc-pair", type=int, default=64)
+    ap.add_argument("--mc-ref", type=int, default=2048)
+    ap.add_argument("--mlmc", action="store_true",
+                    help="multilevel MC of the uncertain vortex against plain MC at equal "
+                         "GPU-seconds (prints its own JSON line)")
+    ap.add_argument("--mlmc-levels", default="65,129,257")
+    ap.add_argument("--mlmc-samples", default="1024,256,64")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -79,6 +117,12 @@ def main():
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported",
               file=sys.stderr)
         sys.exit(2)
+    if args.mc:
+        mc_bench(args)
+        return
+    if args.mlmc:
+        mlmc_bench(args)
+        return
 
     import numpy as np
 
@@ -88,7 +132,9 @@ def main():
     n = args.n
     mesh = riemann_mesh(n, seed=1)
     prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8,
-                   time_scheme="ssprk3", kernel_path=args.path, tile_vertices=args.tile)
+                   time_scheme="ssprk3", kernel_path=args.path, tile_vertices=args.tile,
+                   fused_gradient=args.fused_gradient, graph_steps=args.graph,
+                   block_fu=args.block)
     w0 = riemann(mesh.x, mesh.y)
     print(f"bench: mesh {n}x{n} ({mesh.n_vertex} vertices), building context", file=sys.stderr,
           flush=True)
@@ -112,7 +158,7 @@ def main():
 
     # per-kernel durations (HIP events on the library's stream)
     ku = s.bench(20)
-    tiles = T.host_tiling(mesh, prm.tile_vertices or 128, two_rings=True)
+    tiles = T.host_tiling(mesh, prm.tile_vertices or 128, two_rings=args.fused_gradient)
     kb = kernel_bytes(nv, nf, nc, tiles)
     if ku["dt_us"] == 0.0 and tiles["two_rings"]:  # fused stage kernel
         per_launch = {"stage_fused": ku["flux_us"]}
@@ -127,7 +173,7 @@ def main():
     achieved = kb[dom] / (dur_us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None,
+                "traffic": measured_traffic(dom),
                 "algorithmic_bytes_per_launch": kb[dom],
                 "kernel_us": {k: round(v, 2) for k, v in ku.items()},
                 "step_frac_at_389B_per_vertex_stage":
@@ -166,6 +212,50 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def mc_bench(args):
+    """Surrogate control variate vs plain MC (vortex with uncertain strength,
+    centre, gamma and Mach; QoIs: 64 probe densities, kinetic energy, entropy
+    integral).  value = GPU-seconds plain MC needs for the control variate's
+    standard error divided by the control variate's GPU-seconds (training
+    samples, fit, fine samples and surrogate evaluations all charged)."""
+    from tensum_amd import uq
+    p = uq.vortex_problem(final_time=args.mc_time)
+    say = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
+    r = uq.cv_experiment(p, args.mc_fine, args.mc_coarse, args.mc_train, args.mc_pair,
+                         args.mc_pair, 65536, args.mc_ref, log=say)
+    out = {
+        "metric": "MC moment error per GPU-second (control variate vs plain MC, equal error)",
+        "value": r["speedup_at_equal_error"],
+        "unit": "x fewer GPU-seconds than plain batched MC",
+        "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 solver, bf16/fp32 surrogate",
+        "data": "synthetic (seeded jittered meshes, scrambled Sobol inputs)",
+        "config": {"workload": f"isentropic vortex UQ, t_final {args.mc_time}",
+                   "fine_mesh": args.mc_fine, "coarse_mesh": args.mc_coarse,
+                   "surrogate": "MLP 5-256-256-256-66, GELU, BF16 MFMA, Adam"},
+        "detail": r,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def mlmc_bench(args):
+    from tensum_amd import uq
+    p = uq.vortex_problem(final_time=args.mc_time)
+    levels = tuple(int(x) for x in args.mlmc_levels.split(","))
+    samples = tuple(int(x) for x in args.mlmc_samples.split(","))
+    say = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
+    r = uq.mlmc_experiment(p, levels, samples, args.mc_ref, log=say)
+    out = {"metric": "MLMC moment error at the GPU-seconds of plain MC (uncertain vortex)",
+           "value": r["plain_mc"]["mean_error"] / max(r["mlmc"]["mean_error"], 1e-300),
+           "unit": "plain-MC mean error / MLMC mean error at equal GPU-seconds",
+           "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic (seeded jittered meshes, scrambled Sobol inputs)",
+           "config": {"workload": f"isentropic vortex UQ, t_final {args.mc_time}",
+                      "levels": levels, "samples": samples},
+           "detail": r}
     print(json.dumps(out), flush=True)
 
 

@@ -68,8 +68,11 @@ typedef struct tsm_params {
   int tile_vertices;  /* owned vertices per tile of the tiled kernels (0: 128) */
   int block_grad;     /* threads per tile of the tiled gradient kernel (0: auto) */
   int block_fu;       /* threads per tile of the tiled flux/update kernel (0: auto) */
-  int tile_separate_gradient; /* 1: gradients in their own pass even where the
-                                 mesh allows the fused two-ring stage kernel */
+  int tile_fused_gradient; /* 1: on meshes without periodic sets, form the
+                              gradients inside the stage kernel (two halo rings)
+                              instead of in a pass of their own */
+  int graph_steps;    /* tiled path: capture this many steps in one HIP graph and
+                         replay it (0: launch every kernel from the host) */
 } tsm_params;
 
 /* NPART = 1 part file content (readme/README.md:155-201), 0-based gids. */
@@ -162,6 +165,13 @@ int tsm_global_all(tsm_ctx* ctx, double* ke, double* entropy);
    from `seed`: out[n*dim], row i = point skip+i in [0,1)^dim.  dim <= 16. */
 int tsm_sobol(long n, int dim, long skip, unsigned long long seed, int scramble, double* out);
 int tsm_sobol_max_dim(void);
+
+/* pert(type, x, y, p) of param.in expressions with one sample's random numbers
+   rnd[n_rnd] (src/perturb.cc:16-209, replaces PERT): 1 and 3 multi-modal band
+   perturbations, 2 piecewise constant, 4 the p-th number (1-based).  Returns
+   TSM_E_ARG for an unknown type or a PERT4 index outside [1, n_rnd], where the
+   reference aborts. */
+int tsm_pert(int type, double x, double y, double p, const double* rnd, int n_rnd, double* out);
 
 /* Geometry export in reference order, for parity checks. */
 int tsm_geometry_sizes(tsm_ctx* ctx, int* n_face, int* n_bface);

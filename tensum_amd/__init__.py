@@ -40,7 +40,7 @@ class TsmParams(C.Structure):
                 ("n_time_stamps", C.c_int), ("cell_type", C.c_int), ("liou_fix", C.c_int),
                 ("kernel_path", C.c_int), ("tile_vertices", C.c_int),
                 ("block_grad", C.c_int), ("block_fu", C.c_int),
-                ("tile_separate_gradient", C.c_int)]
+                ("tile_fused_gradient", C.c_int), ("graph_steps", C.c_int)]
 
 
 class TsmMesh(C.Structure):
@@ -88,7 +88,8 @@ class Params:
     tile_vertices: int = 0
     block_grad: int = 0            # threads per tile, gradient kernel (0: auto)
     block_fu: int = 0              # threads per tile, flux/update kernel (0: auto)
-    separate_gradient: bool = False  # gradient pass of its own even where it could be fused
+    fused_gradient: bool = False   # gradients inside the stage kernel (two halo rings)
+    graph_steps: int = 0           # steps per captured HIP graph (0: no graph)
 
     def to_c(self):
         p = TsmParams()
@@ -109,7 +110,8 @@ class Params:
         p.kernel_path = {"auto": 0, "csr": 1}[self.kernel_path]
         p.tile_vertices = self.tile_vertices
         p.block_grad, p.block_fu = self.block_grad, self.block_fu
-        p.tile_separate_gradient = int(self.separate_gradient)
+        p.tile_fused_gradient = int(self.fused_gradient)
+        p.graph_steps = self.graph_steps
         return p
 
 
@@ -187,6 +189,8 @@ def lib():
         L.tsm_bench.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_sobol.argtypes = [C.c_long, C.c_int, C.c_long, C.c_ulonglong, C.c_int, C.c_void_p]
         L.tsm_run_group.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_void_p]
+        L.tsm_pert.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int,
+                               C.c_void_p]
         L.tsm_moments.argtypes = [C.c_void_p] + [C.c_void_p] * 3
         L.tsm_probe_density.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
         L.tsm_global_all.argtypes = [C.c_void_p] + [C.c_void_p] * 2
@@ -263,6 +267,14 @@ def sobol(n, dim, seed=0, scramble=True, skip=0):
     out = np.empty((n, dim))
     _check(lib().tsm_sobol(n, dim, skip, seed, int(scramble), _dptr(out)))
     return out
+
+
+def pert(ptype, x, y, p, rnd):
+    """The reference's pert(type, x, y, p) with one sample's random numbers."""
+    r = np.ascontiguousarray(rnd, dtype=np.float64)
+    out = C.c_double()
+    _check(lib().tsm_pert(ptype, x, y, p, _dptr(r), r.size, C.byref(out)))
+    return out.value
 
 
 def run_group(solvers, nsteps):

@@ -2,7 +2,8 @@
 
 Chooses the vertex tile size (owned vertices per workgroup, which sets the
 halo fraction and the faces each tile recomputes) and the workgroup sizes of
-the stage and gradient kernels by timing a few steps of the actual workload
+the stage and gradient kernels, and whether the gradients are formed inside
+the stage kernel (two halo rings) or in a pass of their own, by timing a few steps of the actual workload
 with the solver's own event timer (tsm_bench).  Every candidate computes the
 same sums in the same order, so tuning changes only the time: the tests check
 that a tuned run is bit-identical to an untuned one.  The reference has no
@@ -21,18 +22,21 @@ FU_BLOCKS = (0, 256, 512)      # 0: the smallest block that covers the tile
 GRAD_BLOCKS = (0, 256)
 
 
-def candidates(tiles=TILE_SIZES, fu_blocks=FU_BLOCKS, grad_blocks=GRAD_BLOCKS, separate=False):
+def candidates(tiles=TILE_SIZES, fu_blocks=FU_BLOCKS, grad_blocks=GRAD_BLOCKS, separate=True,
+               fused=(False,)):
     """Launch shapes to try; the gradient block only matters for the separate
     gradient pass, so it is varied only there."""
     out = []
-    for tb in tiles:
-        for bf in fu_blocks:
-            if bf and bf < tb:
-                continue
-            for bg in (grad_blocks if separate else (0,)):
-                if bg and bg < tb:
+    for fu in fused:
+        for tb in tiles:
+            for bf in fu_blocks:
+                if bf and bf < tb:
                     continue
-                out.append({"tile_vertices": tb, "block_fu": bf, "block_grad": bg})
+                for bg in (grad_blocks if (separate and not fu) else (0,)):
+                    if bg and bg < tb:
+                        continue
+                    out.append({"tile_vertices": tb, "block_fu": bf, "block_grad": bg,
+                                "fused_gradient": fu})
     return out
 
 
@@ -40,7 +44,7 @@ def autotune(mesh, params: Params, state, n_samples=1, steps=10, shapes=None, lo
     """Times every launch shape on `mesh` with `state` in every sample and
     returns (best Params, table of (shape, step_us)).  Shapes the kernels
     cannot take (LDS or block limits) are skipped."""
-    shapes = shapes if shapes is not None else candidates(separate=params.separate_gradient)
+    shapes = shapes if shapes is not None else candidates(separate=not params.fused_gradient)
     table = []
     for sh in shapes:
         prm = dataclasses.replace(params, **sh)
@@ -83,7 +87,7 @@ def workload_key(mesh, params: Params, n_samples):
 
 def shape_of(params: Params):
     return {"tile_vertices": params.tile_vertices, "block_fu": params.block_fu,
-            "block_grad": params.block_grad}
+            "block_grad": params.block_grad, "fused_gradient": params.fused_gradient}
 
 
 def apply(params: Params, shape):

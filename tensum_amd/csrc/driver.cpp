@@ -319,14 +319,20 @@ int main(int argc, char** argv) {
       const std::vector<double>& r = rnd[k];
       for (int q = 0; q < 4; ++q) {
         if (!e[q].parse(S.ic[names[q]], S.constants, err)) { std::cerr << "ERROR --> " << err << std::endl; return 1; }
-        e[q].set_pert([&r](int type, double, double, double kk) {
-          int id = (int)kk;  // PERT4, src/perturb.cc:194-209
-          if (type != 4 || id < 1 || id > (int)r.size()) return (double)NAN;
-          return r[id - 1];
+        e[q].set_pert([&r](int type, double px, double py, double kk) {
+          double v = 0.0;  // src/perturb.cc:16-209
+          if (tsm_pert(type, px, py, kk, r.data(), (int)r.size(), &v) != TSM_OK) return (double)NAN;
+          return v;
         });
       }
       for (int i = 0; i < nv; ++i)
         for (int q = 0; q < 4; ++q) W[4 * i + q] = e[q].eval(P.x[i], P.y[i], 0.0);
+      for (int i = 0; i < 4 * nv; ++i)
+        if (!std::isfinite(W[i])) {
+          std::cerr << "ERROR --> initial condition of " << dirs[k]
+                    << " is not finite (unknown pert type or too few random numbers?)" << std::endl;
+          return 1;
+        }
     }
     if (tsm_set_state(ctx, k, W.data()) != TSM_OK) { std::cerr << "ERROR --> " << tsm_last_error() << std::endl; return 1; }
     if (!restart) std::ofstream(dirs[k] + "/master_file.visit") << "!NBLOCKS 1" << std::endl;

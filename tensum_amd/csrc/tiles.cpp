@@ -209,6 +209,11 @@ bool build_tiles(const Geometry& g, int tb, Tiling& T, std::string& msg, bool ri
         T.cgid.push_back(c);
       }
       T.cptr[v + 1] = (int)T.cent.size();
+      if (t >= T.n_periodic_tiles) {
+        bool corr = false;
+        for (int c : vcells[v]) corr = corr || ccorr[c];
+        if (corr) T.gbv.push_back(v);
+      }
     }
     // periodic sets of this tile: consecutive owned slots
     for (int v = b0; v < b1; ++v) {
@@ -291,6 +296,8 @@ bool validate_tiles(const Geometry& g, const Tiling& T, std::string& msg) {
       }
     }
   }
+  for (int v : T.gbv)
+    if (v < 0 || v >= nv || v < T.vbeg[T.n_periodic_tiles]) { msg = "boundary gradient vertex"; return false; }
   if (T.fused && ((int)T.h2beg.size() != T.ntiles + 1 || (int)T.g1beg.size() != T.ntiles + 1)) {
     msg = "ring-2 tables"; return false;
   }

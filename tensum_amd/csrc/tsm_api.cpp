@@ -52,8 +52,12 @@ struct tsm_ctx {
   size_t red_cap = 0;
   std::vector<double> h_gamma, h_R;
   int nblk_v = 0;
+  // captured steps of the tiled path (graph_steps > 0)
+  hipGraphExec_t gexec = nullptr;
+  int gcur = -1;
 
   ~tsm_ctx() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     for (void* p : allocs) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -155,8 +159,12 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   if (c->tiled) {
     int tb = params->tile_vertices > 0 ? params->tile_vertices : 128;
     if (tb > 256) { delete c; return fail(TSM_E_ARG, "tile_vertices must be <= 256"); }
+    if (params->graph_steps < 0 || params->graph_steps > 4096) {
+      delete c;
+      return fail(TSM_E_ARG, "graph_steps must be in [0, 4096]");
+    }
     const bool grad = (params->flux == 0 || params->flux == 1) && params->reconstruct != 0;
-    const bool ring2 = grad && !params->tile_separate_gradient;
+    const bool ring2 = grad && params->tile_fused_gradient;
     if (!tsm::build_tiles(g, tb, c->tiling, msg, ring2) || !tsm::validate_tiles(g, c->tiling, msg)) {
       delete c;
       return fail(TSM_E_MESH, "tiling: " + msg);
@@ -254,6 +262,9 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     const tsm::Tiling& T = c->tiling;
     tsm::TileGeo& TG = c->TG;
     TG.ntiles = T.ntiles; TG.tb = T.tb; TG.max_slots = T.max_slots; TG.max_faces = T.max_faces;
+    TG.n_periodic_tiles = T.n_periodic_tiles;
+    TG.n_gbv = (int)T.gbv.size();
+    TG.gbv = nullptr;
     TG.fused = T.fused ? 1 : 0;
     TG.max_slots1 = T.max_slots1;
     TG.h2beg = TG.halo2 = TG.g1beg = TG.g1ptr = TG.g1cgid = nullptr;
@@ -272,6 +283,7 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     UT(vbeg, T.vbeg); UT(hbeg, T.hbeg); UT(halo, T.halo); UT(fbeg, T.fbeg); UT(bbeg, T.bbeg);
     UT(tbf, T.tbf); UT(gptr, T.gptr); UT(cptr, T.cptr); UT(cgid, T.cgid); UT(tper, T.tper);
     UT(pinfo, T.pinfo); UT(f_lr, T.f_lr); UT(cent, T.cent); UT(f_n, T.f_n); UT(gent, T.gent);
+    if (!T.gbv.empty()) UT(gbv, T.gbv);
     if (T.fused) {
       UT(h2beg, T.h2beg); UT(halo2, T.halo2); UT(g1beg, T.g1beg); UT(g1ptr, T.g1ptr);
       UT(g1cgid, T.g1cgid); UT(g1ent, T.g1ent);
@@ -451,8 +463,12 @@ void enqueue_tiled_step(tsm_ctx* c, int log_slot, hipEvent_t* ev = nullptr) {
     c->A.W_in = ins[r];
     c->A.W_out = outs[r];
     if (ev) (void)hipEventRecord(ev[e++], c->st);
-    if (tsm::tile_needs_gradient_pass(c->TG, c->P.flux, c->P.reconstruct))
+    if (tsm::tile_needs_gradient_pass(c->TG, c->P.flux, c->P.reconstruct)) {
+      // boundary vertices, then the tiles (measured: on a second stream beside the
+      // tile kernel the two compete for CU slots and the stage takes longer)
+      tsm::launch_tile_gradient_bnd(c->G, c->TG, c->A, c->S, c->st);
       tsm::launch_tile_gradient(c->G, c->TG, c->A, c->S, c->st);
+    }
     if (ev) (void)hipEventRecord(ev[e++], c->st);
     tsm::launch_tile_stage(c->P.flux, c->P.reconstruct, c->G, c->TG, c->A, c->SP, Wn, c->dtpart,
                            c->S, r, c->st);
@@ -466,11 +482,36 @@ void enqueue_tiled_step(tsm_ctx* c, int log_slot, hipEvent_t* ev = nullptr) {
 
 // Steps of the current path; the tiled path first forms dt and the loop
 // test of the first step with the CSR time-step kernel.
+// One HIP graph of graph_steps tiled steps: every kernel argument of a step is
+// fixed (the last stage writes W^n in place) and each sample's stop flag lives
+// on the device, so the graph replays as it was captured.  Recaptured if the
+// state buffer changes.
+hipError_t tiled_graph(tsm_ctx* c) {
+  if (c->gexec && c->gcur == c->cur) return hipSuccess;
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  for (int k = 0; k < c->P.graph_steps; ++k) enqueue_tiled_step(c, -1);
+  e = hipStreamEndCapture(c->st, &g);
+  if (e != hipSuccess) return e;
+  e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e == hipSuccess) c->gcur = c->cur;
+  return e;
+}
+
 void enqueue_steps(tsm_ctx* c, long nsteps, bool with_log) {
   if (c->tiled) {
     c->bind_state();
     if (nsteps > 0) tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
-    for (long k = 0; k < nsteps; ++k) enqueue_tiled_step(c, with_log ? (int)k : -1);
+    long k = 0;
+    if (!with_log && c->P.graph_steps > 0 && nsteps >= c->P.graph_steps &&
+        tiled_graph(c) == hipSuccess) {
+      for (; k + c->P.graph_steps <= nsteps; k += c->P.graph_steps) (void)hipGraphLaunch(c->gexec, c->st);
+      c->bind_state();
+    }
+    for (; k < nsteps; ++k) enqueue_tiled_step(c, with_log ? (int)k : -1);
   } else {
     c->csr_ready();
     for (long k = 0; k < nsteps; ++k) enqueue_step(c, with_log ? (int)k : -1);

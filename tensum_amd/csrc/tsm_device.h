@@ -44,6 +44,9 @@ struct DevGeo {
 struct TileGeo {
   int ntiles, tb, max_slots, max_faces;
   int grad_block, fu_block;  // requested threads per tile (0: smallest that fits)
+  int n_periodic_tiles;      // tiles [0, n) hold the periodic sets
+  int n_gbv;                 // vertices of k_grad_bnd
+  const int* gbv;            // their ids
   int fused, max_slots1;     // two-ring tiles: gradients formed inside the stage kernel
   const int *h2beg, *halo2, *g1beg, *g1ptr, *g1cgid;
   const unsigned* g1ent;
@@ -96,6 +99,8 @@ int tile_grad_block(const TileGeo& T);
 int tile_fu_block(const TileGeo& T);
 int tile_fu_block_max();
 bool tile_needs_gradient_pass(const TileGeo& T, int flux, int rec);
+void launch_tile_gradient_bnd(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
+                              hipStream_t st);
 size_t tile_fu_lds(const TileGeo& T, int flux, int rec);
 void launch_tile_gradient(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
                           hipStream_t st);

@@ -63,6 +63,7 @@ struct Tiling {
   std::vector<int> cgid;             // global cell id per cell entry
   std::vector<int> tper;             // tile holds periodic sets
   std::vector<int> pinfo;            // per new vertex: first member slot << 8 | set size, or -1
+  std::vector<int> gbv;              // outside periodic tiles: vertices with a boundary-corrected cell
   // Second halo ring (meshes without periodic sets): the stage kernel then
   // forms the gradients of the owned and first-ring slots itself.  Slots are
   // owned, first ring (halo), second ring (halo2); the gradient slots are the

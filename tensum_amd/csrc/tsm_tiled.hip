@@ -53,6 +53,42 @@ __device__ __forceinline__ void bsum(double* v, double* red) {
     }
   }
 }
+// Per-block partials of the last stage without a block barrier: every wave
+// reduces its residual-norm sums and smallest local dt by shuffles, its lane 0
+// leaves them in LDS and counts itself in with an LDS integer atomic; the
+// last wave to arrive adds the waves' values in wave order and writes the
+// block's slot.  Waves that finish early leave at once.
+__device__ __forceinline__ void block_partials(double* part, double dtv, double* wred, int* arrive,
+                                               double* partial, double* dtpart, size_t slot) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double x = part[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    part[k] = x;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) dtv = fmin(dtv, __shfl_down(dtv, o, 64));
+  if ((threadIdx.x & 63) != 0) return;
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) wred[5 * w + k] = part[k];
+  wred[5 * w + 4] = dtv;
+  __threadfence_block();
+  if (atomicAdd(arrive, 1) != nw - 1) return;
+  __threadfence_block();
+  double sum[4] = {wred[0], wred[1], wred[2], wred[3]};
+  double m = wred[4];
+  for (int u = 1; u < nw; ++u) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sum[k] += wred[5 * u + k];
+    m = fmin(m, wred[5 * u + 4]);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) partial[slot * 4 + k] = sum[k];
+  dtpart[slot] = m;
+}
+
 __device__ __forceinline__ double bmin(double x, double* red) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_down(x, o, 64));
@@ -148,14 +184,69 @@ __device__ __forceinline__ Cons bflux(const DevGeo& G, const FluxOpts& fo, const
 // ---------------------------------------------------------------------------
 // gradient kernel
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(512, 4) void k_grad_tile(DevGeo G, TileGeo TG, StageArgs A) {
+// Boundary correction of one cell's gradient (src/fv.cc:926-966): for each
+// non-periodic boundary face of the cell, n_b times the mean of the boundary
+// states' entropy variables minus the mean of the vertex ones.
+__device__ __forceinline__ void grad_bnd_correction(const DevGeo& G, const StageArgs& A,
+                                                    const Gas& gas, size_t off, int c,
+                                                    double* g) {
+  for (int z = 0; z < 2; ++z) {
+    const int bf = G.cbf[2 * c + z];
+    if (bf < 0) break;
+    const int v0 = G.bv[2 * bf], v1 = G.bv[2 * bf + 1];
+    const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
+    const int type = G.btype[bf];
+    const Prim w0 = ld_prim(A.W_in, off + v0), w1 = ld_prim(A.W_in, off + v1);
+    const Ent E0 = prim2ent(gas, w0), E1 = prim2ent(gas, w1);
+    Ent q0 = E0, q1 = E1;
+    if (type == TSM_BC_SLIP || type == TSM_BC_NOSLIP || type == TSM_BC_PRESSURE ||
+        type == TSM_BC_INLET) {
+      Prim s0 = w0, s1 = w1;
+      bc1(type, G.bstate + 4 * bf, nx, ny, s0);
+      bc1(type, G.bstate + 4 * bf, nx, ny, s1);
+      q0 = prim2ent(gas, s0);
+      q1 = prim2ent(gas, s1);
+    }
+    const double dq[4] = {(q0.f + q1.f - E0.f - E1.f) / 2.0, (q0.vx + q1.vx - E0.vx - E1.vx) / 2.0,
+                          (q0.vy + q1.vy - E0.vy - E1.vy) / 2.0, (q0.l + q1.l - E0.l - E1.l) / 2.0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      g[2 * q] += nx * dq[q];
+      g[2 * q + 1] += ny * dq[q];
+    }
+  }
+}
+
+// Gradient of one cell from its corners' entropy variables and coordinates:
+// N0 = dr12, N1 = dr20, N2 = dr01 rotated (src/grid_preproc.cc:336-362), then
+// g_k = sum_j N_j V_k(j) (src/fv.cc:900-922).
+__device__ __forceinline__ void cell_grad(double xa, double ya, double xb, double yb, double xd,
+                                          double yd, const double* ea, const double* eb,
+                                          const double* ed, double* g) {
+  const double n0x = (yb - yd), n0y = -(xb - xd);
+  const double n1x = (yd - ya), n1y = -(xd - xa);
+  const double n2x = (ya - yb), n2y = -(xa - xb);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    g[2 * q] = n0x * ea[q] + n1x * eb[q] + n2x * ed[q];
+    g[2 * q + 1] = n0y * ea[q] + n1y * eb[q] + n2y * ed[q];
+  }
+}
+
+// Tiles [t0, t0 + gridDim.x).  PER: periodic tiles, with boundary corrections
+// and the in-tile periodic set sums.  Otherwise vertices with a
+// boundary-corrected cell are left to k_grad_bnd, which keeps this kernel's
+// register count low enough for eight waves per SIMD.
+template <bool PER>
+__global__ __launch_bounds__(512, PER ? 4 : 8) void k_grad_tile(DevGeo G, TileGeo TG, StageArgs A,
+                                                                int t0) {
   extern __shared__ double lds[];
-  const int t = blockIdx.x, s = blockIdx.y;
+  const int t = blockIdx.x + t0, s = blockIdx.y;
   if (A.ctl[s].done) return;
   const int MS = TG.max_slots, TB = TG.tb;
   double* SE = lds;            // 4 x MS entropy variables
   double* SX = lds + 4 * MS;   // 2 x MS coordinates
-  double* SP = lds + 6 * MS;   // 8 x TB periodic exchange
+  double* SP = lds;            // periodic tiles: 8 x TB exchange, after the cell loop
   const int b0 = TG.vbeg[t], nown = TG.vbeg[t + 1] - b0;
   const int h0 = TG.hbeg[t], ns = nown + TG.hbeg[t + 1] - h0;
   const size_t off = (size_t)s * G.nv;
@@ -171,57 +262,24 @@ __global__ __launch_bounds__(512, 4) void k_grad_tile(DevGeo G, TileGeo TG, Stag
   const bool own = tid < nown;
   const int v = b0 + tid;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool skip = false;
   if (own) {
     for (int k = TG.cptr[v]; k < TG.cptr[v + 1]; ++k) {
       const unsigned e = TG.cent[k];
+      if (!PER && (e >> 30)) { skip = true; break; }
       const int a = e & 1023, b = (e >> 10) & 1023, d = (e >> 20) & 1023;
-      const double xa = SX[a], ya = SX[MS + a], xb = SX[b], yb = SX[MS + b], xd = SX[d], yd = SX[MS + d];
-      // cell normals, src/grid_preproc.cc:336-362: N0 = dr12, N1 = dr20, N2 = dr01
-      const double n0x = (yb - yd), n0y = -(xb - xd);
-      const double n1x = (yd - ya), n1y = -(xd - xa);
-      const double n2x = (ya - yb), n2y = -(xa - xb);
+      const double ea[4] = {SE[a], SE[MS + a], SE[2 * MS + a], SE[3 * MS + a]};
+      const double eb[4] = {SE[b], SE[MS + b], SE[2 * MS + b], SE[3 * MS + b]};
+      const double ed[4] = {SE[d], SE[MS + d], SE[2 * MS + d], SE[3 * MS + d]};
       double g[8];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const double ea = SE[q * MS + a], eb = SE[q * MS + b], ed = SE[q * MS + d];
-        g[2 * q] = n0x * ea + n1x * eb + n2x * ed;
-        g[2 * q + 1] = n0y * ea + n1y * eb + n2y * ed;
-      }
-      if (e >> 30) {  // boundary correction, src/fv.cc:926-966
-        const int c = TG.cgid[k];
-        for (int z = 0; z < 2; ++z) {
-          const int bf = G.cbf[2 * c + z];
-          if (bf < 0) break;
-          const int v0 = G.bv[2 * bf], v1 = G.bv[2 * bf + 1];
-          const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
-          const int type = G.btype[bf];
-          const Prim w0 = ld_prim(A.W_in, off + v0), w1 = ld_prim(A.W_in, off + v1);
-          const Ent E0 = prim2ent(gas, w0), E1 = prim2ent(gas, w1);
-          Ent q0 = E0, q1 = E1;
-          if (type == TSM_BC_SLIP || type == TSM_BC_NOSLIP || type == TSM_BC_PRESSURE ||
-              type == TSM_BC_INLET) {
-            Prim s0 = w0, s1 = w1;
-            bc1(type, G.bstate + 4 * bf, nx, ny, s0);
-            bc1(type, G.bstate + 4 * bf, nx, ny, s1);
-            q0 = prim2ent(gas, s0);
-            q1 = prim2ent(gas, s1);
-          }
-          const double dq[4] = {(q0.f + q1.f - E0.f - E1.f) / 2.0,
-                                (q0.vx + q1.vx - E0.vx - E1.vx) / 2.0,
-                                (q0.vy + q1.vy - E0.vy - E1.vy) / 2.0,
-                                (q0.l + q1.l - E0.l - E1.l) / 2.0};
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            g[2 * q] += nx * dq[q];
-            g[2 * q + 1] += ny * dq[q];
-          }
-        }
-      }
+      cell_grad(SX[a], SX[MS + a], SX[b], SX[MS + b], SX[d], SX[MS + d], ea, eb, ed, g);
+      if (PER && (e >> 30)) grad_bnd_correction(G, A, gas, off, TG.cgid[k], g);
 #pragma unroll
       for (int q = 0; q < 8; ++q) acc[q] += g[q];
     }
   }
-  if (TG.tper[t]) {  // periodic sets of this tile: sum members in set order
+  if (PER && TG.tper[t]) {  // periodic sets of this tile: sum members in set order
+    __syncthreads();        // SE no longer read: SP reuses it
     if (own)
 #pragma unroll
       for (int q = 0; q < 8; ++q) SP[q * TB + tid] = acc[q];
@@ -239,7 +297,57 @@ __global__ __launch_bounds__(512, 4) void k_grad_tile(DevGeo G, TileGeo TG, Stag
       }
     }
   }
-  if (own) {
+  if (own && !skip) {
+    const double fact = G.gfact[v];
+    double* dEv = A.dE + 8 * (off + v);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) dEv[q] = acc[q] * fact;
+  }
+}
+
+// Gradients of the vertices k_grad_tile<false> leaves out (a cell with a
+// boundary correction, outside periodic tiles): the same cell order and
+// arithmetic from global memory.  8 lanes per vertex, one cell per lane, then
+// the lane group's first lane adds the cell gradients in cell order (the
+// order of the tile kernel), so the long per-cell load chains run side by side.
+constexpr int GB_LANES = 8;
+__global__ __launch_bounds__(256) void k_grad_bnd(DevGeo G, TileGeo TG, StageArgs A, int nbv) {
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / GB_LANES, j = threadIdx.x % GB_LANES;
+  const int s = blockIdx.y;
+  if (A.ctl[s].done) return;
+  const bool live = i < nbv;
+  const int v = live ? TG.gbv[i] : 0;
+  const int k0 = live ? TG.cptr[v] : 0, nc = live ? TG.cptr[v + 1] - k0 : 0;
+  const size_t off = (size_t)s * G.nv;
+  const Gas gas{A.gamma[s], A.gasR[s]};
+  const int base = (threadIdx.x & 63) & ~(GB_LANES - 1);
+  // the longest cell list of the lane group sets the chunk count for all its lanes
+  int ncm = nc;
+#pragma unroll
+  for (int o = GB_LANES / 2; o > 0; o >>= 1) ncm = max(ncm, __shfl_xor(ncm, o, 64));
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int c0 = 0; c0 < ncm; c0 += GB_LANES) {  // chunks of GB_LANES cells, in cell order
+    double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (c0 + j < nc) {
+      const int k = k0 + c0 + j, c = TG.cgid[k];
+      const int va = G.cv[3 * c], vb = G.cv[3 * c + 1], vd = G.cv[3 * c + 2];
+      const Ent Ea = prim2ent(gas, ld_prim(A.W_in, off + va));
+      const Ent Eb = prim2ent(gas, ld_prim(A.W_in, off + vb));
+      const Ent Ed = prim2ent(gas, ld_prim(A.W_in, off + vd));
+      const double ea[4] = {Ea.f, Ea.vx, Ea.vy, Ea.l}, eb[4] = {Eb.f, Eb.vx, Eb.vy, Eb.l},
+                   ed[4] = {Ed.f, Ed.vx, Ed.vy, Ed.l};
+      cell_grad(G.x[va], G.y[va], G.x[vb], G.y[vb], G.x[vd], G.y[vd], ea, eb, ed, g);
+      if (TG.cent[k] >> 30) grad_bnd_correction(G, A, gas, off, c, g);
+    }
+    for (int m = 0; m < GB_LANES; ++m) {  // every lane takes part in the shuffles
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const double x = __shfl(g[q], base + m, 64);
+        if (c0 + m < nc) acc[q] += x;
+      }
+    }
+  }
+  if (live && j == 0) {
     const double fact = G.gfact[v];
     double* dEv = A.dE + 8 * (off + v);
 #pragma unroll
@@ -275,25 +383,27 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
                                                  const double* Wn, double* dtpart, int r) {
   using L = SlotLayout<FLUX, REC>;
   extern __shared__ double lds[];
+  __shared__ double wred[8 * 5];
+  __shared__ int arrive;
   const int t = blockIdx.x, s = blockIdx.y;
   if (A.ctl[s].done) return;
   const int MS = TG.max_slots, MF = TG.max_faces, TB = TG.tb;
   // LDS: slot rows Q (NQ x MS).  Once the face fluxes are in registers the
   // rows past the primitive state are dead and hold the face fluxes SF
-  // (4 x MF) and, in the last stage, the face normals SN (2 x MF); the
+  // (4 x MF) and, in the last stage, the face normals and areas SN (3 x MF); the
   // periodic exchange SR (4 x TB) reuses the primitive rows once every owned
   // state has been read.
   double* Q = lds;
   double* SF = lds + 4 * MS;
   double* SN = SF + 4 * MF;
   double* SR = lds;
-  double* red = lds + 4 * MS + max((L::NQ - 4) * MS, 6 * MF);  // 16 reduction scratch
   const int b0 = TG.vbeg[t], nown = TG.vbeg[t + 1] - b0;
   const int h0 = TG.hbeg[t], ns = nown + TG.hbeg[t + 1] - h0;
   const int f0 = TG.fbeg[t], nfl = TG.fbeg[t + 1] - f0;
   const size_t off = (size_t)s * G.nv;
   const Gas gas{A.gamma[s], A.gasR[s]};
 
+  if (LAST && threadIdx.x == 0) arrive = 0;  // read after the phase barriers
   // ---- phase 1: slot quantities ------------------------------------------
   for (int i = threadIdx.x; i < ns; i += blockDim.x) {
     const int gid = (i < nown) ? b0 + i : TG.halo[h0 + i - nown];
@@ -315,7 +425,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
   __syncthreads();
 
   // ---- phase 2: local face fluxes (registers, then LDS) ------------------
-  double fl[FPT][4], fn[FPT][2];
+  double fl[FPT][4], fn[FPT][3];
 #pragma unroll
   for (int q = 0; q < FPT; ++q) {
     const int j = threadIdx.x + q * blockDim.x;
@@ -346,7 +456,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
     }
     const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gl, gr, nx * ia, ny * ia);
     fl[q][0] = F.m * area; fl[q][1] = F.mx * area; fl[q][2] = F.my * area; fl[q][3] = F.e * area;
-    fn[q][0] = nx; fn[q][1] = ny;
+    fn[q][0] = nx; fn[q][1] = ny; fn[q][2] = area;
   }
   __syncthreads();  // every slot row read: rows 4.. become SF / SN
 #pragma unroll
@@ -354,7 +464,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
     const int j = threadIdx.x + q * blockDim.x;
     if (j >= nfl) continue;
     SF[j] = fl[q][0]; SF[MF + j] = fl[q][1]; SF[2 * MF + j] = fl[q][2]; SF[3 * MF + j] = fl[q][3];
-    if (LAST) { SN[j] = fn[q][0]; SN[MF + j] = fn[q][1]; }
+    if (LAST) { SN[j] = fn[q][0]; SN[MF + j] = fn[q][1]; SN[2 * MF + j] = fn[q][2]; }
   }
   __syncthreads();
 
@@ -442,7 +552,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
           lam += 0.5 * (fabs(vn) + cm * ar);
         } else {
           const int j = e & 0x3fffu;
-          const double nx = SN[j], ny = SN[MF + j], ar = sqrt(nx * nx + ny * ny);
+          const double nx = SN[j], ny = SN[MF + j], ar = SN[2 * MF + j];
           const double vn = w1.u * nx + w1.v * ny;
           lam += (fabs(vn) + cm * ar) * 1.0;
         }
@@ -466,15 +576,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
     }
     double dtv = 1.0e20;
     if (own) dtv = (P.time_step > 0.0) ? P.time_step : P.cfl * G.dcarea[v] / lam;
-    bsum<4>(part, red);
-    if (threadIdx.x == 0) {
-      double* pp = A.partial + ((size_t)s * gridDim.x + t) * 4;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) pp[k] = part[k];
-    }
-    __syncthreads();
-    const double m = bmin(dtv, red);
-    if (threadIdx.x == 0) dtpart[(size_t)s * gridDim.x + t] = m;
+    block_partials(part, dtv, wred, &arrive, A.partial, dtpart, (size_t)s * gridDim.x + t);
   }
 }
 
@@ -503,6 +605,8 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
                                                                   double* dtpart, int r) {
   using FL = FusedLayout<FLUX, REC>;
   extern __shared__ double lds[];
+  __shared__ double wred[8 * 5];
+  __shared__ int arrive;
   const int t = blockIdx.x, s = blockIdx.y;
   if (A.ctl[s].done) return;
   const int MS1 = TG.max_slots1, MS2 = TG.max_slots, MF = TG.max_faces;
@@ -513,8 +617,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
   double* SE = Q + FL::R1 * MS1;            // 4 x MS2 gradient entropy variables
   double* SX = SE + 4 * MS2;                // 2 x MS2 coordinates
   double* SF = lds + 4 * MS1;               // after the flux phase: 4 x MF fluxes
-  double* SN = SF + 4 * MF;                 // last stage: 2 x MF normals
-  double* red = lds + 4 * MS1 + max((FL::R1 - 4) * MS1 + FL::R2 * MS2, 6 * MF);
+  double* SN = SF + 4 * MF;                 // last stage: 3 x MF normals, areas
   const int b0 = TG.vbeg[t], nown = TG.vbeg[t + 1] - b0;
   const int h0 = TG.hbeg[t], n1 = nown + TG.hbeg[t + 1] - h0;
   const int k0 = TG.h2beg[t], ns = n1 + TG.h2beg[t + 1] - k0;
@@ -522,6 +625,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
   const size_t off = (size_t)s * G.nv;
   const Gas gas{A.gamma[s], A.gasR[s]};
 
+  if (LAST && threadIdx.x == 0) arrive = 0;  // read after the phase barriers
   // ---- phase 1: slot quantities (two rings) ------------------------------
   for (int i = threadIdx.x; i < ns; i += blockDim.x) {
     const int gid = (i < nown) ? b0 + i : (i < n1 ? TG.halo[h0 + i - nown] : TG.halo2[k0 + i - n1]);
@@ -599,7 +703,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
   __syncthreads();
 
   // ---- phase 3: local face fluxes (registers, then LDS) ------------------
-  double fl[FPT][4], fn[FPT][2];
+  double fl[FPT][4], fn[FPT][3];
 #pragma unroll
   for (int q = 0; q < FPT; ++q) {
     const int j = threadIdx.x + q * blockDim.x;
@@ -624,7 +728,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
     const double rx = SX[rr] - SX[l], ry = SX[MS2 + rr] - SX[MS2 + l];
     const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gl, gr, nx * ia, ny * ia);
     fl[q][0] = F.m * area; fl[q][1] = F.mx * area; fl[q][2] = F.my * area; fl[q][3] = F.e * area;
-    fn[q][0] = nx; fn[q][1] = ny;
+    fn[q][0] = nx; fn[q][1] = ny; fn[q][2] = area;
   }
   __syncthreads();  // every slot row read: rows past the primitive state become SF / SN
 #pragma unroll
@@ -632,7 +736,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
     const int j = threadIdx.x + q * blockDim.x;
     if (j >= nfl) continue;
     SF[j] = fl[q][0]; SF[MF + j] = fl[q][1]; SF[2 * MF + j] = fl[q][2]; SF[3 * MF + j] = fl[q][3];
-    if (LAST) { SN[j] = fn[q][0]; SN[MF + j] = fn[q][1]; }
+    if (LAST) { SN[j] = fn[q][0]; SN[MF + j] = fn[q][1]; SN[2 * MF + j] = fn[q][2]; }
   }
   __syncthreads();
 
@@ -696,7 +800,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
           lam += 0.5 * (fabs(vn) + cm * ar);
         } else {
           const int j = e & 0x3fffu;
-          const double nx = SN[j], ny = SN[MF + j], ar = sqrt(nx * nx + ny * ny);
+          const double nx = SN[j], ny = SN[MF + j], ar = SN[2 * MF + j];
           const double vn = w1.u * nx + w1.v * ny;
           lam += (fabs(vn) + cm * ar) * 1.0;
         }
@@ -706,15 +810,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
   if (LAST) {
     double dtv = 1.0e20;
     if (own) dtv = (P.time_step > 0.0) ? P.time_step : P.cfl * G.dcarea[v] / lam;
-    bsum<4>(part, red);
-    if (threadIdx.x == 0) {
-      double* pp = A.partial + ((size_t)s * gridDim.x + t) * 4;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) pp[k] = part[k];
-    }
-    __syncthreads();
-    const double m = bmin(dtv, red);
-    if (threadIdx.x == 0) dtpart[(size_t)s * gridDim.x + t] = m;
+    block_partials(part, dtv, wred, &arrive, A.partial, dtpart, (size_t)s * gridDim.x + t);
   }
 }
 
@@ -728,15 +824,28 @@ __global__ __launch_bounds__(1024) void k_step_final_tiled(StageArgs A, StepPara
   Ctl& c = A.ctl[s];
   if (c.done) return;
   double part[4] = {0, 0, 0, 0};
-  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
-    const double* q = A.partial + ((size_t)s * nblk + i) * 4;
+  double x = 1.0e20;
+  const double4* q4 = reinterpret_cast<const double4*>(A.partial) + (size_t)s * nblk;
+  const double* dq = dtpart + (size_t)s * nblk;
+  int i = threadIdx.x;
+  for (; i + 3 * (int)blockDim.x < nblk; i += 4 * blockDim.x) {  // four independent loads in flight
+    double4 a[4];
+    double d[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) part[k] += q[k];
+    for (int u = 0; u < 4; ++u) { a[u] = q4[i + u * blockDim.x]; d[u] = dq[i + u * blockDim.x]; }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      part[0] += a[u].x; part[1] += a[u].y; part[2] += a[u].z; part[3] += a[u].w;
+      x = fmin(x, d[u]);
+    }
+  }
+  for (; i < nblk; i += blockDim.x) {
+    const double4 a = q4[i];
+    part[0] += a.x; part[1] += a.y; part[2] += a.z; part[3] += a.w;
+    x = fmin(x, dq[i]);
   }
   bsum<4>(part, red);
   __syncthreads();
-  double x = 1.0e20;
-  for (int i = threadIdx.x; i < nblk; i += blockDim.x) x = fmin(x, dtpart[(size_t)s * nblk + i]);
   x = bmin(x, red);
   if (threadIdx.x == 0) {
     double n[4];
@@ -778,7 +887,7 @@ int tile_grad_block(const TileGeo& T) {
 }
 
 size_t tile_grad_lds(const TileGeo& T) {
-  return sizeof(double) * (6 * (size_t)T.max_slots + 8 * (size_t)T.tb);
+  return sizeof(double) * std::max(6 * (size_t)T.max_slots, 8 * (size_t)T.tb);
 }
 
 namespace {
@@ -786,14 +895,14 @@ template <int FLUX, int REC>
 size_t fu_lds(const TileGeo& T, bool) {
   const size_t MS = T.max_slots, MF = T.max_faces;
   return sizeof(double) *
-         (4 * MS + std::max((size_t)(SlotLayout<FLUX, REC>::NQ - 4) * MS, 6 * MF) + 16);
+         (4 * MS + std::max((size_t)(SlotLayout<FLUX, REC>::NQ - 4) * MS, 7 * MF) + 16);
 }
 template <int FLUX, int REC>
 size_t fused_lds(const TileGeo& T) {
   using FL = FusedLayout<FLUX, REC>;
   const size_t MS1 = T.max_slots1, MS2 = T.max_slots, MF = T.max_faces;
   return sizeof(double) *
-         (4 * MS1 + std::max((size_t)(FL::R1 - 4) * MS1 + (size_t)FL::R2 * MS2, 6 * MF) + 16);
+         (4 * MS1 + std::max((size_t)(FL::R1 - 4) * MS1 + (size_t)FL::R2 * MS2, 7 * MF) + 16);
 }
 template <int FLUX, int REC>
 void fu_launch(const DevGeo& G, const TileGeo& T, const StageArgs& A, const StepParams& P,
@@ -867,9 +976,26 @@ bool tile_needs_gradient_pass(const TileGeo& T, int flux, int rec) {
 void launch_tile_gradient(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
                           hipStream_t st) {
   const size_t lds = tile_grad_lds(T);
-  (void)hipFuncSetAttribute((const void*)k_grad_tile, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds);
-  hipLaunchKernelGGL(k_grad_tile, dim3(T.ntiles, S), dim3(tile_grad_block(T)), lds, st, G, T, A);
+  const int np = T.n_periodic_tiles, nrest = T.ntiles - np;
+  if (np > 0) {
+    (void)hipFuncSetAttribute((const void*)k_grad_tile<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_grad_tile<true>, dim3(np, S), dim3(tile_grad_block(T)), lds, st, G, T, A, 0);
+  }
+  if (nrest > 0) {
+    (void)hipFuncSetAttribute((const void*)k_grad_tile<false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_grad_tile<false>, dim3(nrest, S), dim3(tile_grad_block(T)), lds, st, G, T,
+                       A, np);
+  }
+
+}
+
+void launch_tile_gradient_bnd(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
+                              hipStream_t st) {
+  if (T.n_gbv > 0)
+    hipLaunchKernelGGL(k_grad_bnd, dim3((T.n_gbv * GB_LANES + 255) / 256, S), dim3(256), 0, st, G,
+                       T, A, T.n_gbv);
 }
 
 void launch_tile_stage(int flux, int rec, const DevGeo& G, const TileGeo& T, const StageArgs& A,

@@ -138,11 +138,18 @@ def make_batch(problem: Problem, mesh, inputs):
     return s
 
 
-def qoi(problem: Problem, solver: Solver):
-    """(n_samples, n_probe + 2): probe densities, kinetic energy, entropy integral."""
+def qoi(problem: Problem, solver: Solver, max_density=1e6):
+    """(n_samples, n_probe + 2): probe densities, kinetic energy, entropy
+    integral.  Raises FloatingPointError naming the samples whose state is not
+    finite or whose density left [0, max_density] (a run that blew up)."""
     rho = solver.probe_density(*problem.probes)
     ke, ent = solver.global_all()
-    return np.hstack([rho, ke[:, None], ent[:, None]])
+    Q = np.hstack([rho, ke[:, None], ent[:, None]])
+    bad = ~np.all(np.isfinite(Q), axis=1) | np.any((rho < 0) | (rho > max_density), axis=1)
+    if np.any(bad):
+        raise FloatingPointError(f"{problem.name}: samples {np.flatnonzero(bad).tolist()} "
+                                 "have a non-finite or non-physical state")
+    return Q
 
 
 def advance(solvers, chunk=64, max_chunks=100000):
@@ -226,3 +233,119 @@ def mlmc(problem: Problem, levels, samples, seed=0):
         if coarse is not None:
             coarse.close()
     return Estimate(mean, sec - mean * mean, np.sqrt(se2), time.perf_counter() - t0, detail)
+
+
+# ---------------------------------------------------------------------------
+# surrogate control-variate experiment (error per GPU-second)
+# ---------------------------------------------------------------------------
+def _unit(problem: Problem, inputs):
+    return np.stack([inputs[b[0]] for b in problem.bounds], 1)
+
+
+def _sync_time(t0):
+    return time.perf_counter() - t0
+
+
+def cv_experiment(problem: Problem, n_fine, n_coarse, n_train, n_pilot, n_pair, n_surr, n_ref,
+                  epochs=300, batch=64, seed=0, log=None):
+    """Surrogate control variate against plain MC on the same fine mesh.
+
+    Reference: plain MC with n_ref fine samples (its own cost is not charged).
+    Control variate: an MLP trained on n_train coarse-mesh solver samples, its
+    coefficients fitted on n_pilot fine samples, then n_pair fine samples and
+    n_surr surrogate evaluations.  Returns the realised mean and variance
+    errors of both estimators against the reference (RMS over the QoIs with
+    non-zero reference variance, each scaled by the reference's standard
+    deviation), their GPU-seconds, and the GPU-seconds plain MC would need to
+    reach the control variate's standard error."""
+    from .surrogate import Surrogate, _alpha, control_variate
+    say = log or (lambda *_: None)
+    t0 = time.perf_counter()
+    _, Q_ref = plain_mc(problem, n_fine, n_ref, seed=seed + 1000)
+    t_ref = _sync_time(t0)
+    m_ref, v_ref = moments(Q_ref)
+    keep = v_ref > 1e-24 * np.maximum(1.0, m_ref ** 2)
+    sd = np.sqrt(v_ref[keep])
+    per_fine_sample = t_ref / n_ref
+    say(f"reference: {n_ref} samples at {n_fine}^2 in {t_ref:.1f} s")
+
+    t0 = time.perf_counter()
+    u_tr, Q_tr = plain_mc(problem, n_coarse, n_train, seed=seed + 1)
+    t_train_sims = _sync_time(t0)
+    t0 = time.perf_counter()
+    sur = Surrogate(len(problem.bounds), problem.n_qoi, max_batch=max(batch, 4096), seed=seed + 2)
+    losses = sur.fit(_unit(problem, u_tr), Q_tr, epochs=epochs, batch=batch)
+    t_fit = _sync_time(t0)
+    t0 = time.perf_counter()
+    u_a, Q_a = plain_mc(problem, n_fine, n_pilot, seed=seed + 3)
+    u_p, Q_p = plain_mc(problem, n_fine, n_pair, seed=seed + 4)
+    t_fine = _sync_time(t0)
+    t0 = time.perf_counter()
+    G_a = sur(_unit(problem, u_a))
+    G_p = sur(_unit(problem, u_p))
+    G_m = sur(_unit(problem, sample_inputs(n_surr, problem.bounds, seed=seed + 5)))
+    t_eval = _sync_time(t0)
+    est = control_variate(Q_p, G_p, G_m, alpha=_alpha(Q_a, G_a), alpha2=_alpha(Q_a ** 2, G_a ** 2))
+    cv_seconds = t_train_sims + t_fit + t_fine + t_eval
+
+    # plain MC with the same fine-sample budget as the control variate
+    n_plain = n_pilot + n_pair
+    Q_plain = np.vstack([Q_a, Q_p])
+    m_pl, v_pl = moments(Q_plain)
+
+    def err(m, v):
+        return (float(np.sqrt(np.mean(((m[keep] - m_ref[keep]) / sd) ** 2))),
+                float(np.sqrt(np.mean(((v[keep] - v_ref[keep]) / v_ref[keep]) ** 2))))
+
+    e_cv, e_pl = err(est.mean, est.var), err(m_pl, v_pl)
+    # GPU-seconds plain MC needs for the control variate's standard error
+    se_ratio = np.median((np.sqrt(v_ref[keep] / n_pair) / np.maximum(est.mean_se[keep], 1e-300)) ** 2)
+    plain_seconds_equal_error = float(se_ratio) * n_pair * per_fine_sample
+    return {
+        "reference": {"samples": n_ref, "mesh": n_fine, "seconds": t_ref},
+        "control_variate": {"mean_error": e_cv[0], "var_error": e_cv[1], "gpu_seconds": cv_seconds,
+                            "train_samples": n_train, "train_mesh": n_coarse, "pilot": n_pilot,
+                            "paired": n_pair, "surrogate_evals": n_surr,
+                            "final_loss": float(losses[-1]) if len(losses) else None,
+                            "median_variance_reduction": float(np.median(est.variance_reduction[keep])),
+                            "seconds": {"train_sims": t_train_sims, "fit": t_fit, "fine": t_fine,
+                                        "surrogate": t_eval}},
+        "plain_mc": {"mean_error": e_pl[0], "var_error": e_pl[1], "samples": n_plain,
+                     "gpu_seconds": n_plain * per_fine_sample},
+        "plain_seconds_for_cv_error": plain_seconds_equal_error,
+        "speedup_at_equal_error": plain_seconds_equal_error / cv_seconds,
+        "qoi_kept": int(np.sum(keep)),
+    }
+
+
+def mlmc_experiment(problem: Problem, levels, samples, n_ref, seed=0, log=None):
+    """Multilevel MC against plain MC on the finest level at the same
+    GPU-seconds, both judged against a plain-MC reference with n_ref samples
+    on the finest mesh.  Errors: RMS over the QoIs with non-zero reference
+    variance of |estimate - reference| scaled by the reference's standard
+    deviation (means) or variance (variances)."""
+    say = log or (lambda *_: None)
+    t0 = time.perf_counter()
+    _, Q_ref = plain_mc(problem, levels[-1], n_ref, seed=seed + 1000)
+    t_ref = time.perf_counter() - t0
+    m_ref, v_ref = moments(Q_ref)
+    keep = v_ref > 1e-24 * np.maximum(1.0, m_ref ** 2)
+    say(f"reference: {n_ref} samples at {levels[-1]}^2 in {t_ref:.1f} s")
+    est = mlmc(problem, levels, samples, seed=seed)
+    n_plain = max(2, int(round(est.seconds / (t_ref / n_ref))))
+    t0 = time.perf_counter()
+    _, Q_pl = plain_mc(problem, levels[-1], n_plain, seed=seed + 7)
+    t_pl = time.perf_counter() - t0
+    m_pl, v_pl = moments(Q_pl)
+
+    def err(m, v):
+        return (float(np.sqrt(np.mean(((m[keep] - m_ref[keep]) / np.sqrt(v_ref[keep])) ** 2))),
+                float(np.sqrt(np.mean(((v[keep] - v_ref[keep]) / v_ref[keep]) ** 2))))
+
+    e_ml, e_pl = err(est.mean, est.var), err(m_pl, v_pl)
+    return {"reference": {"samples": n_ref, "mesh": levels[-1], "seconds": t_ref},
+            "mlmc": {"levels": list(levels), "samples": list(samples), "seconds": est.seconds,
+                     "mean_error": e_ml[0], "var_error": e_ml[1], "detail": est.detail},
+            "plain_mc": {"samples": n_plain, "seconds": t_pl, "mean_error": e_pl[0],
+                         "var_error": e_pl[1]},
+            "qoi_kept": int(np.sum(keep))}

@@ -324,8 +324,8 @@ def test_fused_two_ring_stage_equals_separate_gradient_pass(flux, rec):
     w[:, 0] *= 1 + 0.1 * np.sin(2 * np.pi * m.x) * np.sin(2 * np.pi * m.y)
     w[:, 3] *= 1 + 0.05 * np.cos(2 * np.pi * m.x)
     out = []
-    for sep in (False, True):
-        prm = T.Params(flux=flux, reconstruct=rec, cfl=0.4, final_time=0.8, separate_gradient=sep)
+    for fused in (True, False):
+        prm = T.Params(flux=flux, reconstruct=rec, cfl=0.4, final_time=0.8, fused_gradient=fused)
         s = T.Solver(m, prm)
         s.set_state(w)
         s.run(15)
@@ -348,3 +348,75 @@ def test_autotuned_shape_gives_identical_results():
         s.run(12)
         out.append(s.get_state())
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+@pytest.mark.parametrize("mesh_kind", ["riemann", "vortex"])
+def test_graph_replays_equal_eager_steps(mesh_kind):
+    """Steps replayed from a captured HIP graph (5 steps per graph) equal
+    host-launched steps bit for bit, replay after replay, with eager work
+    (state reads, residual calls, a second sample's reset) between replays."""
+    if mesh_kind == "riemann":
+        m = riemann_mesh(41)
+        prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8)
+        w = riemann(m.x, m.y)
+    else:
+        m = vortex_mesh(33)
+        prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=10.0)
+        w = vortex(m.x, m.y)
+    g = T.Solver(m, T.Params(**{**prm.__dict__, "graph_steps": 5}), n_samples=2)
+    e = T.Solver(m, prm, n_samples=2)
+    for s in (g, e):
+        s.set_state(w, 0)
+        s.set_state(w, 1)
+    for chunk in (12, 5, 10, 3):
+        dg, de = g.run(chunk), e.run(chunk)
+        assert np.array_equal(dg, de)
+        for k in (0, 1):
+            assert np.array_equal(g.get_state(k), e.get_state(k))
+            assert g.time(k) == e.time(k)
+        g.residual(0)  # eager CSR work between replays
+        e.residual(0)
+    w2 = g.get_state(0)
+    g.set_state(w2, 1)
+    e.set_state(w2, 1)
+    g.run(10)
+    e.run(10)
+    assert np.array_equal(g.get_state(1), e.get_state(1))
+
+
+def _fan_mesh(n=12):
+    """Half disk around the centre of its diameter: an inner fan of n
+    triangles (so the centre, a boundary vertex, has n cells, more than one
+    lane group of the boundary-gradient kernel takes at once) and an outer band
+    of 2n triangles."""
+    th = np.pi * np.arange(n + 1) / n
+    x = np.concatenate([[0.0], 0.5 * np.cos(th), np.cos(th)])
+    y = np.concatenate([[0.0], 0.5 * np.sin(th), np.sin(th)])
+    inn = 1 + np.arange(n + 1)
+    out = n + 2 + np.arange(n + 1)
+    cells = [(0, inn[k], inn[k + 1]) for k in range(n)]
+    for k in range(n):
+        cells += [(inn[k], out[k], out[k + 1]), (inn[k], out[k + 1], inn[k + 1])]
+    # boundary counter-clockwise: the diameter from (-1,0) to (1,0), then the arc
+    bf = [(out[n], inn[n], 1), (inn[n], 0, 1), (0, inn[0], 1), (inn[0], out[0], 1)]
+    bf += [(out[k], out[k + 1], 2) for k in range(n)]
+    return T.Mesh(x=x, y=y, bface_tag=np.array([t for _, _, t in bf], np.int32),
+                  bface_v=np.array([[a, b] for a, b, _ in bf], np.int32).ravel(),
+                  cell_v=np.array(cells, np.int32).ravel(),
+                  periodic_ptr=np.zeros(0, np.int32), periodic_v=np.zeros(0, np.int32),
+                  bc={1: ("slip",), 2: ("outlet",)})
+
+
+def test_boundary_vertex_with_many_cells():
+    m = _fan_mesh(12)
+    st = np.array([1.0, 0.2, 0.1, 1.0])
+    w = np.tile(st, (m.n_vertex, 1))
+    w[:, 3] *= 1 + 0.05 * np.cos(3 * m.x) * (1 + m.y)
+    for fused in (False, True):
+        prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.3, final_time=1.0,
+                       fused_gradient=fused)
+        s, o = _pair(m, prm, w)
+        assert rel_l2(s.gradients(), o.gradients()) < 1e-12
+        s.run(6)
+        o.run(6)
+        assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-12

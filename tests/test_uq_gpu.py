@@ -51,27 +51,40 @@ def test_mlmc_agrees_with_plain_mc_on_the_finest_level():
 
 
 def test_surrogate_control_variate_beats_plain_mc_on_solver_qoi():
-    """Riemann problem with an uncertain split and pressures on a 65^2 mesh:
-    the surrogate is trained on 512 solver samples, its control coefficients
-    fitted on 32 independent ones, and it then serves as control variate for
-    48 further solver samples plus 8192 surrogate evaluations.  The integrated
-    kinetic energy and entropy depend smoothly on the inputs: there the
-    variance of the estimate must drop at least 4x against the plain mean of
-    the same 48 solver samples."""
-    p = uq.riemann_problem(final_time=0.05)
+    """Isentropic vortex with uncertain strength, centre, gamma and Mach on a
+    33^2 mesh (t = 0.5): the surrogate is trained on 512 solver samples, its
+    control coefficients fitted on 32 independent ones, and it then serves as
+    control variate for 48 further solver samples plus 8192 surrogate
+    evaluations.  The QoIs depend smoothly on the inputs, so the variance of
+    the estimate must drop at least 4x against the plain mean of the same 48
+    solver samples (integrated kinetic energy and entropy, and the median over
+    the probes that vary)."""
+    p = uq.vortex_problem(final_time=0.5)
 
     def unit(inp):
         return np.stack([inp[b[0]] for b in p.bounds], 1)
 
-    u_tr, Q_tr = uq.plain_mc(p, 65, 512, seed=11)
-    s = Surrogate(6, p.n_qoi, max_batch=512, seed=2)
+    u_tr, Q_tr = uq.plain_mc(p, 33, 512, seed=11)
+    s = Surrogate(5, p.n_qoi, max_batch=512, seed=2)
     s.fit(unit(u_tr), Q_tr, epochs=300, batch=64, lr=2e-3)
-    u_a, Q_a = uq.plain_mc(p, 65, 32, seed=14)
+    u_a, Q_a = uq.plain_mc(p, 33, 32, seed=14)
     G_a = s(unit(u_a))
-    u_p, Q_p = uq.plain_mc(p, 65, 48, seed=12)
+    u_p, Q_p = uq.plain_mc(p, 33, 48, seed=12)
     U_m = unit(uq.sample_inputs(8192, p.bounds, seed=13))
     est = control_variate(Q_p, s(unit(u_p)), s(U_m), alpha=_alpha(Q_a, G_a),
                           alpha2=_alpha(Q_a ** 2, G_a ** 2))
-    print("variance reduction (probes median, KE, entropy):",
-          np.median(est.variance_reduction[:-2]), est.variance_reduction[-2:])
-    assert np.all(est.variance_reduction[-2:] > 4.0), est.variance_reduction[-2:]
+    varying = est.plain_mean_se[:-2] > 1e-6 * np.abs(est.mean[:-2])
+    vr_probes = np.median(est.variance_reduction[:-2][varying])
+    print("variance reduction (probes median, KE, entropy):", vr_probes, est.variance_reduction[-2:])
+    assert np.all(est.variance_reduction[-2:] > 4.0) and vr_probes > 4.0
+
+
+def test_exploded_samples_are_refused():
+    p = uq.vortex_problem(final_time=0.05)
+    inp = uq.sample_inputs(2, p.bounds, seed=1)
+    s = uq.make_batch(p, p.mesh(17), inp)
+    w = s.get_state(1)
+    w[:, 3] *= 1e9  # a state whose density leaves any physical range
+    s.set_state(w, 1)
+    with pytest.raises(FloatingPointError):
+        uq.qoi(p, s)
