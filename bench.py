@@ -91,16 +91,17 @@ def main():
                     help="second metric: surrogate control-variate MC against plain MC on the "
                          "uncertain isentropic vortex (prints its own JSON line)")
     ap.add_argument("--mc-fine", type=int, default=129)
-    ap.add_argument("--mc-coarse", type=int, default=65)
+    ap.add_argument("--mc-coarse", type=int, default=129, help="mesh of the surrogate's training samples")
     ap.add_argument("--mc-time", type=float, default=10.0)
-    ap.add_argument("--mc-train", type=int, default=1024)
-    ap.add_argument("--mc-pair", type=int, default=64)
-    ap.add_argument("--mc-ref", type=int, default=2048)
+    ap.add_argument("--mc-train", type=int, default=512)
+    ap.add_argument("--mc-pilot", type=int, default=64)
+    ap.add_argument("--mc-pair", type=int, default=1024)
+    ap.add_argument("--mc-ref", type=int, default=4096)
     ap.add_argument("--mlmc", action="store_true",
                     help="multilevel MC of the uncertain vortex against plain MC at equal "
                          "GPU-seconds (prints its own JSON line)")
-    ap.add_argument("--mlmc-levels", default="65,129,257")
-    ap.add_argument("--mlmc-samples", default="1024,256,64")
+    ap.add_argument("--mlmc-levels", default="129,257")
+    ap.add_argument("--mlmc-samples", default="512,64")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -224,7 +225,7 @@ def mc_bench(args):
     from tensum_amd import uq
     p = uq.vortex_problem(final_time=args.mc_time)
     say = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
-    r = uq.cv_experiment(p, args.mc_fine, args.mc_coarse, args.mc_train, args.mc_pair,
+    r = uq.cv_experiment(p, args.mc_fine, args.mc_coarse, args.mc_train, args.mc_pilot,
                          args.mc_pair, 65536, args.mc_ref, log=say)
     out = {
         "metric": "MC moment error per GPU-second (control variate vs plain MC, equal error)",

@@ -257,11 +257,16 @@ bool validate_tiles(const Geometry& g, const Tiling& T, std::string& msg) {
     for (int j = T.bbeg[t]; j < T.bbeg[t + 1]; ++j)
       if (T.tbf[j] < 0 || T.tbf[j] >= (int)g.bface.size()) { msg = "bface id"; return false; }
     for (int v = T.vbeg[t]; v < T.vbeg[t + 1]; ++v) {
+      bool seen_bface = false;  // the kernels take a vertex's faces before its boundary faces
       for (int k = T.gptr[v]; k < T.gptr[v + 1]; ++k) {
         const unsigned e = T.gent[k];
         if (e & 0x4000u) {
           if ((int)(e & 0x3fffu) >= nbt) { msg = "gather bface"; return false; }
-        } else if ((int)(e & 0x3fffu) >= nfl) { msg = "gather face"; return false; }
+          seen_bface = true;
+        } else {
+          if ((int)(e & 0x3fffu) >= nfl) { msg = "gather face"; return false; }
+          if (seen_bface) { msg = "gather face after a boundary face"; return false; }
+        }
       }
       for (int k = T.cptr[v]; k < T.cptr[v + 1]; ++k) {
         const unsigned e = T.cent[k];

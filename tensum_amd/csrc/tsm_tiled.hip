@@ -53,6 +53,27 @@ __device__ __forceinline__ void bsum(double* v, double* red) {
     }
   }
 }
+// Face entries of a vertex's gather held in registers (see k_fu_tile).
+constexpr int GE = 12;
+
+// One face entry of a vertex's gather: +F at the left end, -F at the right.
+__device__ __forceinline__ void gather_face(unsigned e, const double* SF, int MF, double* acc) {
+  const int j = e & 0x3fffu;
+  const double q0 = SF[j], q1 = SF[MF + j], q2 = SF[2 * MF + j], q3 = SF[3 * MF + j];
+  if (e & 0x8000u) { acc[0] -= q0; acc[1] -= q1; acc[2] -= q2; acc[3] -= q3; }
+  else { acc[0] += q0; acc[1] += q1; acc[2] += q2; acc[3] += q3; }
+}
+
+// A face's share of a vertex's spectral radius, |u.n| + c|n|, with the
+// vertex's own new state (src/fv.cc:1511-1531).
+__device__ __forceinline__ double face_lambda(unsigned e, const double* SN, int MF, const Prim& w1,
+                                              double cm) {
+  const int j = e & 0x3fffu;
+  const double nx = SN[j], ny = SN[MF + j], ar = SN[2 * MF + j];
+  const double vn = w1.u * nx + w1.v * ny;
+  return (fabs(vn) + cm * ar) * 1.0;
+}
+
 // Per-block partials of the last stage without a block barrier: every wave
 // reduces its residual-norm sums and smallest local dt by shuffles, its lane 0
 // leaves them in LDS and counts itself in with an LDS integer atomic; the
@@ -478,18 +499,28 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
   if (own) w0 = Prim{Q[tid], Q[MS + tid], Q[2 * MS + tid], Q[3 * MS + tid]};
   const bool per = TG.tper[t] != 0;
   if (per) __syncthreads();  // owned states read: SR may overwrite the primitive rows
+  // A vertex's gather entries: its faces, then its boundary faces.  The first
+  // GE face entries are loaded together into registers (and reused by the
+  // last stage's time-step loop); boundary faces and longer lists follow from
+  // memory, so every sum keeps the reference order.
+  unsigned ge[GE];
+  int gk0 = 0, gne = 0, gnf = 0;
   if (own) {
-    for (int k = TG.gptr[v]; k < TG.gptr[v + 1]; ++k) {
-      const unsigned e = TG.gent[k];
+    gk0 = TG.gptr[v];
+    gne = TG.gptr[v + 1] - gk0;
+#pragma unroll
+    for (int u = 0; u < GE; ++u) ge[u] = (u < gne) ? (unsigned)TG.gent[gk0 + u] : 0x4000u;
+#pragma unroll
+    for (int u = 0; u < GE; ++u)
+      if (u < gne && !(ge[u] & 0x4000u)) { gather_face(ge[u], SF, MF, acc); gnf = u + 1; }
+    for (int k = gnf; k < gne; ++k) {
+      const unsigned e = TG.gent[gk0 + k];
       if (e & 0x4000u) {
         const int b = TG.tbf[bb0 + (e & 0x3fffu)];
         const Cons Fb = bflux<FLUX, REC>(G, A.fo, gas, b, w0);
         acc[0] += Fb.m * 0.5; acc[1] += Fb.mx * 0.5; acc[2] += Fb.my * 0.5; acc[3] += Fb.e * 0.5;
       } else {
-        const int j = e & 0x3fffu;
-        const double q0 = SF[j], q1 = SF[MF + j], q2 = SF[2 * MF + j], q3 = SF[3 * MF + j];
-        if (e & 0x8000u) { acc[0] -= q0; acc[1] -= q1; acc[2] -= q2; acc[3] -= q3; }
-        else { acc[0] += q0; acc[1] += q1; acc[2] += q2; acc[3] += q3; }
+        gather_face(e, SF, MF, acc);
       }
     }
   }
@@ -542,8 +573,11 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
     reinterpret_cast<double4*>(A.W_out)[off + v] = make_double4(w1.T, w1.u, w1.v, w1.p);
     if (LAST) {  // spectral radius for the next step's dt, src/fv.cc:1511-1553
       const double cm = sound_speed(gas, w1);
-      for (int k = TG.gptr[v]; k < TG.gptr[v + 1]; ++k) {
-        const unsigned e = TG.gent[k];
+#pragma unroll
+      for (int u = 0; u < GE; ++u)
+        if (u < gnf) lam += face_lambda(ge[u], SN, MF, w1, cm);
+      for (int k = gnf; k < gne; ++k) {
+        const unsigned e = TG.gent[gk0 + k];
         if (e & 0x4000u) {
           const int bf = TG.tbf[bb0 + (e & 0x3fffu)];
           const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
@@ -551,10 +585,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
           const double vn = w1.u * nx + w1.v * ny;
           lam += 0.5 * (fabs(vn) + cm * ar);
         } else {
-          const int j = e & 0x3fffu;
-          const double nx = SN[j], ny = SN[MF + j], ar = SN[2 * MF + j];
-          const double vn = w1.u * nx + w1.v * ny;
-          lam += (fabs(vn) + cm * ar) * 1.0;
+          lam += face_lambda(e, SN, MF, w1, cm);
         }
       }
     }
@@ -748,19 +779,25 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
   double acc[4] = {0, 0, 0, 0};
   double part[4] = {0, 0, 0, 0};
   double lam = 0.0;
+  unsigned ge[GE];  // see k_fu_tile
+  int gk0 = 0, gne = 0, gnf = 0;
   if (own) {
     const Prim w0 = Prim{Q[tid], Q[MS1 + tid], Q[2 * MS1 + tid], Q[3 * MS1 + tid]};
-    for (int k = TG.gptr[v]; k < TG.gptr[v + 1]; ++k) {
-      const unsigned e = TG.gent[k];
+    gk0 = TG.gptr[v];
+    gne = TG.gptr[v + 1] - gk0;
+#pragma unroll
+    for (int u = 0; u < GE; ++u) ge[u] = (u < gne) ? (unsigned)TG.gent[gk0 + u] : 0x4000u;
+#pragma unroll
+    for (int u = 0; u < GE; ++u)
+      if (u < gne && !(ge[u] & 0x4000u)) { gather_face(ge[u], SF, MF, acc); gnf = u + 1; }
+    for (int k = gnf; k < gne; ++k) {
+      const unsigned e = TG.gent[gk0 + k];
       if (e & 0x4000u) {
         const int b = TG.tbf[bb0 + (e & 0x3fffu)];
         const Cons Fb = bflux<FLUX, REC>(G, A.fo, gas, b, w0);
         acc[0] += Fb.m * 0.5; acc[1] += Fb.mx * 0.5; acc[2] += Fb.my * 0.5; acc[3] += Fb.e * 0.5;
       } else {
-        const int j = e & 0x3fffu;
-        const double q0 = SF[j], q1 = SF[MF + j], q2 = SF[2 * MF + j], q3 = SF[3 * MF + j];
-        if (e & 0x8000u) { acc[0] -= q0; acc[1] -= q1; acc[2] -= q2; acc[3] -= q3; }
-        else { acc[0] += q0; acc[1] += q1; acc[2] += q2; acc[3] += q3; }
+        gather_face(e, SF, MF, acc);
       }
     }
     const double area = G.dcarea[v];
@@ -790,8 +827,11 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
     reinterpret_cast<double4*>(A.W_out)[off + v] = make_double4(w1.T, w1.u, w1.v, w1.p);
     if (LAST) {  // spectral radius for the next step's dt, src/fv.cc:1511-1553
       const double cm = sound_speed(gas, w1);
-      for (int k = TG.gptr[v]; k < TG.gptr[v + 1]; ++k) {
-        const unsigned e = TG.gent[k];
+#pragma unroll
+      for (int u = 0; u < GE; ++u)
+        if (u < gnf) lam += face_lambda(ge[u], SN, MF, w1, cm);
+      for (int k = gnf; k < gne; ++k) {
+        const unsigned e = TG.gent[gk0 + k];
         if (e & 0x4000u) {
           const int bf = TG.tbf[bb0 + (e & 0x3fffu)];
           const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
@@ -799,10 +839,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
           const double vn = w1.u * nx + w1.v * ny;
           lam += 0.5 * (fabs(vn) + cm * ar);
         } else {
-          const int j = e & 0x3fffu;
-          const double nx = SN[j], ny = SN[MF + j], ar = SN[2 * MF + j];
-          const double vn = w1.u * nx + w1.v * ny;
-          lam += (fabs(vn) + cm * ar) * 1.0;
+          lam += face_lambda(e, SN, MF, w1, cm);
         }
       }
     }

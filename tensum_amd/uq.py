@@ -199,9 +199,12 @@ def moments(Q):
 
 def mlmc(problem: Problem, levels, samples, seed=0):
     """Multilevel Monte Carlo: E[Q_L] = E[Q_0] + sum_l E[Q_l - Q_{l-1}] with the
-    same inputs on both meshes of a level pair; E[Q_L^2] the same way, so
-    Var = E[Q^2] - E[Q]^2.  All level pairs advance together (one context per
-    mesh and level, one stream each)."""
+    same inputs on both meshes of a level pair, and the variance telescoped
+    the same way from unbiased sample variances, Var[Q_L] = Var[Q_0] +
+    sum_l (Var[Q_l] - Var[Q_{l-1}]), each difference from the level's own
+    paired samples (raw second moments would cancel catastrophically).  All
+    level pairs advance together (one context per mesh and level, one stream
+    each)."""
     t0 = time.perf_counter()
     meshes = {n: problem.mesh(n) for n in levels}
     groups, ins = [], []
@@ -221,18 +224,19 @@ def mlmc(problem: Problem, levels, samples, seed=0):
     for li, (fine, coarse) in enumerate(groups):
         qf = qoi(problem, fine)
         qc = qoi(problem, coarse) if coarse is not None else np.zeros_like(qf)
-        y1, y2 = qf - qc, qf * qf - qc * qc
+        y1 = qf - qc
         m1, v1 = moments(y1)
-        m2, _ = moments(y2)
+        _, vf = moments(qf)
+        vc = moments(qc)[1] if coarse is not None else 0.0
         mean = mean + m1
-        sec = sec + m2
+        sec = sec + (vf - vc)
         se2 = se2 + v1 / y1.shape[0]
         detail["levels"].append({"n": levels[li], "samples": y1.shape[0],
                                  "var_y": float(np.mean(v1))})
         fine.close()
         if coarse is not None:
             coarse.close()
-    return Estimate(mean, sec - mean * mean, np.sqrt(se2), time.perf_counter() - t0, detail)
+    return Estimate(mean, sec, np.sqrt(se2), time.perf_counter() - t0, detail)
 
 
 # ---------------------------------------------------------------------------
@@ -343,9 +347,13 @@ def mlmc_experiment(problem: Problem, levels, samples, n_ref, seed=0, log=None):
                 float(np.sqrt(np.mean(((v[keep] - v_ref[keep]) / v_ref[keep]) ** 2))))
 
     e_ml, e_pl = err(est.mean, est.var), err(m_pl, v_pl)
-    return {"reference": {"samples": n_ref, "mesh": levels[-1], "seconds": t_ref},
+    sd = np.sqrt(v_ref[keep])
+    return {"reference": {"samples": n_ref, "mesh": levels[-1], "seconds": t_ref,
+                          "mean_se": float(np.sqrt(1.0 / n_ref))},
             "mlmc": {"levels": list(levels), "samples": list(samples), "seconds": est.seconds,
-                     "mean_error": e_ml[0], "var_error": e_ml[1], "detail": est.detail},
+                     "mean_error": e_ml[0], "var_error": e_ml[1],
+                     "mean_se": float(np.sqrt(np.mean((est.mean_se[keep] / sd) ** 2))),
+                     "detail": est.detail},
             "plain_mc": {"samples": n_plain, "seconds": t_pl, "mean_error": e_pl[0],
-                         "var_error": e_pl[1]},
+                         "var_error": e_pl[1], "mean_se": float(np.sqrt(1.0 / n_plain))},
             "qoi_kept": int(np.sum(keep))}
