@@ -78,6 +78,10 @@ def main():
     ap.add_argument("--steps", type=int, default=334)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1025)
+    ap.add_argument("--workload", default="riemann", choices=["riemann", "vortex", "tgv"],
+                    help="riemann: the headline 1025^2 case; vortex: periodic isentropic vortex "
+                         "(KEPES-TeCNO Roe, CFL 0.8); tgv: Navier-Stokes Taylor-Green vortex "
+                         "(KEPEC + viscous, Re 1600); --n sets the mesh")
     ap.add_argument("--tile", type=int, default=0, help="owned vertices per tile (0: 128)")
     ap.add_argument("--path", default="auto", choices=["auto", "csr"])
     ap.add_argument("--fused-gradient", action="store_true",
@@ -131,13 +135,33 @@ def main():
     from cases import riemann, riemann_mesh
 
     n = args.n
-    mesh = riemann_mesh(n, seed=1)
-    prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8,
-                   time_scheme="ssprk3", kernel_path=args.path, tile_vertices=args.tile,
-                   fused_gradient=args.fused_gradient, graph_steps=args.graph,
-                   block_fu=args.block)
-    w0 = riemann(mesh.x, mesh.y)
-    print(f"bench: mesh {n}x{n} ({mesh.n_vertex} vertices), building context", file=sys.stderr,
+    common = dict(time_scheme="ssprk3", kernel_path=args.path, tile_vertices=args.tile,
+                  fused_gradient=args.fused_gradient, graph_steps=args.graph, block_fu=args.block)
+    if args.workload == "riemann":
+        mesh = riemann_mesh(n, seed=1)
+        prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8,
+                       **common)
+        w0 = riemann(mesh.x, mesh.y)
+        workload = "2D Riemann, KEPES-TeCNO Rusanov, minmod, SSP-RK3, CFL 0.5"
+        mesh_name = f"{n}x{n} jittered triangles, seed 1"
+    elif args.workload == "vortex":
+        from cases import vortex, vortex_mesh
+        mesh = vortex_mesh(n, seed=0)
+        prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=1e9,
+                       **common)
+        w0 = vortex(mesh.x, mesh.y)
+        workload = "2D isentropic vortex, KEPES-TeCNO Roe, minmod, SSP-RK3, CFL 0.8"
+        mesh_name = f"{n}x{n} jittered triangles on periodic [-5,5]^2, seed 0"
+    else:
+        from cases import taylor_green, tgv_mesh
+        mesh = tgv_mesh(n, seed=2)
+        gam, mach = 1.4, 0.1
+        prm = T.Params(flux="kepec", reconstruct="first", model="ns", cfl=0.5, final_time=1e9,
+                       mu_ref=1.0 / 1600.0, prandtl=0.72, **common)
+        w0 = taylor_green(mesh.x, mesh.y, mach=mach, gam=gam)
+        workload = "2D Taylor-Green vortex, Navier-Stokes, KEPEC + viscous, Re 1600, SSP-RK3"
+        mesh_name = f"{n}x{n} jittered triangles on periodic [0,2pi]^2, seed 2"
+    print(f"bench: {args.workload} mesh {n}x{n} ({mesh.n_vertex} vertices), building context", file=sys.stderr,
           flush=True)
     s = T.Solver(mesh, prm)
     s.set_state(w0)
@@ -174,7 +198,8 @@ def main():
     achieved = kb[dom] / (dur_us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(dom),
+                "traffic": (measured_traffic(dom) if args.workload == "riemann" and n == 1025
+                            else None),
                 "algorithmic_bytes_per_launch": kb[dom],
                 "kernel_us": {k: round(v, 2) for k, v in ku.items()},
                 "step_frac_at_389B_per_vertex_stage":
@@ -206,9 +231,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded jittered-triangle mesh, four-quadrant Riemann initial state)",
-        "config": {"workload": "2D Riemann, KEPES-TeCNO Rusanov, minmod, SSP-RK3, CFL 0.5",
-                   "mesh": f"{n}x{n} jittered triangles, seed 1", "n_vertex": nv, "n_face": nf,
+        "data": f"synthetic (seeded jittered-triangle mesh, {args.workload} initial state)",
+        "config": {"workload": workload, "mesh": mesh_name, "n_vertex": nv, "n_face": nf,
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
