@@ -124,6 +124,13 @@ int tsm_compute_dt(tsm_ctx* ctx, int sample, double* dt_global);
    (optional, n_samples*nsteps*4, sample-major) receives per step
    (iter, dt_global, t, res_total), zero rows after a sample stopped. */
 int tsm_run(tsm_ctx* ctx, long nsteps, long* done, double* log);
+/* One pass of the loop body of FiniteVolume::solve (src/fv.cc:2277-2316) for
+   every running sample; t_target > 0 additionally clamps the step so that a
+   sample's time lands on t_target exactly instead of passing it (as the
+   reference does at an output time stamp).  dt_out[n_samples] and
+   res_norm_out[n_samples] (optional) receive each sample's last dt and
+   res_total (src/fv.cc:1862-1876). */
+int tsm_step(tsm_ctx* ctx, double t_target, double* dt_out, double* res_norm_out);
 /* tsm_run over several contexts at once (meshes of different resolution, e.g.
    the levels of a multilevel Monte Carlo run): each context's steps go to its
    own stream, interleaved, so the levels share the GPU.  done[i] receives the

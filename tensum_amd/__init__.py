@@ -188,6 +188,7 @@ def lib():
         L.tsm_geometry.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         L.tsm_bench.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_sobol.argtypes = [C.c_long, C.c_int, C.c_long, C.c_ulonglong, C.c_int, C.c_void_p]
+        L.tsm_step.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
         L.tsm_run_group.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_void_p]
         L.tsm_pert.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int,
                                C.c_void_p]
@@ -342,6 +343,13 @@ class Solver:
         log = np.zeros((self.n_samples, nsteps, 4)) if want_log else None
         _check(lib().tsm_run(self._h, nsteps, _dptr(done), _dptr(log) if want_log else None))
         return (done, log[0] if self.n_samples == 1 else log) if want_log else done
+
+    def step(self, t_target=0.0):
+        """One time step of every running sample (optionally landing on t_target);
+        returns each sample's dt and residual norm."""
+        dt, res = np.zeros(self.n_samples), np.zeros(self.n_samples)
+        _check(lib().tsm_step(self._h, t_target, _dptr(dt), _dptr(res)))
+        return dt, res
 
     def status(self, sample=0):
         v = np.zeros(3)

@@ -153,8 +153,9 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     return fail(TSM_E_MESH, "mesh too large for 30-bit face ids");
   }
   // ---- path and vertex numbering -----------------------------------------
-  c->tiled = params->kernel_path == 0 && params->model == 0 && params->time_scheme == 0 &&
-             !params->liou_fix;
+  // tiled stage kernels: SSP-RK3 without the Liou fix, Euler or Navier-Stokes
+  // (NS meshes have no non-periodic boundary cells, checked above)
+  c->tiled = params->kernel_path == 0 && params->time_scheme == 0 && !params->liou_fix;
   std::vector<int> perm(nv), inv(nv);
   if (c->tiled) {
     int tb = params->tile_vertices > 0 ? params->tile_vertices : 128;
@@ -164,7 +165,7 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
       return fail(TSM_E_ARG, "graph_steps must be in [0, 4096]");
     }
     const bool grad = (params->flux == 0 || params->flux == 1) && params->reconstruct != 0;
-    const bool ring2 = grad && params->tile_fused_gradient;
+    const bool ring2 = grad && params->tile_fused_gradient && params->model == 0;
     if (!tsm::build_tiles(g, tb, c->tiling, msg, ring2) || !tsm::validate_tiles(g, c->tiling, msg)) {
       delete c;
       return fail(TSM_E_MESH, "tiling: " + msg);
@@ -273,7 +274,8 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     TG.fu_block = std::max(0, params->block_fu);
     const int bf = tsm::tile_fu_block(TG), bg = tsm::tile_grad_block(TG);
     if (bf > tsm::tile_fu_block_max() || bf % 64 || bg > 512 || bg % 64 ||
-        tsm::tile_fu_lds(TG, params->flux, params->reconstruct) > 160 * 1024 ||
+        (params->model == 1 ? tsm::tile_fu_lds_ns(TG, params->flux, params->reconstruct)
+                            : tsm::tile_fu_lds(TG, params->flux, params->reconstruct)) > 160 * 1024 ||
         tsm::tile_grad_lds(TG) > 160 * 1024) {
       delete c;
       return fail(TSM_E_ARG, "tiled kernels: block size or LDS out of range (blocks " +
@@ -432,11 +434,13 @@ extern "C" int tsm_compute_dt(tsm_ctx* c, int s, double* dt) {
 namespace {
 // One full time step for every live sample with the CSR-gather kernels
 // (src/fv.cc:2277-2316).
-void enqueue_step(tsm_ctx* c, int log_slot) {
+// dt_ready: the caller already ran the time-step pass (which also stores
+// U_old) and may have adjusted dt.
+void enqueue_step(tsm_ctx* c, int log_slot, bool dt_ready = false) {
   const int nrk = c->P.time_scheme == 0 ? 3 : 4;
   const int mode = c->P.time_scheme == 0 ? 1 : 2;
   c->bind_state();
-  tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
+  if (!dt_ready) tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
   if (c->P.liou_fix) tsm::launch_ssw(c->G, c->A, c->S, c->st);
   for (int r = 0; r < nrk; ++r) {
     c->bind_state();
@@ -551,6 +555,45 @@ extern "C" int tsm_run(tsm_ctx* c, long nsteps, long* done, double* log) {
     }
   }
   c->A.log = (log ? c->A.log : c->A.log);
+  return TSM_OK;
+}
+
+extern "C" int tsm_step(tsm_ctx* c, double t_target, double* dt_out, double* res_out) {
+  if (!c) return fail(TSM_E_ARG, "bad argument");
+  const int S = c->S;
+  std::vector<tsm::Ctl> before(S), after(S);
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(before.data(), c->A.ctl, S * sizeof(tsm::Ctl), hipMemcpyDeviceToHost));
+  if (c->A.log_cap < 1) {  // the step's (iter, dt, t, res) rows
+    double* p = nullptr;
+    int rc = c->alloc(&p, (size_t)S * 4);
+    if (rc) return rc;
+    c->A.log = p;
+    c->A.log_cap = 1;
+  }
+  if (c->tiled) {
+    c->bind_state();
+    tsm::launch_dt(c->G, c->A, c->SP, S, c->st);
+    if (t_target > 0.0) tsm::launch_clamp_dt(c->A, S, t_target, c->st);
+    enqueue_tiled_step(c, 0);
+  } else {
+    c->csr_ready();
+    c->bind_state();
+    tsm::launch_dt(c->G, c->A, c->SP, S, c->st);
+    if (t_target > 0.0) tsm::launch_clamp_dt(c->A, S, t_target, c->st);
+    enqueue_step(c, 0, /*dt_ready=*/true);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(after.data(), c->A.ctl, S * sizeof(tsm::Ctl), hipMemcpyDeviceToHost));
+  std::vector<double> rows((size_t)S * c->A.log_cap * 4);
+  HIP_TRY(hipMemcpy(rows.data(), c->A.log, rows.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int s = 0; s < S; ++s) {
+    const bool ran = after[s].iter > before[s].iter;
+    const double* r = &rows[(size_t)s * c->A.log_cap * 4];
+    if (dt_out) dt_out[s] = ran ? r[1] : 0.0;
+    if (res_out) res_out[s] = after[s].res_total;
+  }
   return TSM_OK;
 }
 

@@ -61,6 +61,48 @@ struct ViscParams {
   int mu_model, visc_active, heat_active;
 };
 
+
+// Viscous flux of a cell delivered to one of its vertices (src/material.cc:
+// 158-230): s = the mean of the cell's three entropy-variable states, dE =
+// the cell gradient of the entropy variables (x, y per component), (nx, ny) =
+// the vertex's cell normal.  mu(T) with T = -1/(R s_l), kappa = mu Cp / Pr.
+__device__ __forceinline__ Cons viscous_cell_flux(const Gas& gas, const ViscParams& vp,
+                                                  const Ent& s, const double* dE, double nx,
+                                                  double ny) {
+  const double R = gas.R;
+  double T = -1 / (R * s.l);
+  double mu;
+  if (vp.mu_model == 0) mu = vp.mu_ref;
+  else if (vp.mu_model == 1) mu = vp.mu_ref * pow(T, 1.5) / (T + vp.T_0);
+  else mu = vp.mu_ref * pow(T / vp.T_ref, vp.omega);
+  double Cp = gas.gamma * gas.R / (gas.gamma - 1.0);
+  double k = mu * Cp / vp.prandtl;
+  if (!vp.visc_active) mu = 0.0;
+  if (!vp.heat_active) k = 0.0;
+  double e1 = s.vx / s.l, e2 = s.vy / s.l, e4 = 1.0 / s.l;
+  double c1 = 4.0 * mu / 3.0, c2 = k / R, c3 = 2 * mu / 3.0;
+  // dE components: [0,1] f, [2,3] vx, [4,5] vy, [6,7] l  (x, y)
+  double d1x = dE[2], d1y = dE[3], d2x = dE[4], d2y = dE[5], d4x = dE[6], d4y = dE[7];
+  double K11_1 = c1 * (-d1x + e1 * d4x);
+  double K11_2 = mu * (-d2x + e2 * d4x);
+  double K11_4 = c1 * e1 * d1x + mu * (e2 * d2x) + (-c1 * e1 * e1 - mu * (e2 * e2) + c2 * e4) * d4x;
+  double K12_1 = c3 * (d2y - e2 * d4y);
+  double K12_2 = mu * (-d1y + e1 * d4y);
+  double K12_4 = mu * e2 * d1y - c3 * e1 * d2y - (mu - c3) * e1 * e2 * d4y;
+  double K21_1 = mu * (-d2x + e2 * d4x);
+  double K21_2 = c3 * (d1x - e1 * d4x);
+  double K21_4 = -c3 * e2 * d1x + mu * e1 * d2x - (mu - c3) * e1 * e2 * d4x;
+  double K22_1 = mu * (-d1y + e1 * d4y);
+  double K22_2 = c1 * (-d2y + e2 * d4y);
+  double K22_4 = mu * (e1 * d1y) + c1 * e2 * d2y + (-mu * (e1 * e1) - c1 * e2 * e2 + c2 * e4) * d4y;
+  Cons F;
+  F.m = 0.0;
+  F.mx = -((K11_1 + K12_1) * nx + (K21_1 + K22_1) * ny) * e4;
+  F.my = -((K11_2 + K12_2) * nx + (K21_2 + K22_2) * ny) * e4;
+  F.e = -((K11_4 + K12_4) * nx + (K21_4 + K22_4) * ny) * e4;
+  return F;
+}
+
 struct StepParams {
   double min_residue, final_time, time_step, cfl;
   long long max_iter;
@@ -102,6 +144,7 @@ bool tile_needs_gradient_pass(const TileGeo& T, int flux, int rec);
 void launch_tile_gradient_bnd(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
                               hipStream_t st);
 size_t tile_fu_lds(const TileGeo& T, int flux, int rec);
+size_t tile_fu_lds_ns(const TileGeo& T, int flux, int rec);
 void launch_tile_gradient(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
                           hipStream_t st);
 void launch_tile_stage(int flux, int rec, const DevGeo& G, const TileGeo& T, const StageArgs& A,
@@ -122,6 +165,7 @@ void launch_global(const DevGeo& G, const StageArgs& A, int s, double* out, hipS
 void launch_errnorm(const DevGeo& G, const StageArgs& A, int s, const double* ex, double* out,
                     hipStream_t st);
 
+void launch_clamp_dt(const StageArgs& A, int S, double t_target, hipStream_t st);
 void launch_moments(const DevGeo& G, const StageArgs& A, int S, double* mean, double* m2,
                     double* s2, hipStream_t st);
 void launch_probes(const StageArgs& A, int nv, int np, const int* pc, const double* pw, int S,

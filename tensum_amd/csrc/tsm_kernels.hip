@@ -242,7 +242,12 @@ __global__ __launch_bounds__(256) void k_flux(DevGeo G, StageArgs A) {
     double ssw = 0.0;
     if (FLUX == F_ROE && A.ssw) ssw = A.ssw[off + l] + A.ssw[off + r];
     Side Ls = make_side<true, FLUX == F_ROE>(gas, Wl), Rs = make_side<true, FLUX == F_ROE>(gas, Wr);
-    F = tecno_flux<FLUX == F_ROE, REC>(gas, A.fo, rx, ry, Ls, Rs, gl, gr, unx, uny, ssw);
+    double gdl[4] = {0, 0, 0, 0}, gdr[4] = {0, 0, 0, 0};
+    if (REC != R_FIRST) {
+      project_grad(gl, rx, ry, gdl);
+      project_grad(gr, rx, ry, gdr);
+    }
+    F = tecno_flux<FLUX == F_ROE, REC>(gas, A.fo, rx, ry, Ls, Rs, gdl, gdr, unx, uny, ssw);
   } else if (FLUX == F_KEPEC) {
     F = kepec_flux(gas, Wl, Wr, unx, uny);
   } else if (FLUX == F_KEP) {
@@ -315,38 +320,7 @@ __device__ __forceinline__ Cons viscous_part(const DevGeo& G, const StageArgs& A
   if (j == 0) { nx = (yb - yd); ny = -(xb - xd); }
   else if (j == 1) { nx = (yd - ya); ny = -(xd - xa); }
   else { nx = (ya - yb); ny = -(xa - xb); }
-  const double R = gas.R;
-  double T = -1 / (R * s.l);
-  double mu;
-  if (A.vp.mu_model == 0) mu = A.vp.mu_ref;
-  else if (A.vp.mu_model == 1) mu = A.vp.mu_ref * pow(T, 1.5) / (T + A.vp.T_0);
-  else mu = A.vp.mu_ref * pow(T / A.vp.T_ref, A.vp.omega);
-  double Cp = gas.gamma * gas.R / (gas.gamma - 1.0);
-  double k = mu * Cp / A.vp.prandtl;
-  if (!A.vp.visc_active) mu = 0.0;
-  if (!A.vp.heat_active) k = 0.0;
-  double e1 = s.vx / s.l, e2 = s.vy / s.l, e4 = 1.0 / s.l;
-  double c1 = 4.0 * mu / 3.0, c2 = k / R, c3 = 2 * mu / 3.0;
-  // dE components: [0,1] f, [2,3] vx, [4,5] vy, [6,7] l  (x, y)
-  double d1x = dE[2], d1y = dE[3], d2x = dE[4], d2y = dE[5], d4x = dE[6], d4y = dE[7];
-  double K11_1 = c1 * (-d1x + e1 * d4x);
-  double K11_2 = mu * (-d2x + e2 * d4x);
-  double K11_4 = c1 * e1 * d1x + mu * (e2 * d2x) + (-c1 * e1 * e1 - mu * (e2 * e2) + c2 * e4) * d4x;
-  double K12_1 = c3 * (d2y - e2 * d4y);
-  double K12_2 = mu * (-d1y + e1 * d4y);
-  double K12_4 = mu * e2 * d1y - c3 * e1 * d2y - (mu - c3) * e1 * e2 * d4y;
-  double K21_1 = mu * (-d2x + e2 * d4x);
-  double K21_2 = c3 * (d1x - e1 * d4x);
-  double K21_4 = -c3 * e2 * d1x + mu * e1 * d2x - (mu - c3) * e1 * e2 * d4x;
-  double K22_1 = mu * (-d1y + e1 * d4y);
-  double K22_2 = c1 * (-d2y + e2 * d4y);
-  double K22_4 = mu * (e1 * d1y) + c1 * e2 * d2y + (-mu * (e1 * e1) - c1 * e2 * e2 + c2 * e4) * d4y;
-  Cons F;
-  F.m = 0.0;
-  F.mx = -((K11_1 + K12_1) * nx + (K21_1 + K22_1) * ny) * e4;
-  F.my = -((K11_2 + K12_2) * nx + (K21_2 + K22_2) * ny) * e4;
-  F.e = -((K11_4 + K12_4) * nx + (K21_4 + K22_4) * ny) * e4;
-  return F;
+  return viscous_cell_flux(gas, A.vp, s, dE, nx, ny);
 }
 
 // ---------------------------------------------------------------------------
@@ -769,6 +743,19 @@ void launch_dt(const DevGeo& G, const StageArgs& A, const StepParams& P, int S, 
   hipLaunchKernelGGL(k_dt, dim3(nb, S), dim3(256), 0, st, G, A, P);
   hipLaunchKernelGGL(k_dt_final, dim3(S), dim3(256), 0, st, A, P, nb);
 }
+// Caller's stop time for the next step: a running sample whose t + dt would
+// pass t_target steps to t_target exactly (the stamp flag stays as it is).
+__global__ void k_clamp_dt(StageArgs A, int S, double t_target) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  Ctl& c = A.ctl[s];
+  if (c.done || !(c.t < t_target)) return;
+  if (c.t + c.dt > t_target) c.dt = t_target - c.t;
+}
+void launch_clamp_dt(const StageArgs& A, int S, double t_target, hipStream_t st) {
+  hipLaunchKernelGGL(k_clamp_dt, dim3((S + 63) / 64), dim3(64), 0, st, A, S, t_target);
+}
+
 void launch_step_final(const DevGeo& G, const StageArgs& A, const StepParams& P, int S,
                        int log_slot, hipStream_t st) {
   int nb = (G.nv + 255) / 256;

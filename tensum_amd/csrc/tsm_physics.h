@@ -276,12 +276,20 @@ __device__ __forceinline__ Cons avg_flux(const Gas& g, const Prim& l, const Prim
   return F;
 }
 
+// Gradients of the entropy variables (f, vx, vy, l; x then y per component)
+// projected on the edge vector r = x_r - x_l: the only form in which the
+// reconstruction uses them (src/reconstruct.cc:13-96).
+__device__ __forceinline__ void project_grad(const double* g, double rx, double ry, double* gd) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) gd[j] = g[2 * j] * rx + g[2 * j + 1] * ry;
+}
+
 // src/kepes_tecno_roe.cc:11-173, src/kepes_tecno_rusanov.cc:11-164.
-// gl/gr: left/right vertex gradients of (f, vx, vy, l), x then y.
+// gdl/gdr: left/right vertex gradients projected on the edge (project_grad).
 template <bool ROE, int REC>
 __device__ __forceinline__ Cons tecno_flux(const Gas& g, const FluxOpts& o, double rx, double ry,
-                                           const Side& L, const Side& Rr, const double* gl,
-                                           const double* gr, double unx, double uny,
+                                           const Side& L, const Side& Rr, const double* gdl,
+                                           const double* gdr, double unx, double uny,
                                            double ssw) {
   // Multiply-adds fused here: the flux is bounded by FP64 issue, and fusing
   // changes each product-sum by at most an ulp.
@@ -343,14 +351,6 @@ __device__ __forceinline__ Cons tecno_flux(const Gas& g, const FluxOpts& o, doub
   const double* Vr = Rr.V;
   double h = 0.0;
   if (REC == R_TVB) h = sqrt(rx * rx + ry * ry);
-  double gdl[4], gdr[4];
-  if (REC != R_FIRST) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      gdl[j] = gl[2 * j] * rx + gl[2 * j + 1] * ry;
-      gdr[j] = gr[2 * j] * rx + gr[2 * j + 1] * ry;
-    }
-  }
   double dZ[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {

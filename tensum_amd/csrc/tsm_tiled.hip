@@ -169,10 +169,10 @@ __device__ __forceinline__ void bc2(int type, const double* bs, double nx, doubl
 
 template <int FLUX, int REC>
 __device__ __forceinline__ Cons pair_flux(const Gas& gas, const FluxOpts& fo, double rx, double ry,
-                                          const Side& L, const Side& R, const double* gl,
-                                          const double* gr, double unx, double uny) {
+                                          const Side& L, const Side& R, const double* gdl,
+                                          const double* gdr, double unx, double uny) {
   if (FLUX == F_ROE || FLUX == F_RUS)
-    return tecno_flux<FLUX == F_ROE, REC>(gas, fo, rx, ry, L, R, gl, gr, unx, uny, 0.0);
+    return tecno_flux<FLUX == F_ROE, REC>(gas, fo, rx, ry, L, R, gdl, gdr, unx, uny, 0.0);
   if (FLUX == F_KEPEC) return kepec_flux_s(gas, L, R, unx, uny);
   if (FLUX == F_KEP) return kep_flux(gas, L.w, R.w, unx, uny);
   if (FLUX == F_ROEEC) return roe_ec_flux(gas, L.w, R.w, unx, uny);
@@ -379,16 +379,21 @@ __global__ __launch_bounds__(256) void k_grad_bnd(DevGeo G, TileGeo TG, StageArg
 // ---------------------------------------------------------------------------
 // flux + gather + update kernel
 // ---------------------------------------------------------------------------
-template <int FLUX, int REC>
+template <int FLUX, int REC, bool NS = false>
 struct SlotLayout {
   static constexpr bool TEC = FLUX == F_ROE || FLUX == F_RUS;
   static constexpr bool GRAD = TEC && REC != R_FIRST;
-  // rows: T u v p | rho beta | V0..V3 (TeCNO) | c (Roe) | dE x8, x, y (gradients)
-  static constexpr int QS = 6, QC = 6 + (TEC ? 4 : 0), QD = QC + (FLUX == F_ROE ? 1 : 0);
-  static constexpr int QX = QD + 8;
-  static constexpr int NQ = QD + (GRAD ? 10 : 0);
+  // rows: T u v p | (viscous: x y E0..E3) | rho beta | V0..V3 (TeCNO) | c (Roe) |
+  //       dE x8 (gradients) | x y (gradients without viscous terms).
+  // The first KEEP rows stay valid for the gather phase; the rows after them
+  // take the face fluxes once the flux phase is done.
+  static constexpr int KEEP = 4 + (NS ? 6 : 0);
+  static constexpr int QE = 6;  // viscous: entropy variables
+  static constexpr int QR = KEEP, QB = KEEP + 1;
+  static constexpr int QS = KEEP + 2, QC = QS + (TEC ? 4 : 0), QD = QC + (FLUX == F_ROE ? 1 : 0);
+  static constexpr int QX = NS ? 4 : QD + 8;
+  static constexpr int NQ = QD + (GRAD ? 8 : 0) + ((GRAD && !NS) ? 2 : 0);
 };
-
 // Faces per thread of the flux phase: the host launches at least
 // ceil(max_faces / FPT) threads per tile.
 constexpr int FPT = 2;
@@ -399,10 +404,10 @@ constexpr int FPT = 2;
 #define TSM_FU_WAVES 3
 #endif
 
-template <int FLUX, int REC, int LAST>
+template <int FLUX, int REC, int LAST, bool NS>
 __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs A, StepParams P,
                                                  const double* Wn, double* dtpart, int r) {
-  using L = SlotLayout<FLUX, REC>;
+  using L = SlotLayout<FLUX, REC, NS>;
   extern __shared__ double lds[];
   __shared__ double wred[8 * 5];
   __shared__ int arrive;
@@ -415,7 +420,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
   // periodic exchange SR (4 x TB) reuses the primitive rows once every owned
   // state has been read.
   double* Q = lds;
-  double* SF = lds + 4 * MS;
+  double* SF = lds + L::KEEP * MS;
   double* SN = SF + 4 * MF;
   double* SR = lds;
   const int b0 = TG.vbeg[t], nown = TG.vbeg[t + 1] - b0;
@@ -431,16 +436,22 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
     const Prim w = ld_prim(A.W_in, off + gid);
     const Side q = make_side<L::TEC, FLUX == F_ROE>(gas, w);
     Q[i] = w.T; Q[MS + i] = w.u; Q[2 * MS + i] = w.v; Q[3 * MS + i] = w.p;
-    Q[4 * MS + i] = q.rho; Q[5 * MS + i] = q.beta;
+    Q[L::QR * MS + i] = q.rho; Q[L::QB * MS + i] = q.beta;
     if (L::TEC)
 #pragma unroll
       for (int k = 0; k < 4; ++k) Q[(L::QS + k) * MS + i] = q.V[k];
     if (FLUX == F_ROE) Q[L::QC * MS + i] = q.c;
-    if (L::GRAD) {
+    if (L::GRAD)
 #pragma unroll
       for (int k = 0; k < 8; ++k) Q[(L::QD + k) * MS + i] = A.dE[8 * (off + gid) + k];
+    if (L::GRAD || NS) {
       Q[L::QX * MS + i] = G.x[gid];
       Q[(L::QX + 1) * MS + i] = G.y[gid];
+    }
+    if (NS) {  // entropy variables of the viscous cell fluxes (prim2ent)
+      const Ent e = prim2ent(gas, w);
+      Q[L::QE * MS + i] = e.f; Q[(L::QE + 1) * MS + i] = e.vx;
+      Q[(L::QE + 2) * MS + i] = e.vy; Q[(L::QE + 3) * MS + i] = e.l;
     }
   }
   __syncthreads();
@@ -459,8 +470,8 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
     Side Ls, Rs;
     Ls.w = Prim{Q[l], Q[MS + l], Q[2 * MS + l], Q[3 * MS + l]};
     Rs.w = Prim{Q[rr], Q[MS + rr], Q[2 * MS + rr], Q[3 * MS + rr]};
-    Ls.rho = Q[4 * MS + l]; Rs.rho = Q[4 * MS + rr];
-    Ls.beta = Q[5 * MS + l]; Rs.beta = Q[5 * MS + rr];
+    Ls.rho = Q[L::QR * MS + l]; Rs.rho = Q[L::QR * MS + rr];
+    Ls.beta = Q[L::QB * MS + l]; Rs.beta = Q[L::QB * MS + rr];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       Ls.V[k] = L::TEC ? Q[(L::QS + k) * MS + l] : 0.0;
@@ -468,14 +479,17 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
     }
     Ls.c = (FLUX == F_ROE) ? Q[L::QC * MS + l] : 0.0;
     Rs.c = (FLUX == F_ROE) ? Q[L::QC * MS + rr] : 0.0;
-    double gl[8], gr[8], rx = 0.0, ry = 0.0;
+    double gdl[4] = {0, 0, 0, 0}, gdr[4] = {0, 0, 0, 0}, rx = 0.0, ry = 0.0;
     if (L::GRAD) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) { gl[k] = Q[(L::QD + k) * MS + l]; gr[k] = Q[(L::QD + k) * MS + rr]; }
       rx = Q[L::QX * MS + rr] - Q[L::QX * MS + l];
       ry = Q[(L::QX + 1) * MS + rr] - Q[(L::QX + 1) * MS + l];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        gdl[k] = Q[(L::QD + 2 * k) * MS + l] * rx + Q[(L::QD + 2 * k + 1) * MS + l] * ry;
+        gdr[k] = Q[(L::QD + 2 * k) * MS + rr] * rx + Q[(L::QD + 2 * k + 1) * MS + rr] * ry;
+      }
     }
-    const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gl, gr, nx * ia, ny * ia);
+    const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gdl, gdr, nx * ia, ny * ia);
     fl[q][0] = F.m * area; fl[q][1] = F.mx * area; fl[q][2] = F.my * area; fl[q][3] = F.e * area;
     fn[q][0] = nx; fn[q][1] = ny; fn[q][2] = area;
   }
@@ -521,6 +535,40 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
         acc[0] += Fb.m * 0.5; acc[1] += Fb.mx * 0.5; acc[2] += Fb.my * 0.5; acc[3] += Fb.e * 0.5;
       } else {
         gather_face(e, SF, MF, acc);
+      }
+    }
+    if (NS) {  // viscous cell fluxes after the faces, in cell order (src/fv.cc:1197-1232)
+      const double fact = 1.0 / 3.0;
+      for (int k = TG.cptr[v]; k < TG.cptr[v + 1]; ++k) {
+        const unsigned e = TG.cent[k];
+        const int sa = e & 1023, sb = (e >> 10) & 1023, sd = (e >> 20) & 1023;
+        const int j = (sa == tid) ? 0 : ((sb == tid) ? 1 : 2);
+        double ea[4], eb[4], ed[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          ea[q] = Q[(L::QE + q) * MS + sa]; eb[q] = Q[(L::QE + q) * MS + sb];
+          ed[q] = Q[(L::QE + q) * MS + sd];
+        }
+        Ent sm;
+        sm.f = ea[0] * fact + eb[0] * fact + ed[0] * fact;
+        sm.vx = ea[1] * fact + eb[1] * fact + ed[1] * fact;
+        sm.vy = ea[2] * fact + eb[2] * fact + ed[2] * fact;
+        sm.l = ea[3] * fact + eb[3] * fact + ed[3] * fact;
+        const double xa = Q[L::QX * MS + sa], ya = Q[(L::QX + 1) * MS + sa];
+        const double xb = Q[L::QX * MS + sb], yb = Q[(L::QX + 1) * MS + sb];
+        const double xd = Q[L::QX * MS + sd], yd = Q[(L::QX + 1) * MS + sd];
+        double dE[8];
+        cell_grad(xa, ya, xb, yb, xd, yd, ea, eb, ed, dE);
+        const double cf = 1.0 / (2.0 * G.carea[TG.cgid[k]]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) dE[q] *= cf;
+        double nx, ny;
+        if (j == 0) { nx = (yb - yd); ny = -(xb - xd); }
+        else if (j == 1) { nx = (yd - ya); ny = -(xd - xa); }
+        else { nx = (ya - yb); ny = -(xa - xb); }
+        const Cons Fv = viscous_cell_flux(gas, A.vp, sm, dE, nx, ny);
+        const double cm = -(0.5 * 1.0);
+        acc[0] += Fv.m * cm; acc[1] += Fv.mx * cm; acc[2] += Fv.my * cm; acc[3] += Fv.e * cm;
       }
     }
   }
@@ -586,6 +634,25 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
           lam += 0.5 * (fabs(vn) + cm * ar);
         } else {
           lam += face_lambda(e, SN, MF, w1, cm);
+        }
+      }
+      if (NS) {  // viscous share, after every inviscid term (src/fv.cc:1555-1580)
+        double mu;
+        const double T = w1.T;
+        if (A.vp.mu_model == 0) mu = A.vp.mu_ref;
+        else if (A.vp.mu_model == 1) mu = A.vp.mu_ref * pow(T, 1.5) / (T + A.vp.T_0);
+        else mu = A.vp.mu_ref * pow(T / A.vp.T_ref, A.vp.omega);
+        const double rho = density(gas, w1);
+        for (int k = 0; k < gne; ++k) {
+          const unsigned e = TG.gent[gk0 + k];
+          if (e & 0x4000u) continue;
+          const int j = e & 0x3fffu;
+          const unsigned lr = TG.f_lr[f0 + j];
+          const int sl = lr & 0xffff, sr = lr >> 16;
+          const double dx = Q[L::QX * MS + sr] * 1.0 + Q[L::QX * MS + sl] * -1.0;
+          const double dy = Q[(L::QX + 1) * MS + sr] * 1.0 + Q[(L::QX + 1) * MS + sl] * -1.0;
+          const double dl = sqrt(dx * dx + dy * dy);
+          lam += (mu * SN[2 * MF + j] / dl / rho) * 1.0;
         }
       }
     }
@@ -753,11 +820,14 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
     for (int k = 0; k < 4; ++k) { Ls.V[k] = QV[k * MS1 + l]; Rs.V[k] = QV[k * MS1 + rr]; }
     Ls.c = FL::ROE ? QC[l] : 0.0;
     Rs.c = FL::ROE ? QC[rr] : 0.0;
-    double gl[8], gr[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) { gl[k] = QD[k * MS1 + l]; gr[k] = QD[k * MS1 + rr]; }
     const double rx = SX[rr] - SX[l], ry = SX[MS2 + rr] - SX[MS2 + l];
-    const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gl, gr, nx * ia, ny * ia);
+    double gdl[4], gdr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      gdl[k] = QD[2 * k * MS1 + l] * rx + QD[(2 * k + 1) * MS1 + l] * ry;
+      gdr[k] = QD[2 * k * MS1 + rr] * rx + QD[(2 * k + 1) * MS1 + rr] * ry;
+    }
+    const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gdl, gdr, nx * ia, ny * ia);
     fl[q][0] = F.m * area; fl[q][1] = F.mx * area; fl[q][2] = F.my * area; fl[q][3] = F.e * area;
     fn[q][0] = nx; fn[q][1] = ny; fn[q][2] = area;
   }
@@ -928,11 +998,11 @@ size_t tile_grad_lds(const TileGeo& T) {
 }
 
 namespace {
-template <int FLUX, int REC>
+template <int FLUX, int REC, bool NS = false>
 size_t fu_lds(const TileGeo& T, bool) {
+  using L = SlotLayout<FLUX, REC, NS>;
   const size_t MS = T.max_slots, MF = T.max_faces;
-  return sizeof(double) *
-         (4 * MS + std::max((size_t)(SlotLayout<FLUX, REC>::NQ - 4) * MS, 7 * MF) + 16);
+  return sizeof(double) * (L::KEEP * MS + std::max((size_t)(L::NQ - L::KEEP) * MS, 7 * MF) + 16);
 }
 template <int FLUX, int REC>
 size_t fused_lds(const TileGeo& T) {
@@ -959,16 +1029,19 @@ void fu_launch(const DevGeo& G, const TileGeo& T, const StageArgs& A, const Step
     }
     return;
   }
-  const size_t lds = fu_lds<FLUX, REC>(T, r == 2);
   dim3 grid(T.ntiles, S), blk(tile_fu_block(T));
-  if (r == 2) {
-    auto k = k_fu_tile<FLUX, REC, 1>;
-    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k, grid, blk, lds, st, G, T, A, P, Wn, dtpart, r);
+  auto go = [&](auto kern, size_t lds) {
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, grid, blk, lds, st, G, T, A, P, Wn, dtpart, r);
+  };
+  if (A.ns) {
+    const size_t lds = fu_lds<FLUX, REC, true>(T, r == 2);
+    if (r == 2) go(k_fu_tile<FLUX, REC, 1, true>, lds);
+    else go(k_fu_tile<FLUX, REC, 0, true>, lds);
   } else {
-    auto k = k_fu_tile<FLUX, REC, 0>;
-    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k, grid, blk, lds, st, G, T, A, P, Wn, dtpart, r);
+    const size_t lds = fu_lds<FLUX, REC, false>(T, r == 2);
+    if (r == 2) go(k_fu_tile<FLUX, REC, 1, false>, lds);
+    else go(k_fu_tile<FLUX, REC, 0, false>, lds);
   }
 }
 template <int FLUX>
@@ -992,6 +1065,12 @@ int tile_fu_block(const TileGeo& T) {
   return std::max(b, T.fu_block);
 }
 int tile_fu_block_max() { return 512; }
+
+size_t tile_fu_lds_ns(const TileGeo& T, int flux, int rec) {
+  if (flux == F_ROE) return rec == R_FIRST ? fu_lds<F_ROE, R_FIRST, true>(T, true) : fu_lds<F_ROE, R_MINMOD, true>(T, true);
+  if (flux == F_RUS) return rec == R_FIRST ? fu_lds<F_RUS, R_FIRST, true>(T, true) : fu_lds<F_RUS, R_MINMOD, true>(T, true);
+  return fu_lds<F_KEPEC, R_FIRST, true>(T, true);
+}
 
 size_t tile_fu_lds(const TileGeo& T, int flux, int rec) {
   if (flux == F_ROE) {

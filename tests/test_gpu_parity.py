@@ -136,14 +136,23 @@ def test_rk4_and_fixed_dt():
     assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-11
 
 
-def test_navier_stokes_tgv():
+@pytest.mark.parametrize("path,flux,rec", [("auto", "kepec", "first"), ("csr", "kepec", "first"),
+                                           ("auto", "kepes_tecno_roe", "minmod")])
+def test_navier_stokes_tgv(path, flux, rec):
+    """Taylor-Green vortex with viscous and heat fluxes: tiled stage kernels
+    (viscous cell fluxes from LDS, viscous time-step term) and the CSR path
+    against the oracle, per-step logs included."""
     m = tgv_mesh(25)
-    prm = T.Params(model="ns", flux="kepec", mu_ref=1.0 / 1600, prandtl=0.72, cfl=0.5,
-                   final_time=10.0)
+    prm = T.Params(model="ns", flux=flux, reconstruct=rec, mu_ref=1.0 / 1600, prandtl=0.72,
+                   cfl=0.5, final_time=10.0, kernel_path=path)
     s, o = _pair(m, prm, taylor_green(m.x, m.y))
-    assert rel_l2(s.residual(), o.residual()) < 1e-12
-    s.run(100)
-    o.run(100)
+    # at Mach 0.1 the residual (~1e-3) is a difference of fluxes carrying the
+    # background pressure 1/(gamma M^2) ~ 71: the ulp-level differences of the
+    # fused multiply-adds in the TeCNO flux grow by that ratio
+    assert rel_l2(s.residual(), o.residual()) < (1e-12 if flux == "kepec" else 1e-11)
+    _, lg = s.run(100, want_log=True)
+    _, lo = o.run(100, want_log=True)
+    assert np.allclose(lg[:, 1:3], lo[:, 1:3], rtol=1e-12, atol=0)
     assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-10
     k1, e1 = s.global_integrals()
     k2, e2 = o.global_integrals()
@@ -420,3 +429,26 @@ def test_boundary_vertex_with_many_cells():
         s.run(6)
         o.run(6)
         assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-12
+
+
+@pytest.mark.parametrize("path", ["auto", "csr"])
+def test_step_matches_run_and_lands_on_target(path):
+    """tsm_step advances exactly like tsm_run; with a target time it clamps the
+    step so the sample's time lands on it."""
+    m = vortex_mesh(17)
+    prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=10.0,
+                   kernel_path=path)
+    a, b = T.Solver(m, prm), T.Solver(m, prm)
+    for s in (a, b):
+        s.set_state(vortex(m.x, m.y))
+    _, log = a.run(5, want_log=True)
+    for k in range(5):
+        dt, res = b.step()
+        assert dt[0] == log[k, 1] and res[0] == log[k, 3]
+    assert np.array_equal(a.get_state(), b.get_state())
+    t0 = b.time()[0]
+    target = t0 + 0.3 * log[-1, 1]
+    b.step(target)
+    assert abs(b.time()[0] - target) <= 2e-16 * target  # t + (target - t)
+    b.step()
+    assert b.time()[0] > target
