@@ -353,10 +353,14 @@ extern "C" int tsm_set_sample_material(tsm_ctx* c, int s, double gamma, double R
 extern "C" int tsm_set_state(tsm_ctx* c, int s, const double* prim) {
   if (!c || !prim || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
   const int nv = c->G.nv;
-  for (int i = 0; i < nv; ++i)
-    if (!(prim[4 * i] > 0.0) || !(prim[4 * i + 3] > 0.0))
-      return fail(TSM_E_STATE, "initial temperature and pressure must be positive (vertex " +
-                                   std::to_string(i) + ")");
+  for (int i = 0; i < nv; ++i) {
+    if (!(prim[4 * i] > 0.0) || !(prim[4 * i + 3] > 0.0) || !std::isfinite(prim[4 * i]) ||
+        !std::isfinite(prim[4 * i + 3]))
+      return fail(TSM_E_STATE, "initial temperature and pressure must be positive and finite "
+                               "(vertex " + std::to_string(i) + ")");
+    if (!std::isfinite(prim[4 * i + 1]) || !std::isfinite(prim[4 * i + 2]))
+      return fail(TSM_E_STATE, "initial velocity is not finite (vertex " + std::to_string(i) + ")");
+  }
   std::vector<double> aos(4 * (size_t)nv);
   for (int i = 0; i < nv; ++i)
     for (int k = 0; k < 4; ++k) aos[4 * (size_t)c->perm[i] + k] = prim[4 * i + k];
