@@ -119,17 +119,21 @@ int tsm_gradients(tsm_ctx* ctx, int sample, double* dE);
 int tsm_compute_dt(tsm_ctx* ctx, int sample, double* dt_global);
 
 /* Advances every sample by up to nsteps time steps of the loop body of
-   FiniteVolume::solve (src/fv.cc:2273-2368); a sample stops by itself when it
-   reaches final_time.  done[s] receives the steps sample s performed; log
-   (optional, n_samples*nsteps*4, sample-major) receives per step
-   (iter, dt_global, t, res_total), zero rows after a sample stopped. */
+   FiniteVolume::solve (src/fv.cc:2273-2368); a sample stops by itself when its
+   normalised residual falls to min_residue, when it reaches final_time
+   (unsteady) or after max_iter steps (steady).  In steady mode (unsteady = 0)
+   every vertex advances with its own time step cfl*area/lambda (local time
+   stepping, src/fv.cc:1591-1618) and t advances by the smallest of them.
+   done[s] receives the steps sample s performed; log (optional,
+   n_samples*nsteps*4, sample-major) receives per step (iter, dt_global, t,
+   res_total), zero rows after a sample stopped. */
 int tsm_run(tsm_ctx* ctx, long nsteps, long* done, double* log);
 /* One pass of the loop body of FiniteVolume::solve (src/fv.cc:2277-2316) for
    every running sample; t_target > 0 additionally clamps the step so that a
    sample's time lands on t_target exactly instead of passing it (as the
    reference does at an output time stamp).  dt_out[n_samples] and
    res_norm_out[n_samples] (optional) receive each sample's last dt and
-   res_total (src/fv.cc:1862-1876). */
+   res_total (src/fv.cc:1862-1876).  t_target > 0 is refused in steady mode. */
 int tsm_step(tsm_ctx* ctx, double t_target, double* dt_out, double* res_norm_out);
 /* tsm_run over several contexts at once (meshes of different resolution, e.g.
    the levels of a multilevel Monte Carlo run): each context's steps go to its
@@ -143,6 +147,12 @@ int tsm_time(tsm_ctx* ctx, int sample, double* t, long* iter);
    res_total; iter; counter = index of the next output time stamp
    (src/fv.cc:2324-2343); done = the sample has left its time loop. */
 int tsm_status(tsm_ctx* ctx, int sample, double* v, long* iter, int* counter, int* done);
+/* Residual norms of the sample's last step before normalisation, as the
+   steady-mode residual log prints them (FiniteVolume::compute_residual_norm,
+   src/fv.cc:1821-1851; log_messages, src/fv.cc:1944-1957): norms[4] =
+   sqrt(sum_i w_i (R_i,q / A_i)^2) / n_vertex for q = mass, x momentum,
+   y momentum, energy. */
+int tsm_residual_norms(tsm_ctx* ctx, int sample, double* norms);
 
 /* FiniteVolume::compute_error_norm (src/fv.cc:1712-1795): exact primitive
    field in, out[24] = (L1, L2, Linf) for (T, p, u, v, w, rho, Mach, entropy). */

@@ -1115,6 +1115,9 @@ double orc_compute_dt(orc_solver* s) {
   compute_dt(s);
   return s->dt_global;
 }
+void orc_local_dt(const orc_solver* s, double* dt) {
+  for (int i = 0; i < s->nv; ++i) dt[i] = s->dt[i];
+}
 
 // src/fv.cc:2230-2395 loop body
 long orc_run(orc_solver* s, long nsteps, double* log) {

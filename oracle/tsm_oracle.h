@@ -81,6 +81,9 @@ void orc_get_state(const orc_solver* s, double* prim);
 void orc_gradients(orc_solver* s, double* dE /* V*8: (f,vx,vy,l) x (x,y) */);
 void orc_residual(orc_solver* s, double* R /* V*4 */);
 double orc_compute_dt(orc_solver* s);
+/* per-vertex time steps of the last compute_dt (local in steady mode, all
+   equal to the global one in unsteady mode), V */
+void orc_local_dt(const orc_solver* s, double* dt);
 
 /* time loop: run up to nsteps of the reference while-loop body; returns steps
    done.  log (optional) receives 4 doubles per step: iter, dt, t, res_total */

@@ -182,6 +182,7 @@ def lib():
         L.tsm_time.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_long)]
         L.tsm_status.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_long),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.tsm_residual_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_error_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.tsm_global.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tsm_geometry_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -356,6 +357,13 @@ class Solver:
         it, cnt, dn = C.c_long(), C.c_int(), C.c_int()
         _check(lib().tsm_status(self._h, sample, _dptr(v), C.byref(it), C.byref(cnt), C.byref(dn)))
         return dict(t=v[0], dt=v[1], res=v[2], iter=it.value, counter=cnt.value, done=bool(dn.value))
+
+    def residual_norms(self, sample=0):
+        """Last step's residual norms (mass, x/y momentum, energy) before
+        normalisation, as the steady-mode residual log prints them."""
+        out = np.zeros(4)
+        _check(lib().tsm_residual_norms(self._h, sample, _dptr(out)))
+        return out
 
     def time(self, sample=0):
         t, it = C.c_double(), C.c_long()

@@ -225,7 +225,6 @@ int main(int argc, char** argv) {
     }
     return 0;
   }
-  if (S.time_mode != "unsteady") { std::cerr << "ERROR --> steady time_mode is not supported" << std::endl; return 1; }
   if (S.time_scheme == "heuns") { std::cerr << "ERROR --> heuns is not supported (it is broken in the reference)" << std::endl; return 1; }
   PartFile P;
   if (!read_part_file(loc + "/PARTITION/part_file_0.dat", P)) return 1;
@@ -339,15 +338,19 @@ int main(int argc, char** argv) {
     if (S.write_log) res_f[k].open(dirs[k] + "/residual.dat", std::ios::app);
     if (S.has_global) glob_f[k].open(dirs[k] + "/global.dat", std::ios::app);
   }
-  auto output = [&](int k, int counter, double t) {
+  const bool steady = S.time_mode == "steady";
+  // a steady run lists only its final solution in the master file and keeps
+  // overwriting sol0001 every write_frequency steps (src/fv.cc:2241-2245, :2318-2323, :2371-2377)
+  auto output = [&](int k, int counter, double t, bool master = true) {
     tsm_get_state(ctx, k, W.data());
     tsm_gradients(ctx, k, dE.data());
     std::string name = stamp_name("sol", counter);
     write_vtk(dirs[k] + "/" + name + ".vtk", P, W, dE, dcarea, M, S, t);
-    std::ofstream(dirs[k] + "/master_file.visit", std::ios::app) << name << ".vtk" << std::endl;
+    if (master) std::ofstream(dirs[k] + "/master_file.visit", std::ios::app) << name << ".vtk" << std::endl;
   };
-  // online statistics across samples at each stamp (Welford in sample order)
-  const int nst = S.n_time_stamps + 1;
+  // online statistics across samples at each stamp (Welford in sample order);
+  // a steady run has two: the initial and the converged state
+  const int nst = steady ? 2 : S.n_time_stamps + 1;
   std::vector<std::vector<std::vector<double>>> snap;
   if (S.online_stat) snap.assign(nst, std::vector<std::vector<double>>(ns));
   auto keep = [&](int k, int counter) {
@@ -356,7 +359,7 @@ int main(int argc, char** argv) {
     snap[counter][k] = W;
   };
   for (int k = 0; k < ns; ++k) {
-    if (S.write_soln) output(k, 0, 0.0);
+    if (S.write_soln) output(k, 0, 0.0, !(steady && restart));
     keep(k, 0);
   }
   // time loop, all samples together; outputs as each reaches its stamps
@@ -377,16 +380,25 @@ int main(int argc, char** argv) {
       tsm_status(ctx, k, v, &iter, &counter, &dn);
       if (done[k] > 0) {
         const double* L = &logbuf[4 * (size_t)k];
-        if (S.write_log)
+        if (S.write_log && steady) {  // src/fv.cc:1944-1957
+          double rn[4];
+          tsm_residual_norms(ctx, k, rn);
+          res_f[k] << std::setw(10) << (long)L[0] << "  " << std::scientific << std::setprecision(4)
+                   << L[1] << "  " << L[3] << "  " << rn[0] << "  " << rn[1] << "  " << rn[2] << "  "
+                   << 0.0 << "  " << rn[3] << std::endl;
+        } else if (S.write_log) {
           res_f[k] << std::setw(10) << (long)L[0] << "  " << std::scientific << std::setprecision(12)
                    << L[1] << "  " << L[2] << "  " << L[3] << std::endl;
+        }
         if (S.has_global) {
           double ke, ent;
           tsm_global(ctx, k, &ke, &ent);
           glob_f[k] << std::defaultfloat << std::setprecision(6) << (long)L[0] << "  " << L[2]
                     << "  " << ke << "  " << ent << std::endl;
         }
-        if (counter > cnt_before[k]) {
+        if (steady) {
+          if (S.write_soln && iter % S.write_frequency == 0) output(k, 1, v[0], false);
+        } else if (counter > cnt_before[k]) {
           if (S.write_soln) output(k, cnt_before[k], v[0]);
           keep(k, cnt_before[k]);
           cnt_before[k] = counter;
@@ -405,6 +417,10 @@ int main(int argc, char** argv) {
       if (dn || done[k] == 0) {
         finished[k] = 1;
         --live;
+        if (steady) {  // final output of a steady run (src/fv.cc:2371-2382)
+          if (S.write_soln) output(k, 1, v[0]);
+          keep(k, 1);
+        }
         if (!std::isfinite(v[2]))
           std::cerr << "    " << dirs[k] << ": non-finite residual at t = " << v[0] << ", time loop left" << std::endl;
       }

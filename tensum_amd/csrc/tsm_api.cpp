@@ -111,11 +111,14 @@ int validate_params(const tsm_params& p) {
   if (p.reconstruct < 0 || p.reconstruct > 6) return fail(TSM_E_ARG, "unknown reconstruction");
   if (p.time_scheme != 0 && p.time_scheme != 1)
     return fail(TSM_E_ARG, "time_scheme must be ssprk3 (0) or rk4 (1)");
-  if (!p.unsteady) return fail(TSM_E_ARG, "steady time_mode (local time stepping) is not supported");
+  if (p.unsteady != 0 && p.unsteady != 1) return fail(TSM_E_ARG, "unsteady must be 0 or 1");
   if (p.time_step == 0.0 && p.cfl == 0.0) return fail(TSM_E_ARG, "Either time_step or cfl must be non-zero");
   if (p.time_step > 0.0 && p.cfl > 0.0) return fail(TSM_E_ARG, "both time_step and cfl specified");
-  if (!(p.final_time > 0.0)) return fail(TSM_E_ARG, "final_time must be positive for unsteady flow");
-  if (p.n_time_stamps <= 0) return fail(TSM_E_ARG, "time_stamps must be > 0");
+  if (p.unsteady && !(p.final_time > 0.0))
+    return fail(TSM_E_ARG, "final_time must be positive for unsteady flow");
+  if (p.unsteady && p.n_time_stamps <= 0) return fail(TSM_E_ARG, "time_stamps must be > 0");
+  if (!p.unsteady && p.max_iter <= 0) return fail(TSM_E_ARG, "max_iter must be > 0");
+  if (!(p.min_residue >= 0.0)) return fail(TSM_E_ARG, "min_residue must be >= 0");
   if (p.model == 0 && p.mu_ref > 0.0) return fail(TSM_E_ARG, "mu_ref must be 0 for the model=euler");
   return TSM_OK;
 }
@@ -304,14 +307,19 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   AL(c->E[0], 4 * cs); AL(c->E[1], 4 * cs);
   AL(A.Uold, 4 * cs); AL(A.dE, 8 * cs); AL(A.F, 4 * A.fs); AL(A.R, 4 * cs);
   if (params->time_scheme == 1) AL(A.R2, 4 * cs);
+  if (!params->unsteady) AL(A.dtl, cs);  // local time steps (src/fv.cc:1591-1618)
   if (params->liou_fix) AL(A.ssw, cs);
   AL(A.partial, (size_t)S * std::max(c->nblk_v, c->tiling.ntiles) * 4);
   if (c->tiled) AL(c->dtpart, (size_t)S * c->tiling.ntiles);
   AL(A.ctl, (size_t)S);
   double* dg; double* dr; double* ts;
   AL(dg, (size_t)S); AL(dr, (size_t)S);
-  std::vector<double> tst(params->n_time_stamps + 1);
-  double tdt = params->final_time / params->n_time_stamps;
+  // time stamps of an unsteady run (src/fv.cc:50-60); a steady run has none
+  // and no final time (src/parameter.cc:250-252)
+  const int n_stamps = params->unsteady ? params->n_time_stamps : 1;
+  const double final_time = params->unsteady ? params->final_time : 1.0e20;
+  std::vector<double> tst(n_stamps + 1);
+  double tdt = final_time / n_stamps;
   for (size_t i = 0; i < tst.size(); ++i) tst[i] = i * tdt;
   AL(ts, tst.size());
 #undef AL
@@ -329,9 +337,8 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   A.ns = params->model == 1;
   A.ns_samples = S;
   A.ignore_done = 0;
-  c->SP = tsm::StepParams{params->min_residue, params->final_time, params->time_step,
-                          params->cfl, (long long)params->max_iter, params->unsteady,
-                          params->n_time_stamps};
+  c->SP = tsm::StepParams{params->min_residue, final_time, params->time_step,
+                          params->cfl, (long long)params->max_iter, params->unsteady, n_stamps};
   std::vector<tsm::Ctl> ctl(S);
   for (auto& q : ctl) { q = tsm::Ctl{}; q.res_total = 1e20; q.counter = 1; q.done = 1; }
   HIP_TRY(hipMemcpy(A.ctl, ctl.data(), S * sizeof(tsm::Ctl), hipMemcpyHostToDevice));
@@ -564,6 +571,8 @@ extern "C" int tsm_run(tsm_ctx* c, long nsteps, long* done, double* log) {
 
 extern "C" int tsm_step(tsm_ctx* c, double t_target, double* dt_out, double* res_out) {
   if (!c) return fail(TSM_E_ARG, "bad argument");
+  if (t_target > 0.0 && !c->P.unsteady)
+    return fail(TSM_E_ARG, "t_target needs an unsteady run (steady steps use local time steps)");
   const int S = c->S;
   std::vector<tsm::Ctl> before(S), after(S);
   HIP_TRY(hipStreamSynchronize(c->st));
@@ -660,6 +669,15 @@ extern "C" int tsm_status(tsm_ctx* c, int s, double* v, long* iter, int* counter
   if (iter) *iter = (long)q.iter;
   if (counter) *counter = q.counter;
   if (done) *done = q.done;
+  return TSM_OK;
+}
+
+extern "C" int tsm_residual_norms(tsm_ctx* c, int s, double* norms) {
+  if (!c || s < 0 || s >= c->S || !norms) return fail(TSM_E_ARG, "bad argument");
+  tsm::Ctl q;
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(&q, c->A.ctl + s, sizeof(q), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 4; ++k) norms[k] = q.rn[k];
   return TSM_OK;
 }
 

@@ -16,6 +16,7 @@ struct Ctl {
   double t, dt, res0, res_total;
   long long iter;
   int counter, reached, done, pad;
+  double rn[4];  // last step's residual norms: mass, x/y momentum, energy
 };
 
 // Read-only mesh arrays (device pointers).
@@ -124,6 +125,7 @@ struct StageArgs {
   double* R2;      // 4 components (rk4)
   double* ssw;
   double* partial; // per-block reduction scratch
+  double* dtl;     // per-vertex local time step (steady mode), null in unsteady mode
   Ctl* ctl;
   const double* gamma; const double* gasR;
   const double* tstamps;

@@ -403,7 +403,8 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
           part[k] = (q * q) * w;
         }
       }
-      double dt = A.ctl[s].dt;
+      // steady mode advances each vertex with its own dt (src/fv.cc:1591-1618)
+      double dt = A.dtl ? A.dtl[off + v] : A.ctl[s].dt;
       double factor = dt / (1.0 * area);
       Prim w0 = load_prim(A.W_in, off + v);
       Cons U;
@@ -523,6 +524,7 @@ __global__ __launch_bounds__(256) void k_dt(DevGeo G, StageArgs A, StepParams P)
       }
       dtv = P.cfl * G.dcarea[v] / tot;
     }
+    if (A.dtl) A.dtl[off + v] = dtv;
   }
   double bm = block_min(dtv, lds);
   if (threadIdx.x == 0) A.partial[(size_t)s * gridDim.x + blockIdx.x] = bm;
@@ -569,7 +571,10 @@ __global__ __launch_bounds__(256) void k_step_final(StageArgs A, StepParams P, i
   block_sum<4>(part, lds);
   if (threadIdx.x == 0) {
     double n[4];
-    for (int k = 0; k < 4; ++k) n[k] = sqrt(part[k]) / nv;
+    for (int k = 0; k < 4; ++k) {
+      n[k] = sqrt(part[k]) / nv;
+      c.rn[k] = n[k];
+    }
     double tot = sqrt((n[0] * n[0]) + (n[1] * n[1] + n[2] * n[2]) + (n[3] * n[3]));
     if (c.iter == 0) {
       c.res0 = tot;

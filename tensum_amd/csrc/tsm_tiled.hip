@@ -605,7 +605,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
     }
     const double a_rk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
     const double b_rk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
-    const double factor = A.ctl[s].dt / (1.0 * area);
+    const double factor = (A.dtl ? A.dtl[off + v] : A.ctl[s].dt) / (1.0 * area);
     const double b = b_rk[r], a = a_rk[r], c = -factor * b_rk[r];
     Cons U = prim2con(gas, w0);
     U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
@@ -673,7 +673,10 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
       }
     }
     double dtv = 1.0e20;
-    if (own) dtv = (P.time_step > 0.0) ? P.time_step : P.cfl * G.dcarea[v] / lam;
+    if (own) {
+      dtv = (P.time_step > 0.0) ? P.time_step : P.cfl * G.dcarea[v] / lam;
+      if (A.dtl) A.dtl[off + v] = dtv;  // steady mode: read by this vertex only, next step
+    }
     block_partials(part, dtv, wred, &arrive, A.partial, dtpart, (size_t)s * gridDim.x + t);
   }
 }
@@ -881,7 +884,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
     }
     const double a_rk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
     const double b_rk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
-    const double factor = A.ctl[s].dt / (1.0 * area);
+    const double factor = (A.dtl ? A.dtl[off + v] : A.ctl[s].dt) / (1.0 * area);
     const double b = b_rk[r], a = a_rk[r], c = -factor * b_rk[r];
     Cons U = prim2con(gas, w0);
     U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
@@ -916,7 +919,10 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
   }
   if (LAST) {
     double dtv = 1.0e20;
-    if (own) dtv = (P.time_step > 0.0) ? P.time_step : P.cfl * G.dcarea[v] / lam;
+    if (own) {
+      dtv = (P.time_step > 0.0) ? P.time_step : P.cfl * G.dcarea[v] / lam;
+      if (A.dtl) A.dtl[off + v] = dtv;  // steady mode: read by this vertex only, next step
+    }
     block_partials(part, dtv, wred, &arrive, A.partial, dtpart, (size_t)s * gridDim.x + t);
   }
 }
@@ -956,7 +962,10 @@ __global__ __launch_bounds__(1024) void k_step_final_tiled(StageArgs A, StepPara
   x = bmin(x, red);
   if (threadIdx.x == 0) {
     double n[4];
-    for (int k = 0; k < 4; ++k) n[k] = sqrt(part[k]) / nv;
+    for (int k = 0; k < 4; ++k) {
+      n[k] = sqrt(part[k]) / nv;
+      c.rn[k] = n[k];
+    }
     const double tot = sqrt((n[0] * n[0]) + (n[1] * n[1] + n[2] * n[2]) + (n[3] * n[3]));
     if (c.iter == 0) {
       c.res0 = tot;

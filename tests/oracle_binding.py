@@ -52,6 +52,7 @@ def lib():
         L.orc_residual.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_compute_dt.restype = C.c_double
         L.orc_compute_dt.argtypes = [C.c_void_p]
+        L.orc_local_dt.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_run.restype = C.c_long
         L.orc_run.argtypes = [C.c_void_p, C.c_long, C.c_void_p]
         L.orc_time.restype = C.c_double
@@ -133,6 +134,11 @@ class Oracle:
 
     def compute_dt(self):
         return lib().orc_compute_dt(self._h)
+
+    def local_dt(self):
+        a = np.empty(self.nv)
+        lib().orc_local_dt(self._h, _p(a))
+        return a
 
     def run(self, nsteps, want_log=False):
         log = np.zeros((max(nsteps, 1), 4))

@@ -102,3 +102,45 @@ def test_driver_run_matches_oracle(tmp_path):
     head = int((sd / "LAST_RESTART_HEAD.txt").read_text())
     rst = np.loadtxt(sd / f"restart_{head}_0.dat", max_rows=mesh.n_vertex)
     assert rst.shape == (mesh.n_vertex, 5)
+
+
+@pytest.mark.gpu
+def test_driver_steady_run_matches_oracle(tmp_path):
+    """time_mode steady: local time steps, the 8-column steady residual log
+    (src/fv.cc:1944-1957), sol0001 rewritten every `frequency` steps and listed
+    once, at the end (src/fv.cc:2241-2245, :2318-2323, :2371-2377)."""
+    mesh = vortex_mesh(17)
+    write_part_file(mesh, str(tmp_path), periodic_tags=(1, 2, 3, 4))
+    text = VORTEX_PARAM.format(final_time=1.0, stamps=2)
+    for a, b in (("time_mode    unsteady", "time_mode    steady"), ("max_iter     5000", "max_iter     30"),
+                 ("frequency             100", "frequency             10"),
+                 ("find_error            yes", "find_error            no")):
+        assert a in text
+        text = text.replace(a, b)
+    (tmp_path / "param.in").write_text(text)
+    r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    sd = tmp_path / "SAMPLE_0"
+    res = np.loadtxt(sd / "residual.dat")
+    o = O.Oracle(mesh, T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8,
+                                time_mode="steady", max_iter=30))
+    o.set_state(vortex(mesh.x, mesh.y, box=None))
+    n, log = o.run(10 ** 6, want_log=True)
+    assert n == 30 and res.shape == (30, 8)
+    assert np.array_equal(res[:, 0], log[:, 0])
+    np.testing.assert_allclose(res[:, 1], log[:, 1], rtol=6e-5)   # dt_global, 5 digits
+    np.testing.assert_allclose(res[:, 2], log[:, 3], rtol=6e-5)   # normalised residual
+    comp = res[:, 3:]
+    assert np.all(comp[:, 3] == 0.0)                              # z momentum
+    tot = np.sqrt((comp[:, [0, 1, 2, 4]] ** 2).sum(1))
+    np.testing.assert_allclose(tot / tot[0], log[:, 3], rtol=3e-4)
+    master = (sd / "master_file.visit").read_text().split()
+    assert master == ["!NBLOCKS", "1", "sol0000_0.vtk", "sol0001_0.vtk"]
+    assert not (sd / "sol0002_0.vtk").exists()
+    text = (sd / "sol0001_0.vtk").read_text()
+    assert "TIME" not in text                                     # steady files carry no time
+    lines = text.splitlines()
+    i = lines.index("SCALARS pressure double 1")
+    p = np.array([float(v) for v in lines[i + 2:i + 2 + mesh.n_vertex]])
+    np.testing.assert_allclose(p, o.get_state()[:, 3], rtol=1e-10)

@@ -19,7 +19,8 @@ def _mesh():
 
 @pytest.mark.parametrize("change,needle", [
     ({"gamma": 1.0}, "gamma"),
-    ({"time_mode": "steady"}, "steady"),
+    ({"time_mode": "steady", "max_iter": 0}, "max_iter"),
+    ({"min_residue": -1.0}, "min_residue"),
     ({"cfl": 0.0}, "time_step or cfl"),
     ({"time_step": 0.1}, "both time_step and cfl"),
     ({"final_time": 0.0}, "final_time"),
