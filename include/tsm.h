@@ -89,6 +89,11 @@ typedef struct tsm_mesh {
   const int* periodic_v;     /* set members */
   int n_bc;                  /* boundary table: tag -> type + constant state */
   const int* bc_tag; const int* bc_type; const double* bc_state; /* 4*n_bc (T,u,v,p) */
+  /* n_bc or NULL: BoundaryCondition::adiabatic (src/bc.h:226-276).  An
+     adiabatic noslip wall keeps the fluid's temperature (no temperature in
+     its param.in block) and its viscous wall flux carries no heat; slip walls
+     are always adiabatic; NULL makes noslip walls isothermal at their T. */
+  const int* bc_adiabatic;
 } tsm_mesh;
 
 typedef struct tsm_ctx tsm_ctx;

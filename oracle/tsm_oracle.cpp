@@ -381,8 +381,12 @@ Flx inviscid_bflux(const Mat& M, const Prim& w, const Prim& wb, V2 n) {
   return F;
 }
 
-// src/material.cc:158-230 (interior cells; z terms are exact zeros)
-void viscous_flux(const Mat& M, const Ent& s, const V2* dE, const V2* nrm, Flx* out) {
+// src/material.cc:158-230 (z terms are exact zeros); with `wall` set, the
+// boundary flux of src/material.cc:236-302 through the single normal nrm[0]:
+// the same terms, minus Cor = -(0 + c2 e4^2 (dE_l . n)) in energy when the wall
+// is adiabatic
+void viscous_flux(const Mat& M, const Ent& s, const V2* dE, const V2* nrm, Flx* out,
+                  bool wall = false, bool adiabatic = false) {
   const double R = M.p.gas_const;
   double mu = M.viscosity(-1 / (R * s.l));
   double k = mu * M.Cp / M.p.prandtl;
@@ -406,12 +410,16 @@ void viscous_flux(const Mat& M, const Ent& s, const V2* dE, const V2* nrm, Flx* 
   K22[2] = c1 * (-dE[2].y + e2 * dE[3].y);
   K22[3] = mu * (e1 * dE[1].y) + c1 * e2 * dE[2].y +
            (-mu * (e1 * e1) - c1 * e2 * e2 + c2 * e4) * dE[3].y;
-  for (int j = 0; j < 3; ++j) {
+  for (int j = 0; j < (wall ? 1 : 3); ++j) {
     V2 n = nrm[j];
     out[j].m = 0.0;
     out[j].mx = -((K11[1] + K12[1]) * n.x + (K21[1] + K22[1]) * n.y) * e4;
     out[j].my = -((K11[2] + K12[2]) * n.x + (K21[2] + K22[2]) * n.y) * e4;
     out[j].e = -((K11[3] + K12[3]) * n.x + (K21[3] + K22[3]) * n.y) * e4;
+    if (wall) {
+      double Cor = adiabatic ? -1.0 * (0.0 + c2 * e4 * e4 * (dE[3].x * n.x + dE[3].y * n.y)) : 0.0;
+      out[j].e = out[j].e - Cor;
+    }
   }
 }
 
@@ -431,7 +439,7 @@ struct Cell {
   V2 normal[3];
   std::vector<int> nbr_bface;
 };
-struct BC { int type; Prim state; };
+struct BC { int type; Prim state; bool adiabatic; };
 
 }  // namespace
 
@@ -474,6 +482,9 @@ bool build_geometry(orc_solver* s, const orc_mesh* m) {
     b.type = m->bc_type[i];
     b.state = Prim{m->bc_state[4 * i], m->bc_state[4 * i + 1], m->bc_state[4 * i + 2],
                    m->bc_state[4 * i + 3]};
+    // src/bc.h:233-276: slip walls are adiabatic; a noslip wall is unless it
+    // names a temperature
+    b.adiabatic = b.type == BC_SLIP || (b.type == BC_NOSLIP && m->bc_adiabatic && m->bc_adiabatic[i]);
     s->bc[m->bc_tag[i]] = b;
   }
   s->cell.resize(s->nc);
@@ -730,11 +741,15 @@ void bc_apply2(const orc_solver* s, const BC& b, const Face& bf, Prim* st) {
       st[1] = st[0];
       break;
     }
-    case BC_NOSLIP:
+    case BC_NOSLIP:  // src/bc.h:411-431
       st[0].u = b.state.u; st[0].v = b.state.v;
       st[1].u = st[0].u; st[1].v = st[0].v;
       st[1].p = st[0].p;
-      st[0].T = b.state.T; st[1].T = b.state.T;
+      if (b.adiabatic) {
+        st[1].T = st[0].T;
+      } else {
+        st[0].T = b.state.T; st[1].T = b.state.T;
+      }
       break;
     case BC_PRESSURE: st[1].p = b.state.p; break;
     case BC_INLET: st[0] = b.state; st[1] = st[0]; break;
@@ -755,7 +770,10 @@ void bc_apply1(const BC& b, const Face& bf, Prim& st) {
       st.v += un.y * -fact;
       break;
     }
-    case BC_NOSLIP: st.u = b.state.u; st.v = b.state.v; st.T = b.state.T; break;
+    case BC_NOSLIP:  // src/bc.h:438-451
+      st.u = b.state.u; st.v = b.state.v;
+      if (!b.adiabatic) st.T = b.state.T;
+      break;
     case BC_PRESSURE: st.p = b.state.p; break;
     case BC_INLET: st = b.state; break;
     default: break;
@@ -863,13 +881,14 @@ void compute_inviscid_residual(orc_solver* s) {
   }
 }
 
-// src/fv.cc:1197-1232 (cells without a non-periodic boundary face)
+// src/fv.cc:1197-1232 (cells without a non-periodic boundary face), then
+// src/fv.cc:1236-1312 (cells with one, once per such boundary face)
 bool compute_viscous_residual(orc_solver* s) {
   const Mat& M = s->M;
   const double fact = 1.0 / 3.0;
   for (int i = 0; i < s->nc; ++i) {
     const Cell& c = s->cell[i];
-    if (!c.nbr_bface.empty()) { g_err = "viscous boundary cells are not restated"; return false; }
+    if (!c.nbr_bface.empty()) continue;
     const Ent &e0 = s->E[c.v[0]], &e1 = s->E[c.v[1]], &e2 = s->E[c.v[2]];
     Ent a;
     a.f = e0.f * fact + e1.f * fact + e2.f * fact;
@@ -879,6 +898,34 @@ bool compute_viscous_residual(orc_solver* s) {
     Flx out[3];
     viscous_flux(M, a, &s->dEc[(size_t)i * 4], c.normal, out);
     for (int j = 0; j < 3; ++j) addF(s->Rs[c.v[j]], out[j], -(0.5 * 1.0));
+  }
+  for (const auto& bf : s->bface) {
+    const Cell& c = s->cell[bf.lcell];
+    if (c.nbr_bface.empty() || bf.periodic) continue;
+    // mean entropy variables of the boundary states of the cell's boundary
+    // edges, each edge's end points in face order; the states come from the
+    // one-state apply with the edge's face (its half dual face normal)
+    Ent a{0.0, 0.0, 0.0, 0.0};
+    for (int ind : c.nbr_bface) {
+      const Face& f = s->face[c.face[ind]];
+      const BC& b = s->bcof(f.type);
+      for (int k = 0; k < 2; ++k) {
+        Prim st = s->W[f.v[k]];
+        bc_apply1(b, f, st);
+        Ent e = prim2ent(M, st);
+        a.f += e.f; a.vx += e.vx; a.vy += e.vy; a.l += e.l;
+      }
+    }
+    const double sc = 0.5 * 1.0 / c.nbr_bface.size();
+    a.f *= sc; a.vx *= sc; a.vy *= sc; a.l *= sc;
+    const V2* dEc = &s->dEc[(size_t)bf.lcell * 4];
+    Flx out[3];
+    viscous_flux(M, a, dEc, c.normal, out);
+    for (int j = 0; j < 3; ++j) addF(s->Rs[c.v[j]], out[j], -(0.5 * 1.0));
+    Flx fb;
+    viscous_flux(M, a, dEc, &bf.normal, &fb, true, s->bcof(bf.type).adiabatic);
+    addF(s->Rs[bf.v[0]], fb, 0.5 * 1.0);
+    addF(s->Rs[bf.v[1]], fb, 0.5 * 1.0);
   }
   return true;
 }
@@ -1236,6 +1283,17 @@ void orc_moments(int nv, int S, const double* prim, const double* gamma, const d
 }
 
 double orc_logavg(double a, double b) { return logavg(a, b); }
+
+void orc_viscous_flux(const orc_params* p, const double* ent, const double* dE, const double* n,
+                      int wall, int adiabatic, double* F) {
+  Mat M; M.p = *p; M.Cp = p->gamma * p->gas_const / (p->gamma - 1.0);
+  V2 g[4];
+  for (int k = 0; k < 4; ++k) g[k] = V2{dE[2 * k], dE[2 * k + 1]};
+  V2 nrm[3] = {V2{n[0], n[1]}, V2{0, 0}, V2{0, 0}};
+  Flx out[3];
+  viscous_flux(M, Ent{ent[0], ent[1], ent[2], ent[3]}, g, nrm, out, wall != 0, adiabatic != 0);
+  F[0] = out[0].m; F[1] = out[0].mx; F[2] = out[0].my; F[3] = out[0].e;
+}
 
 void orc_num_flux(const orc_params* p, const double* xl, const double* xr, const double* Wl,
                   const double* Wr, const double* dEl, const double* dEr, const double* n,

@@ -57,6 +57,9 @@ typedef struct orc_mesh {
      state (T,u,v,p) used by farfield/inlet/pressure/noslip */
   int n_bc;
   const int* bc_tag; const int* bc_type; const double* bc_state; /* 4*n_bc */
+  /* n_bc or NULL: BoundaryCondition::adiabatic (src/bc.h:226-276); slip walls
+     are always adiabatic, NULL leaves noslip walls isothermal */
+  const int* bc_adiabatic;
 } orc_mesh;
 
 typedef struct orc_solver orc_solver;
@@ -102,6 +105,11 @@ void orc_moments(int nv, int S, const double* prim, const double* gamma, const d
 
 /* pointwise pieces */
 double orc_logavg(double a, double b);
+/* viscous flux through normal n of a state with entropy variables ent[4]
+   (f, vx, vy, l) and gradient dE[8]; wall = 1: the wall flux
+   (src/material.cc:236-302) with the adiabatic correction when adiabatic */
+void orc_viscous_flux(const orc_params* p, const double* ent, const double* dE, const double* n,
+                      int wall, int adiabatic, double* F);
 /* primitive (T,u,v,p) left/right, gradients 8 each, coords, normal, ssw */
 void orc_num_flux(const orc_params* p, const double* xl, const double* xr,
                   const double* Wl, const double* Wr, const double* dEl,

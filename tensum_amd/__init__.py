@@ -51,7 +51,8 @@ class TsmMesh(C.Structure):
                 ("n_periodic", C.c_int), ("periodic_ptr", C.POINTER(C.c_int)),
                 ("periodic_v", C.POINTER(C.c_int)),
                 ("n_bc", C.c_int), ("bc_tag", C.POINTER(C.c_int)),
-                ("bc_type", C.POINTER(C.c_int)), ("bc_state", C.POINTER(C.c_double))]
+                ("bc_type", C.POINTER(C.c_int)), ("bc_state", C.POINTER(C.c_double)),
+                ("bc_adiabatic", C.POINTER(C.c_int))]
 
 
 @dataclass
@@ -125,7 +126,9 @@ class Mesh:
     cell_v: np.ndarray
     periodic_ptr: np.ndarray
     periodic_v: np.ndarray
-    bc: dict = field(default_factory=dict)   # tag -> (type name, (T,u,v,p))
+    # tag -> (type name, (T,u,v,p)[, adiabatic]); a noslip wall is isothermal at
+    # its state's T unless adiabatic is true (slip walls always are)
+    bc: dict = field(default_factory=dict)
 
     @property
     def n_vertex(self):
@@ -156,6 +159,9 @@ class Mesh:
         st = np.array([self.bc[t][1] if len(self.bc[t]) > 1 else (0, 0, 0, 0) for t in tags],
                       dtype=np.float64).reshape(-1)
         m.bc_state = ptr(st if st.size else np.zeros(4), C.c_double)
+        adia = [int(self.bc[t][2]) if len(self.bc[t]) > 2 else int(self.bc[t][0] == "slip")
+                for t in tags]
+        m.bc_adiabatic = ptr(np.array(adia if adia else [0], dtype=np.int32), C.c_int)
         return m, keep
 
 

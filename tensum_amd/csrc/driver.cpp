@@ -231,7 +231,7 @@ int main(int argc, char** argv) {
   const int nv = (int)P.x.size();
 
   // boundary table: constant boundary states from the expressions
-  std::vector<int> bc_tag, bc_type;
+  std::vector<int> bc_tag, bc_type, bc_adiabatic;
   std::vector<double> bc_state;
   const std::map<std::string, int> bcode = {{"slip", 1}, {"noslip", 2}, {"farfield", 3}, {"inlet", 4},
                                             {"outlet", 5}, {"pressure", 6}, {"periodic", 7}};
@@ -253,6 +253,9 @@ int main(int argc, char** argv) {
       st[k] = a;
     }
     bc_state.insert(bc_state.end(), st, st + 4);
+    // a noslip wall without a temperature is adiabatic (src/bc.h:246-276)
+    bc_adiabatic.push_back(kv.second.type == "slip" ||
+                           (kv.second.type == "noslip" && !kv.second.fn.count("temperature")));
   }
   tsm_mesh m{};
   m.n_vertex = nv; m.x = P.x.data(); m.y = P.y.data();
@@ -260,6 +263,7 @@ int main(int argc, char** argv) {
   m.n_cell = (int)P.cv.size() / 3; m.cell_v = P.cv.data();
   m.n_periodic = (int)P.pptr.size() - 1; m.periodic_ptr = P.pptr.data(); m.periodic_v = P.pv.data();
   m.n_bc = (int)bc_tag.size(); m.bc_tag = bc_tag.data(); m.bc_type = bc_type.data(); m.bc_state = bc_state.data();
+  m.bc_adiabatic = bc_adiabatic.data();
 
   // samples and their random inputs (src/fv.cc:502-551)
   const int ns = (int)S.sample_list.size();

@@ -7,6 +7,7 @@
 // sums) is a device kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -144,21 +145,17 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   }
   const tsm::Geometry& g = c->geo;
   const int nv = g.nv, nc = g.nc, nf = (int)g.face.size(), nb = (int)g.bface.size();
-  if (params->model == 1) {
-    for (int k = 0; k < nc; ++k)
-      if (g.cell_has_nonper_bface[k]) {
-        delete c;
-        return fail(TSM_E_ARG, "Navier-Stokes with non-periodic boundary cells is not supported");
-      }
-  }
+  bool wall_cells = false;  // Navier-Stokes cells with a non-periodic bface
+  if (params->model == 1)
+    for (int k = 0; k < nc; ++k) wall_cells = wall_cells || g.cell_has_nonper_bface[k];
   if ((size_t)nf >= (1u << 30) || (size_t)nb >= (1u << 30)) {
     delete c;
     return fail(TSM_E_MESH, "mesh too large for 30-bit face ids");
   }
   // ---- path and vertex numbering -----------------------------------------
-  // tiled stage kernels: SSP-RK3 without the Liou fix, Euler or Navier-Stokes
-  // (NS meshes have no non-periodic boundary cells, checked above)
-  c->tiled = params->kernel_path == 0 && params->time_scheme == 0 && !params->liou_fix;
+  // tiled stage kernels: SSP-RK3 without the Liou fix, Euler, or Navier-Stokes
+  // without wall cells (those run on the CSR kernels)
+  c->tiled = params->kernel_path == 0 && params->time_scheme == 0 && !params->liou_fix && !wall_cells;
   std::vector<int> perm(nv), inv(nv);
   if (c->tiled) {
     int tb = params->tile_vertices > 0 ? params->tile_vertices : 128;
@@ -236,8 +233,11 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     bn[2 * b] = g.bface[b].nx; bn[2 * b + 1] = g.bface[b].ny;
     btype[b] = g.bface[b].bc;
     for (int i = 0; i < mesh->n_bc; ++i)
-      if (mesh->bc_tag[i] == g.bface[b].tag)
+      if (mesh->bc_tag[i] == g.bface[b].tag) {
         for (int k = 0; k < 4; ++k) bstate[4 * b + k] = mesh->bc_state ? mesh->bc_state[4 * i + k] : 0.0;
+        if (btype[b] == TSM_BC_NOSLIP && mesh->bc_adiabatic && mesh->bc_adiabatic[i])
+          btype[b] = tsm::BC_NOSLIP_ADIABATIC;
+      }
   }
   std::vector<double> xs(nv), ys(nv), dca(nv), gfact(nv), wgt(nv);
   std::vector<int> pset(nv);
@@ -250,6 +250,60 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   std::vector<int> pptr(1, 0), pv;
   for (auto& P : g.periodic) { for (int v : P) pv.push_back(perm[v]); pptr.push_back((int)pv.size()); }
 
+  // Navier-Stokes wall cells (src/fv.cc:1236-1312): each cell's boundary edges
+  // in its slot order with their bfaces, and per vertex the bfaces, in bface
+  // order, whose cell holds the vertex (the order the reference's bface loop
+  // adds the cell and wall fluxes in)
+  std::vector<int> blc(std::max(nb, 1), -1), cwf(2 * (size_t)nc, -1), cwb(2 * (size_t)nc, -1);
+  std::vector<int> vbe_ptr(nv + 1, 0), vbe;
+  if (wall_cells) {
+    std::vector<int> face_bface(nf, -1);
+    {
+      std::vector<std::pair<long long, int>> key;
+      for (int b = 0; b < nb; ++b) {
+        if (g.bface[b].periodic) continue;
+        int a0 = g.bface[b].v[0], a1 = g.bface[b].v[1];
+        key.push_back({(long long)std::min(a0, a1) * nv + std::max(a0, a1), b});
+      }
+      std::sort(key.begin(), key.end());
+      for (int f = 0; f < nf; ++f) {
+        if (g.face[f].type < 0 || g.face[f].periodic) continue;
+        int a0 = g.face[f].v[0], a1 = g.face[f].v[1];
+        long long k = (long long)std::min(a0, a1) * nv + std::max(a0, a1);
+        auto it = std::lower_bound(key.begin(), key.end(), std::make_pair(k, -1));
+        if (it == key.end() || it->first != k) { delete c; return fail(TSM_E_MESH, "boundary edge without a bface"); }
+        face_bface[f] = it->second;
+      }
+    }
+    for (int k = 0; k < nc; ++k) {
+      int t = 0;
+      for (int j = 0; j < 3; ++j) {
+        int f = g.cell_slot_face[3 * k + j];
+        if (f < 0 || face_bface[f] < 0) continue;
+        if (t == 2) { delete c; return fail(TSM_E_MESH, "cell with three boundary edges"); }
+        cwf[2 * k + t] = f;
+        cwb[2 * k + t] = face_bface[f];
+        ++t;
+      }
+    }
+    for (int b = 0; b < nb; ++b) {
+      if (g.bface[b].periodic) continue;
+      blc[b] = g.bface[b].lcell;
+      for (int j = 0; j < 3; ++j) vbe_ptr[cv[3 * blc[b] + j] + 1]++;
+    }
+    for (int i = 0; i < nv; ++i) vbe_ptr[i + 1] += vbe_ptr[i];
+    vbe.resize(vbe_ptr[nv]);
+    std::vector<int> pos(vbe_ptr.begin(), vbe_ptr.end() - 1);
+    for (int b = 0; b < nb; ++b) {
+      if (g.bface[b].periodic) continue;
+      for (int j = 0; j < 3; ++j) {
+        int v = cv[3 * blc[b] + j];
+        bool on = v == bvv[2 * b] || v == bvv[2 * b + 1];
+        vbe[pos[v]++] = (int)((on ? 0x80000000u : 0u) | (unsigned)b);
+      }
+    }
+  }
+  if (vbe.empty()) vbe.push_back(0);
   // ---- device ------------------------------------------------------------
   hipError_t he = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
   if (he != hipSuccess) { delete c; return fail(TSM_E_HIP, "hipStreamCreate failed"); }
@@ -261,6 +315,7 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   UP(fv, fv); UP(fn, fn); UP(vfe_ptr, vfe_ptr); UP(vfe, vfe); UP(vc_ptr, vc_ptr); UP(vc, vc);
   UP(cv, cv); UP(cbf, cbf); UP(cnb, cnb); UP(bv, bvv); UP(bn, bn); UP(btype, btype);
   UP(bstate, bstate); UP(pset, pset); UP(pptr, pptr); UP(pv, pv);
+  UP(blc, blc); UP(cwf, cwf); UP(cwb, cwb); UP(vbe_ptr, vbe_ptr); UP(vbe, vbe);
 #undef UP
   if (c->tiled) {
     const tsm::Tiling& T = c->tiling;

@@ -32,6 +32,12 @@ struct DevGeo {
   const int* cv;        // 3*nc
   const int* cbf;       // 2*nc: non-periodic bfaces of the cell (bface order), -1 padded
   const int* cnb;       // nc: cell has a non-periodic bface
+  // Navier-Stokes wall cells (cells with a non-periodic bface)
+  const int* blc;       // nb: the bface's cell
+  const int* cwf;       // 2*nc: the cell's boundary edges as face ids, cell slot order, -1 padded
+  const int* cwb;       // 2*nc: the bface of each of those edges
+  const int* vbe_ptr;   // nv+1
+  const int* vbe;       // per vertex: bfaces whose cell holds it, ascending; bit 31 = vertex on the bface
   const int* bv;        // 2*nb (oriented)
   const double* bn;     // 2*nb outward edge normal
   const int* btype;     // nb
@@ -63,13 +69,83 @@ struct ViscParams {
 };
 
 
+// Device-only boundary code: a noslip wall without a temperature of its own
+// (BoundaryCondition::adiabatic, src/bc.h:246-276).  Slip walls are adiabatic
+// by construction.
+constexpr int BC_NOSLIP_ADIABATIC = 8;
+
+__device__ __forceinline__ bool bc_adiabatic(int type) {
+  return type == TSM_BC_SLIP || type == BC_NOSLIP_ADIABATIC;
+}
+// types whose one-state apply changes the state (the gradient's boundary
+// correction recomputes the entropy variables only for these)
+__device__ __forceinline__ bool bc_sets_state(int type) {
+  return type == TSM_BC_SLIP || type == TSM_BC_NOSLIP || type == BC_NOSLIP_ADIABATIC ||
+         type == TSM_BC_PRESSURE || type == TSM_BC_INLET;
+}
+
+// One-state boundary condition (src/bc.h:615-654, :397-451); (nx, ny) is the
+// normal of the face the state belongs to.
+__device__ __forceinline__ void bc_apply1(int type, const double* bs, double nx, double ny,
+                                          Prim& st) {
+  switch (type) {
+    case TSM_BC_SLIP: {
+      double im = 1.0 / sqrt(nx * nx + ny * ny);
+      double ux = nx * im, uy = ny * im;
+      double fact = st.u * ux + st.v * uy;
+      st.u += ux * -fact;
+      st.v += uy * -fact;
+      break;
+    }
+    case TSM_BC_NOSLIP: st.u = bs[1]; st.v = bs[2]; st.T = bs[0]; break;
+    case BC_NOSLIP_ADIABATIC: st.u = bs[1]; st.v = bs[2]; break;
+    case TSM_BC_PRESSURE: st.p = bs[3]; break;
+    case TSM_BC_INLET: st = Prim{bs[0], bs[1], bs[2], bs[3]}; break;
+    default: break;
+  }
+}
+// Two-state boundary condition (src/bc.h:569-610, :385-431)
+__device__ __forceinline__ void bc_apply2(int type, const double* bs, double nx, double ny,
+                                          Prim& s0, Prim& s1) {
+  switch (type) {
+    case TSM_BC_SLIP: {
+      double im = 1.0 / sqrt(nx * nx + ny * ny);
+      double ux = nx * im, uy = ny * im;
+      double fact = s0.u * ux + s0.v * uy;
+      s0.u += ux * -fact;
+      s0.v += uy * -fact;
+      s1 = s0;
+      break;
+    }
+    case TSM_BC_NOSLIP:
+      s0.u = bs[1]; s0.v = bs[2];
+      s1.u = s0.u; s1.v = s0.v; s1.p = s0.p;
+      s0.T = bs[0]; s1.T = bs[0];
+      break;
+    case BC_NOSLIP_ADIABATIC:
+      s0.u = bs[1]; s0.v = bs[2];
+      s1.u = s0.u; s1.v = s0.v; s1.p = s0.p;
+      s1.T = s0.T;
+      break;
+    case TSM_BC_PRESSURE: s1.p = bs[3]; break;
+    case TSM_BC_INLET: s0 = Prim{bs[0], bs[1], bs[2], bs[3]}; s1 = s0; break;
+    case TSM_BC_OUTLET: s1 = s0; break;
+    case TSM_BC_FARFIELD: s1 = Prim{bs[0], bs[1], bs[2], bs[3]}; break;
+    default: break;
+  }
+}
+
 // Viscous flux of a cell delivered to one of its vertices (src/material.cc:
 // 158-230): s = the mean of the cell's three entropy-variable states, dE =
 // the cell gradient of the entropy variables (x, y per component), (nx, ny) =
 // the vertex's cell normal.  mu(T) with T = -1/(R s_l), kappa = mu Cp / Pr.
+// With adiabatic_wall set it is the wall flux of an adiabatic wall
+// (viscous_bflux, src/material.cc:236-302): the heat-conduction share of the
+// energy flux is taken out again; a non-adiabatic wall flux is the cell flux
+// through the wall's normal.
 __device__ __forceinline__ Cons viscous_cell_flux(const Gas& gas, const ViscParams& vp,
                                                   const Ent& s, const double* dE, double nx,
-                                                  double ny) {
+                                                  double ny, bool adiabatic_wall = false) {
   const double R = gas.R;
   double T = -1 / (R * s.l);
   double mu;
@@ -101,6 +177,10 @@ __device__ __forceinline__ Cons viscous_cell_flux(const Gas& gas, const ViscPara
   F.mx = -((K11_1 + K12_1) * nx + (K21_1 + K22_1) * ny) * e4;
   F.my = -((K11_2 + K12_2) * nx + (K21_2 + K22_2) * ny) * e4;
   F.e = -((K11_4 + K12_4) * nx + (K21_4 + K22_4) * ny) * e4;
+  if (adiabatic_wall) {  // src/material.cc:260-264, :300
+    const double Cor = -1.0 * (0.0 + c2 * e4 * e4 * (d4x * nx + d4y * ny));
+    F.e = F.e - Cor;
+  }
   return F;
 }
 

@@ -75,52 +75,6 @@ __device__ __forceinline__ double block_min(double x, double* lds) {
 }
 
 // ---------------------------------------------------------------------------
-// One-state boundary condition, src/bc.h:615-654
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void bc_apply1(int type, const double* bs, double nx, double ny,
-                                          Prim& st) {
-  switch (type) {
-    case TSM_BC_SLIP: {
-      double im = 1.0 / sqrt(nx * nx + ny * ny);
-      double ux = nx * im, uy = ny * im;
-      double fact = st.u * ux + st.v * uy;
-      st.u += ux * -fact;
-      st.v += uy * -fact;
-      break;
-    }
-    case TSM_BC_NOSLIP: st.u = bs[1]; st.v = bs[2]; st.T = bs[0]; break;
-    case TSM_BC_PRESSURE: st.p = bs[3]; break;
-    case TSM_BC_INLET: st = Prim{bs[0], bs[1], bs[2], bs[3]}; break;
-    default: break;
-  }
-}
-// Two-state boundary condition, src/bc.h:569-610
-__device__ __forceinline__ void bc_apply2(int type, const double* bs, double nx, double ny,
-                                          Prim& s0, Prim& s1) {
-  switch (type) {
-    case TSM_BC_SLIP: {
-      double im = 1.0 / sqrt(nx * nx + ny * ny);
-      double ux = nx * im, uy = ny * im;
-      double fact = s0.u * ux + s0.v * uy;
-      s0.u += ux * -fact;
-      s0.v += uy * -fact;
-      s1 = s0;
-      break;
-    }
-    case TSM_BC_NOSLIP:
-      s0.u = bs[1]; s0.v = bs[2];
-      s1.u = s0.u; s1.v = s0.v; s1.p = s0.p;
-      s0.T = bs[0]; s1.T = bs[0];
-      break;
-    case TSM_BC_PRESSURE: s1.p = bs[3]; break;
-    case TSM_BC_INLET: s0 = Prim{bs[0], bs[1], bs[2], bs[3]}; s1 = s0; break;
-    case TSM_BC_OUTLET: s1 = s0; break;
-    case TSM_BC_FARFIELD: s1 = Prim{bs[0], bs[1], bs[2], bs[3]}; break;
-    default: break;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Cell gradient G_c = sum_j N_cj V(v_j) with the boundary correction of
 // src/fv.cc:887-966 (unscaled: G_c = 2 A_c grad V).
 // ---------------------------------------------------------------------------
@@ -152,8 +106,7 @@ __device__ __forceinline__ void cell_gradient(const DevGeo& G, const StageArgs& 
       int type = G.btype[bf];
       Ent E0 = load_ent(A.E_in, off + v0), E1 = load_ent(A.E_in, off + v1);
       Ent q0 = E0, q1 = E1;
-      if (type == TSM_BC_SLIP || type == TSM_BC_NOSLIP || type == TSM_BC_PRESSURE ||
-          type == TSM_BC_INLET) {
+      if (bc_sets_state(type)) {
         Prim s0 = load_prim(A.W_in, off + v0), s1 = load_prim(A.W_in, off + v1);
         bc_apply1(type, G.bstate + 4 * bf, nx, ny, s0);
         bc_apply1(type, G.bstate + 4 * bf, nx, ny, s1);
@@ -296,20 +249,53 @@ __device__ __forceinline__ Cons bface_flux(const DevGeo& G, const StageArgs& A, 
   return inviscid_bflux(gas, s0, s1, nx, ny);
 }
 
+// Mean entropy variables of a wall cell (src/fv.cc:1245-1266): the one-state
+// boundary states of the ends of the cell's boundary edges (slot order, each
+// edge's ends in face order, a slip edge projected with its half dual face
+// normal), summed from zero and scaled by 0.5 / (number of edges).
+__device__ __forceinline__ Ent wall_cell_state(const DevGeo& G, const StageArgs& A, const Gas& gas,
+                                               int c) {
+  Ent s{0.0, 0.0, 0.0, 0.0};
+  int n = 0;
+  for (int t = 0; t < 2; ++t) {
+    const int f = G.cwf[2 * c + t];
+    if (f < 0) break;
+    const int b = G.cwb[2 * c + t];
+    const int type = G.btype[b];
+    const double nx = G.fn[2 * f], ny = G.fn[2 * f + 1];
+    for (int k = 0; k < 2; ++k) {
+      Prim st = load_prim(A.W_in, A.off + G.fv[2 * f + k]);
+      bc_apply1(type, G.bstate + 4 * b, nx, ny, st);
+      const Ent e = prim2ent(gas, st);
+      s.f += e.f; s.vx += e.vx; s.vy += e.vy; s.l += e.l;
+    }
+    ++n;
+  }
+  const double sc = 0.5 * 1.0 / n;
+  s.f *= sc; s.vx *= sc; s.vy *= sc; s.l *= sc;
+  return s;
+}
+
 // Viscous flux of cell c delivered to its local vertex slot j
-// (src/fv.cc:1197-1232, src/material.cc:158-230).
+// (src/fv.cc:1197-1232, src/material.cc:158-230); a wall cell uses its wall
+// state and, for a vertex on its bface b, also returns the wall flux through b
+// (src/fv.cc:1268-1308, src/material.cc:236-302).
 __device__ __forceinline__ Cons viscous_part(const DevGeo& G, const StageArgs& A, const Gas& gas,
-                                             int c, int j) {
+                                             int c, int j, int wall_b = -1, Cons* Fw = nullptr) {
   const size_t off = A.off;
   int a = G.cv[3 * c], b = G.cv[3 * c + 1], d = G.cv[3 * c + 2];
-  Ent ea = load_ent(A.E_in, off + a), eb = load_ent(A.E_in, off + b),
-      ed = load_ent(A.E_in, off + d);
-  const double fact = 1.0 / 3.0;
   Ent s;
-  s.f = ea.f * fact + eb.f * fact + ed.f * fact;
-  s.vx = ea.vx * fact + eb.vx * fact + ed.vx * fact;
-  s.vy = ea.vy * fact + eb.vy * fact + ed.vy * fact;
-  s.l = ea.l * fact + eb.l * fact + ed.l * fact;
+  if (G.cnb[c]) {
+    s = wall_cell_state(G, A, gas, c);
+  } else {
+    Ent ea = load_ent(A.E_in, off + a), eb = load_ent(A.E_in, off + b),
+        ed = load_ent(A.E_in, off + d);
+    const double fact = 1.0 / 3.0;
+    s.f = ea.f * fact + eb.f * fact + ed.f * fact;
+    s.vx = ea.vx * fact + eb.vx * fact + ed.vx * fact;
+    s.vy = ea.vy * fact + eb.vy * fact + ed.vy * fact;
+    s.l = ea.l * fact + eb.l * fact + ed.l * fact;
+  }
   double dE[8];
   cell_gradient(G, A, gas, c, G.cnb[c] != 0, dE);
   double cf = 1.0 / (2.0 * G.carea[c]);
@@ -320,6 +306,9 @@ __device__ __forceinline__ Cons viscous_part(const DevGeo& G, const StageArgs& A
   if (j == 0) { nx = (yb - yd); ny = -(xb - xd); }
   else if (j == 1) { nx = (yd - ya); ny = -(xd - xa); }
   else { nx = (ya - yb); ny = -(xa - xb); }
+  if (wall_b >= 0)
+    *Fw = viscous_cell_flux(gas, A.vp, s, dE, G.bn[2 * wall_b], G.bn[2 * wall_b + 1],
+                            bc_adiabatic(G.btype[wall_b]));
   return viscous_cell_flux(gas, A.vp, s, dE, nx, ny);
 }
 
@@ -373,12 +362,30 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
         }
       }
       if (A.ns) {
+        // interior cells in cell order, then wall cells once per bface in
+        // bface order (src/fv.cc:1199-1232, :1236-1312)
         for (int e = G.vc_ptr[m]; e < G.vc_ptr[m + 1]; ++e) {
-          int c = G.vc[e] & 0x7fffffff;
+          if (G.vc[e] < 0) continue;  // wall cell
+          int c = G.vc[e];
           int j = (G.cv[3 * c] == m) ? 0 : ((G.cv[3 * c + 1] == m) ? 1 : 2);
           Cons Fv = viscous_part(G, As, gas, c, j);
           const double cm = -(0.5 * 1.0);
           acc[0] += Fv.m * cm; acc[1] += Fv.mx * cm; acc[2] += Fv.my * cm; acc[3] += Fv.e * cm;
+        }
+        for (int e = G.vbe_ptr[m]; e < G.vbe_ptr[m + 1]; ++e) {
+          const unsigned ent = (unsigned)G.vbe[e];
+          const int b = (int)(ent & 0x7fffffffu);
+          const bool on = (ent >> 31) != 0;
+          const int c = G.blc[b];
+          const int j = (G.cv[3 * c] == m) ? 0 : ((G.cv[3 * c + 1] == m) ? 1 : 2);
+          Cons Fw;
+          Cons Fv = viscous_part(G, As, gas, c, j, on ? b : -1, &Fw);
+          const double cm = -(0.5 * 1.0);
+          acc[0] += Fv.m * cm; acc[1] += Fv.mx * cm; acc[2] += Fv.my * cm; acc[3] += Fv.e * cm;
+          if (on) {
+            const double cw = 0.5 * 1.0;
+            acc[0] += Fw.m * cw; acc[1] += Fw.mx * cw; acc[2] += Fw.my * cw; acc[3] += Fw.e * cw;
+          }
         }
       }
       if (p < 0) {

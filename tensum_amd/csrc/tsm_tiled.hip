@@ -123,50 +123,6 @@ __device__ __forceinline__ double bmin(double x, double* red) {
   return x;
 }
 
-// One-state boundary condition (src/bc.h:615-654), used by the gradient's
-// boundary correction.
-__device__ __forceinline__ void bc1(int type, const double* bs, double nx, double ny, Prim& st) {
-  switch (type) {
-    case TSM_BC_SLIP: {
-      double im = 1.0 / sqrt(nx * nx + ny * ny);
-      double ux = nx * im, uy = ny * im;
-      double fact = st.u * ux + st.v * uy;
-      st.u += ux * -fact;
-      st.v += uy * -fact;
-      break;
-    }
-    case TSM_BC_NOSLIP: st.u = bs[1]; st.v = bs[2]; st.T = bs[0]; break;
-    case TSM_BC_PRESSURE: st.p = bs[3]; break;
-    case TSM_BC_INLET: st = Prim{bs[0], bs[1], bs[2], bs[3]}; break;
-    default: break;
-  }
-}
-// Two-state boundary condition (src/bc.h:569-610).
-__device__ __forceinline__ void bc2(int type, const double* bs, double nx, double ny, Prim& s0,
-                                    Prim& s1) {
-  switch (type) {
-    case TSM_BC_SLIP: {
-      double im = 1.0 / sqrt(nx * nx + ny * ny);
-      double ux = nx * im, uy = ny * im;
-      double fact = s0.u * ux + s0.v * uy;
-      s0.u += ux * -fact;
-      s0.v += uy * -fact;
-      s1 = s0;
-      break;
-    }
-    case TSM_BC_NOSLIP:
-      s0.u = bs[1]; s0.v = bs[2];
-      s1.u = s0.u; s1.v = s0.v; s1.p = s0.p;
-      s0.T = bs[0]; s1.T = bs[0];
-      break;
-    case TSM_BC_PRESSURE: s1.p = bs[3]; break;
-    case TSM_BC_INLET: s0 = Prim{bs[0], bs[1], bs[2], bs[3]}; s1 = s0; break;
-    case TSM_BC_OUTLET: s1 = s0; break;
-    case TSM_BC_FARFIELD: s1 = Prim{bs[0], bs[1], bs[2], bs[3]}; break;
-    default: break;
-  }
-}
-
 template <int FLUX, int REC>
 __device__ __forceinline__ Cons pair_flux(const Gas& gas, const FluxOpts& fo, double rx, double ry,
                                           const Side& L, const Side& R, const double* gdl,
@@ -186,7 +142,7 @@ __device__ __forceinline__ Cons bflux(const DevGeo& G, const FluxOpts& fo, const
   int type = G.btype[b];
   double nx = G.bn[2 * b], ny = G.bn[2 * b + 1];
   Prim s0 = wm, s1 = wm;
-  bc2(type, G.bstate + 4 * b, nx, ny, s0, s1);
+  bc_apply2(type, G.bstate + 4 * b, nx, ny, s0, s1);
   if (type == TSM_BC_FARFIELD) {
     double area = sqrt(nx * nx + ny * ny);
     double ia = 1.0 / area;
@@ -220,11 +176,10 @@ __device__ __forceinline__ void grad_bnd_correction(const DevGeo& G, const Stage
     const Prim w0 = ld_prim(A.W_in, off + v0), w1 = ld_prim(A.W_in, off + v1);
     const Ent E0 = prim2ent(gas, w0), E1 = prim2ent(gas, w1);
     Ent q0 = E0, q1 = E1;
-    if (type == TSM_BC_SLIP || type == TSM_BC_NOSLIP || type == TSM_BC_PRESSURE ||
-        type == TSM_BC_INLET) {
+    if (bc_sets_state(type)) {
       Prim s0 = w0, s1 = w1;
-      bc1(type, G.bstate + 4 * bf, nx, ny, s0);
-      bc1(type, G.bstate + 4 * bf, nx, ny, s1);
+      bc_apply1(type, G.bstate + 4 * bf, nx, ny, s0);
+      bc_apply1(type, G.bstate + 4 * bf, nx, ny, s1);
       q0 = prim2ent(gas, s0);
       q1 = prim2ent(gas, s1);
     }
@@ -775,11 +730,10 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
           const Prim w0 = ld_prim(A.W_in, off + v0), w1 = ld_prim(A.W_in, off + v1);
           const Ent E0 = prim2ent(gas, w0), E1 = prim2ent(gas, w1);
           Ent q0 = E0, q1 = E1;
-          if (type == TSM_BC_SLIP || type == TSM_BC_NOSLIP || type == TSM_BC_PRESSURE ||
-              type == TSM_BC_INLET) {
+          if (bc_sets_state(type)) {
             Prim s0 = w0, s1 = w1;
-            bc1(type, G.bstate + 4 * bf, nx, ny, s0);
-            bc1(type, G.bstate + 4 * bf, nx, ny, s1);
+            bc_apply1(type, G.bstate + 4 * bf, nx, ny, s0);
+            bc_apply1(type, G.bstate + 4 * bf, nx, ny, s1);
             q0 = prim2ent(gas, s0);
             q1 = prim2ent(gas, s1);
           }

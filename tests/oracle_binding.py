@@ -62,6 +62,8 @@ def lib():
         L.orc_logavg.restype = C.c_double
         L.orc_logavg.argtypes = [C.c_double, C.c_double]
         L.orc_num_flux.argtypes = [C.POINTER(OrcParams)] + [C.c_void_p] * 7 + [C.c_double, C.c_void_p]
+        L.orc_viscous_flux.argtypes = [C.POINTER(OrcParams)] + [C.c_void_p] * 3 + [C.c_int, C.c_int,
+                                                                                   C.c_void_p]
         L.orc_limited_slope.restype = C.c_double
         L.orc_limited_slope.argtypes = [C.POINTER(OrcParams), C.c_double, C.c_double, C.c_double]
         L.orc_reconstruct.argtypes = [C.POINTER(OrcParams), C.c_double] + [C.c_void_p] * 6
@@ -167,6 +169,14 @@ def num_flux(prm, xl, xr, Wl, Wr, dEl, dEr, n, ssw=0.0):
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (xl, xr, Wl, Wr, dEl, dEr, n)]
     F = np.empty(4)
     lib().orc_num_flux(C.byref(P), *[_p(a) for a in arrs], float(ssw), _p(F))
+    return F
+
+
+def viscous_flux(prm, ent, dE, n, wall=False, adiabatic=False):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (ent, dE, n)]
+    F = np.empty(4)
+    lib().orc_viscous_flux(C.byref(params(prm)), *[_p(a) for a in arrs], int(wall), int(adiabatic),
+                           _p(F))
     return F
 
 

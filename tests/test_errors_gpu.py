@@ -48,12 +48,6 @@ def test_bad_meshes_are_refused():
         T.Solver(m, T.Params(**BASE), n_samples=0)
 
 
-def test_navier_stokes_with_wall_cells_is_refused():
-    m = T.jittered_mesh(9, bc={t: ("outlet",) for t in (1, 2, 3, 4)})
-    with pytest.raises(T.TsmError, match="Navier-Stokes"):
-        T.Solver(m, T.Params(model="ns", flux="kepec", mu_ref=0.01, cfl=0.5, final_time=1.0))
-
-
 def test_bad_states_and_indices_are_refused():
     m = _mesh()
     s = T.Solver(m, T.Params(**BASE), n_samples=2)
