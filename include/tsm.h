@@ -110,6 +110,16 @@ void tsm_destroy(tsm_ctx* ctx);
    the reference can vary only IC/BC expressions per sample, src/perturb.cc). */
 int tsm_set_sample_material(tsm_ctx* ctx, int sample, double gamma, double gas_const);
 
+/* Boundary values of one sample, replacing the boundary table's constant
+   states: state[8b+4k .. 8b+4k+3] = (T,u,v,p) of bface b (mesh order) at its
+   end point k (the vertex bface_v[2b+k]).  The reference evaluates each
+   boundary's expressions at every end point's coordinates with the sample's
+   random numbers (BoundaryCondition::apply, src/bc.h:411-562; pert,
+   src/perturb.cc:16-209); values that do not depend on time are passed
+   here.  Only the fields a boundary type reads are used; they must be finite,
+   with T > 0 and p > 0 where the type reads them. */
+int tsm_set_bc_state(tsm_ctx* ctx, int sample, const double* state);
+
 /* Initial state at t = 0 (replaces the IC / restart branch of
    FiniteVolume::initialize_sample_run, src/fv.cc:320-402). prim: 4*n_vertex. */
 int tsm_set_state(tsm_ctx* ctx, int sample, const double* prim);

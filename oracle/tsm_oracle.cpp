@@ -465,6 +465,16 @@ struct orc_solver {
   unsigned counter = 0;
   bool reached = false;
   std::vector<double> time_instance;
+  // boundary states per (bface, end point) of this run's sample, set by
+  // orc_set_bc_state (the reference evaluates each boundary expression at the
+  // end point's coordinates with the sample's random numbers, src/bc.h:411-562);
+  // empty: every bface takes its tag's constant state
+  std::vector<Prim> bst;
+  std::vector<int> face_bface;  // the bface of each boundary edge's face, else -1
+  Prim bstate_at(int bi, int vertex) const {
+    if (bst.empty()) return bc.at(bface[bi].type).state;
+    return bst[2 * (size_t)bi + (bface[bi].v[1] == vertex ? 1 : 0)];
+  }
   const BC& bcof(int tag) const { return bc.at(tag); }
 };
 
@@ -717,6 +727,19 @@ bool build_geometry(orc_solver* s, const orc_mesh* m) {
         c.nbr_bface.push_back(j);
     }
   }
+  {
+    std::map<std::pair<int, int>, int> pair_bface;
+    for (size_t bi = 0; bi < s->bface.size(); ++bi) {
+      const Face& b = s->bface[bi];
+      pair_bface[{std::min(b.v[0], b.v[1]), std::max(b.v[0], b.v[1])}] = (int)bi;
+    }
+    s->face_bface.assign(face.size(), -1);
+    for (size_t i = 0; i < face.size(); ++i) {
+      if (face[i].type == -1) continue;
+      auto it = pair_bface.find({std::min(face[i].v[0], face[i].v[1]), std::max(face[i].v[0], face[i].v[1])});
+      if (it != pair_bface.end()) s->face_bface[i] = it->second;
+    }
+  }
   s->pweight.assign(s->nv, 1.0);
   for (auto& P_ : s->per) {
     double w = 1.0 / (double(P_.size()) * 1.0);
@@ -729,8 +752,9 @@ bool build_geometry(orc_solver* s, const orc_mesh* m) {
 
 namespace {
 
-// Boundary state for the two-state apply, src/bc.h:569-610
-void bc_apply2(const orc_solver* s, const BC& b, const Face& bf, Prim* st) {
+// Boundary state for the two-state apply, src/bc.h:569-610; bs = the
+// boundary expressions' values at the state's vertex
+void bc_apply2(const orc_solver* s, const BC& b, const Prim& bs, const Face& bf, Prim* st) {
   switch (b.type) {
     case BC_SLIP: {
       double im = 1.0 / bf.measure;
@@ -742,25 +766,25 @@ void bc_apply2(const orc_solver* s, const BC& b, const Face& bf, Prim* st) {
       break;
     }
     case BC_NOSLIP:  // src/bc.h:411-431
-      st[0].u = b.state.u; st[0].v = b.state.v;
+      st[0].u = bs.u; st[0].v = bs.v;
       st[1].u = st[0].u; st[1].v = st[0].v;
       st[1].p = st[0].p;
       if (b.adiabatic) {
         st[1].T = st[0].T;
       } else {
-        st[0].T = b.state.T; st[1].T = b.state.T;
+        st[0].T = bs.T; st[1].T = bs.T;
       }
       break;
-    case BC_PRESSURE: st[1].p = b.state.p; break;
-    case BC_INLET: st[0] = b.state; st[1] = st[0]; break;
+    case BC_PRESSURE: st[1].p = bs.p; break;
+    case BC_INLET: st[0] = bs; st[1] = st[0]; break;
     case BC_OUTLET: st[1] = st[0]; break;
-    case BC_FARFIELD: st[1] = b.state; break;
+    case BC_FARFIELD: st[1] = bs; break;
     default: break;
   }
   (void)s;
 }
 // One-state apply, src/bc.h:615-654
-void bc_apply1(const BC& b, const Face& bf, Prim& st) {
+void bc_apply1(const BC& b, const Prim& bs, const Face& bf, Prim& st) {
   switch (b.type) {
     case BC_SLIP: {
       double im = 1.0 / bf.measure;
@@ -771,11 +795,11 @@ void bc_apply1(const BC& b, const Face& bf, Prim& st) {
       break;
     }
     case BC_NOSLIP:  // src/bc.h:438-451
-      st.u = b.state.u; st.v = b.state.v;
-      if (!b.adiabatic) st.T = b.state.T;
+      st.u = bs.u; st.v = bs.v;
+      if (!b.adiabatic) st.T = bs.T;
       break;
-    case BC_PRESSURE: st.p = b.state.p; break;
-    case BC_INLET: st = b.state; break;
+    case BC_PRESSURE: st.p = bs.p; break;
+    case BC_INLET: st = bs; break;
     default: break;
   }
 }
@@ -803,13 +827,14 @@ void compute_gradients(orc_solver* s) {
       g.y = c.normal[0].y * a0[k] + c.normal[1].y * a1[k] + c.normal[2].y * a2[k];
     }
   }
-  for (auto& bf : s->bface) {
+  for (size_t bi = 0; bi < s->bface.size(); ++bi) {
+    const Face& bf = s->bface[bi];
     if (bf.periodic) continue;
     const BC& b = s->bcof(bf.type);
     int v0 = bf.v[0], v1 = bf.v[1];
     Prim st0 = s->W[v0], st1 = s->W[v1];
-    bc_apply1(b, bf, st0);
-    bc_apply1(b, bf, st1);
+    bc_apply1(b, s->bstate_at((int)bi, v0), bf, st0);
+    bc_apply1(b, s->bstate_at((int)bi, v1), bf, st1);
     Ent q0 = prim2ent(M, st0), q1 = prim2ent(M, st1);
     const Ent &E0 = s->E[v0], &E1 = s->E[v1];
     double d[4] = {(q0.f + q1.f - E0.f - E1.f) / 2.0, (q0.vx + q1.vx - E0.vx - E1.vx) / 2.0,
@@ -864,13 +889,14 @@ void compute_inviscid_residual(orc_solver* s) {
     addF(s->Rs[cr], F, -1.0 * 1.0);
   }
   const V2 zero4[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  for (auto& bf : s->bface) {
+  for (size_t bi = 0; bi < s->bface.size(); ++bi) {
+    const Face& bf = s->bface[bi];
     if (bf.periodic) continue;
     const BC& b = s->bcof(bf.type);
     for (int side = 0; side < 2; ++side) {
       int c = bf.v[side];
       Prim st[2] = {s->W[c], s->W[c]};
-      bc_apply2(s, b, bf, st);
+      bc_apply2(s, b, s->bstate_at((int)bi, c), bf, st);
       Flx F;
       if (b.type == BC_FARFIELD)
         F = num_flux(M, s->x[c], s->x[c], st[0], st[1], zero4, zero4, bf.normal, 0.0);
@@ -909,9 +935,10 @@ bool compute_viscous_residual(orc_solver* s) {
     for (int ind : c.nbr_bface) {
       const Face& f = s->face[c.face[ind]];
       const BC& b = s->bcof(f.type);
+      const int fb = s->face_bface[c.face[ind]];
       for (int k = 0; k < 2; ++k) {
         Prim st = s->W[f.v[k]];
-        bc_apply1(b, f, st);
+        bc_apply1(b, s->bstate_at(fb, f.v[k]), f, st);
         Ent e = prim2ent(M, st);
         a.f += e.f; a.vx += e.vx; a.vy += e.vy; a.l += e.l;
       }
@@ -1205,6 +1232,27 @@ long orc_run(orc_solver* s, long nsteps, double* log) {
   return done;
 }
 double orc_time(const orc_solver* s) { return s->elapsed; }
+
+int orc_set_bc_state(orc_solver* s, const int* bface_v, const double* state) {
+  // state: per input bface b and end point k (bface_v[2b+k]) the values (T,u,v,p)
+  std::map<std::pair<int, int>, int> in;
+  const int nb = (int)s->bface.size();
+  for (int b = 0; b < nb; ++b)
+    in[{std::min(bface_v[2 * b], bface_v[2 * b + 1]), std::max(bface_v[2 * b], bface_v[2 * b + 1])}] = b;
+  s->bst.assign(2 * (size_t)nb, Prim{0, 0, 0, 0});
+  for (int bi = 0; bi < nb; ++bi) {
+    const Face& f = s->bface[bi];
+    auto it = in.find({std::min(f.v[0], f.v[1]), std::max(f.v[0], f.v[1])});
+    if (it == in.end()) { g_err = "bface not in the input list"; return -1; }
+    const int b = it->second;
+    for (int e = 0; e < 2; ++e) {
+      const int k = bface_v[2 * b + 1] == f.v[e] ? 1 : 0;
+      const double* q = state + 8 * (size_t)b + 4 * k;
+      s->bst[2 * (size_t)bi + e] = Prim{q[0], q[1], q[2], q[3]};
+    }
+  }
+  return 0;
+}
 
 // src/fv.cc:1712-1795
 void orc_error_norms(const orc_solver* s, const double* ex, double* out) {

@@ -92,6 +92,10 @@ void orc_local_dt(const orc_solver* s, double* dt);
    done.  log (optional) receives 4 doubles per step: iter, dt, t, res_total */
 long orc_run(orc_solver* s, long nsteps, double* log);
 double orc_time(const orc_solver* s);
+/* boundary states of this run per input bface b and end point k (the vertex
+   bface_v[2b+k]): state[8b+4k..] = (T,u,v,p), the boundary expressions
+   evaluated there; replaces the tags' constant states.  0 or -1. */
+int orc_set_bc_state(orc_solver* s, const int* bface_v, const double* state);
 /* error norms vs a given exact primitive field: out[24] =
    (L1,L2,Linf) x (T,p,u,v,w,rho,mach,entropy) */
 void orc_error_norms(const orc_solver* s, const double* exact_prim, double* out);

@@ -189,6 +189,7 @@ def lib():
         L.tsm_status.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_long),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.tsm_residual_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.tsm_set_bc_state.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_error_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.tsm_global.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tsm_geometry_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -306,6 +307,7 @@ class Solver:
         _check(L.tsm_create(C.byref(m), C.byref(self._p), n_samples, C.byref(h)))
         self._h = h
         self.nv = mesh.n_vertex
+        self.n_bface = int(mesh.bface_tag.size)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -363,6 +365,14 @@ class Solver:
         it, cnt, dn = C.c_long(), C.c_int(), C.c_int()
         _check(lib().tsm_status(self._h, sample, _dptr(v), C.byref(it), C.byref(cnt), C.byref(dn)))
         return dict(t=v[0], dt=v[1], res=v[2], iter=it.value, counter=cnt.value, done=bool(dn.value))
+
+    def set_bc_state(self, values, sample=0):
+        """Boundary values of one sample per bface end point: values (n_bface,
+        2, 4) = (T,u,v,p) at the vertices bface_v[b, 0] and bface_v[b, 1]."""
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if a.size != 8 * self.n_bface:
+            raise TsmError(f"expected {8 * self.n_bface} boundary values, got {a.size}")
+        _check(lib().tsm_set_bc_state(self._h, sample, _dptr(a)))
 
     def residual_norms(self, sample=0):
         """Last step's residual norms (mass, x/y momentum, energy) before

@@ -13,6 +13,7 @@
 // status 1 (the reference exits with 0, src/mpi_utils.h:29-35).
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -235,23 +236,31 @@ int main(int argc, char** argv) {
   std::vector<double> bc_state;
   const std::map<std::string, int> bcode = {{"slip", 1}, {"noslip", 2}, {"farfield", 3}, {"inlet", 4},
                                             {"outlet", 5}, {"pressure", 6}, {"periodic", 7}};
+  // Each boundary's expressions (src/bc.h:411-562) are evaluated at every
+  // bface end point with each sample's random numbers once the samples are
+  // known (below); the table holds the values at the tag's first end point
+  // for sample 0.  Expressions of t are refused.
+  const char* bnames[4] = {"temperature", "xvelocity", "yvelocity", "pressure"};
+  std::map<int, std::vector<tsm::Expr>> bexpr;   // tag -> 4 expressions (empty: field not given)
+  std::map<int, std::vector<bool>> bhas;
   for (auto& kv : S.bc) {
     bc_tag.push_back(kv.first);
     bc_type.push_back(bcode.at(kv.second.type));
-    double st[4] = {0, 0, 0, 0};
-    const char* names[4] = {"temperature", "xvelocity", "yvelocity", "pressure"};
+    std::vector<tsm::Expr>& ex = bexpr[kv.first];
+    std::vector<bool>& has = bhas[kv.first];
+    ex.resize(4);
+    has.assign(4, false);
     for (int k = 0; k < 4; ++k) {
-      auto it = kv.second.fn.find(names[k]);
+      auto it = kv.second.fn.find(bnames[k]);
       if (it == kv.second.fn.end()) continue;
-      tsm::Expr e;
-      if (!e.parse(it->second, S.constants, err)) { std::cerr << "ERROR --> " << err << std::endl; return 1; }
-      double a = e.eval(0.0, 0.0, 0.0), b = e.eval(0.37, -1.3, 0.0);
-      if (e.uses_time() || a != b) {
-        std::cerr << "ERROR --> boundary " << kv.first << ": only constant boundary values are supported" << std::endl;
+      if (!ex[k].parse(it->second, S.constants, err)) { std::cerr << "ERROR --> " << err << std::endl; return 1; }
+      if (ex[k].uses_time()) {
+        std::cerr << "ERROR --> boundary " << kv.first << ": time-dependent boundary values are not supported" << std::endl;
         return 1;
       }
-      st[k] = a;
+      has[k] = true;
     }
+    double st[4] = {0, 0, 0, 0};
     bc_state.insert(bc_state.end(), st, st + 4);
     // a noslip wall without a temperature is adiabatic (src/bc.h:246-276)
     bc_adiabatic.push_back(kv.second.type == "slip" ||
@@ -287,11 +296,56 @@ int main(int argc, char** argv) {
       rnd[k] = it->second;
     }
   }
+  // boundary values per sample and bface end point
+  const int nbf = (int)P.btag.size();
+  std::vector<std::vector<double>> bvals(ns, std::vector<double>(8 * (size_t)nbf, 0.0));
+  for (int k = 0; k < ns; ++k) {
+    const std::vector<double>& r = rnd[k];
+    auto pert = [&r](int type, double px, double py, double kk) {
+      double v = 0.0;  // src/perturb.cc:16-209
+      if (tsm_pert(type, px, py, kk, r.data(), (int)r.size(), &v) != TSM_OK) return (double)NAN;
+      return v;
+    };
+    for (int b = 0; b < nbf; ++b) {
+      auto& ex = bexpr[P.btag[b]];
+      const auto& has = bhas[P.btag[b]];
+      for (int e = 0; e < 2; ++e) {
+        const int v = P.bv[2 * b + e];
+        for (int q = 0; q < 4; ++q) {
+          if (!has[q]) continue;
+          ex[q].set_pert(pert);
+          bvals[k][8 * (size_t)b + 4 * e + q] = ex[q].eval(P.x[v], P.y[v], 0.0);
+        }
+      }
+    }
+  }
+  // table: each tag's values at its first end point for sample 0; per-sample
+  // values are passed only if some value differs from its tag's table entry
+  std::vector<int> btab(nbf);
+  std::vector<bool> filled(bc_tag.size(), false);
+  for (int b = 0; b < nbf; ++b) {
+    btab[b] = (int)(std::find(bc_tag.begin(), bc_tag.end(), P.btag[b]) - bc_tag.begin());
+    if (ns > 0 && !filled[btab[b]]) {
+      for (int q = 0; q < 4; ++q) bc_state[4 * (size_t)btab[b] + q] = bvals[0][8 * (size_t)b + q];
+      filled[btab[b]] = true;
+    }
+  }
+  bool varying = false;
+  for (int k = 0; k < ns && !varying; ++k)
+    for (int b = 0; b < nbf && !varying; ++b)
+      for (int i = 0; i < 8; ++i)
+        if (bvals[k][8 * (size_t)b + i] != bc_state[4 * (size_t)btab[b] + i % 4]) varying = true;
   tsm_ctx* ctx = nullptr;
   if (tsm_create(&m, &S.p, ns, &ctx) != TSM_OK) {
     std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
     return 1;
   }
+  if (varying)
+    for (int k = 0; k < ns; ++k)
+      if (tsm_set_bc_state(ctx, k, bvals[k].data()) != TSM_OK) {
+        std::cerr << "ERROR --> boundary values of SAMPLE_" << S.sample_list[k] << ": " << tsm_last_error() << std::endl;
+        return 1;
+      }
   const Mat M{S.p.gamma, S.p.gas_const};
   std::vector<double> dcarea(nv), mcarea(nv);
   tsm_geometry(ctx, nullptr, nullptr, dcarea.data(), mcarea.data());

@@ -242,9 +242,11 @@ bool build_geometry(const tsm_mesh& m, int cell_type, Geometry& g, std::string& 
       b.nx *= -1.0; b.ny *= -1.0;
     }
   }
+  g.removed_faces = 0;
   if (cell_type == 1) {  // src/grid_preproc.cc:640-655 remove_empty_faces
     std::vector<Face> kept;
     for (auto& f : face) if (f.measure > 0.0) kept.push_back(f);
+    g.removed_faces = (int)(face.size() - kept.size());
     face.swap(kept);
   }
   // src/grid_preproc.cc:811-831 (slot ids index the possibly shortened list)

@@ -49,6 +49,10 @@ struct tsm_ctx {
   tsm::TileGeo TG{};
   double* dtpart = nullptr;
   std::vector<int> perm;    // part-file vertex id -> device vertex id
+  // input bface b, end point k (bface_v[2b+k]) -> offset of its boundary
+  // values in a sample's block of G.bstate; device boundary codes
+  std::vector<long long> bmap;
+  std::vector<int> h_btype;
   double* red = nullptr;   // reduction partials for global / error norms
   size_t red_cap = 0;
   std::vector<double> h_gamma, h_R;
@@ -148,6 +152,12 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   bool wall_cells = false;  // Navier-Stokes cells with a non-periodic bface
   if (params->model == 1)
     for (int k = 0; k < nc; ++k) wall_cells = wall_cells || g.cell_has_nonper_bface[k];
+  if (wall_cells && g.removed_faces > 0) {
+    // the reference then reads the cells' face slots against the shortened
+    // face list (src/grid_preproc.cc:811-831), which the wall-cell tables do not follow
+    delete c;
+    return fail(TSM_E_ARG, "Navier-Stokes wall cells with Voronoi duals that lost empty faces are not supported");
+  }
   if ((size_t)nf >= (1u << 30) || (size_t)nb >= (1u << 30)) {
     delete c;
     return fail(TSM_E_MESH, "mesh too large for 30-bit face ids");
@@ -227,14 +237,15 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   std::vector<int> cnb(nc);
   for (int k = 0; k < nc; ++k) cnb[k] = (cbf[2 * k] >= 0) ? 1 : 0;
   std::vector<int> bvv(2 * (size_t)nb), btype(nb);
-  std::vector<double> bn(2 * (size_t)nb), bstate(4 * (size_t)nb, 0.0);
+  std::vector<double> bn(2 * (size_t)nb), bstate(8 * (size_t)nb, 0.0);
   for (int b = 0; b < nb; ++b) {
     bvv[2 * b] = perm[g.bface[b].v[0]]; bvv[2 * b + 1] = perm[g.bface[b].v[1]];
     bn[2 * b] = g.bface[b].nx; bn[2 * b + 1] = g.bface[b].ny;
     btype[b] = g.bface[b].bc;
     for (int i = 0; i < mesh->n_bc; ++i)
       if (mesh->bc_tag[i] == g.bface[b].tag) {
-        for (int k = 0; k < 4; ++k) bstate[4 * b + k] = mesh->bc_state ? mesh->bc_state[4 * i + k] : 0.0;
+        for (int k = 0; k < 4; ++k)  // both end points take the tag's constant state
+          bstate[8 * b + k] = bstate[8 * b + 4 + k] = mesh->bc_state ? mesh->bc_state[4 * i + k] : 0.0;
         if (btype[b] == TSM_BC_NOSLIP && mesh->bc_adiabatic && mesh->bc_adiabatic[i])
           btype[b] = tsm::BC_NOSLIP_ADIABATIC;
       }
@@ -315,6 +326,27 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   UP(fv, fv); UP(fn, fn); UP(vfe_ptr, vfe_ptr); UP(vfe, vfe); UP(vc_ptr, vc_ptr); UP(vc, vc);
   UP(cv, cv); UP(cbf, cbf); UP(cnb, cnb); UP(bv, bvv); UP(bn, bn); UP(btype, btype);
   UP(bstate, bstate); UP(pset, pset); UP(pptr, pptr); UP(pv, pv);
+  G.bs_sample = 0;
+  c->h_btype = btype;
+  if (mesh->n_bface != nb) { delete c; return fail(TSM_E_MESH, "boundary face count changed in preprocessing"); }
+  {
+    std::vector<std::pair<long long, int>> key(nb);
+    for (int b = 0; b < nb; ++b) {
+      int a0 = g.bface[b].v[0], a1 = g.bface[b].v[1];
+      key[b] = {(long long)std::min(a0, a1) * nv + std::max(a0, a1), b};
+    }
+    std::sort(key.begin(), key.end());
+    c->bmap.assign(2 * (size_t)nb, 0);
+    for (int bi = 0; bi < nb; ++bi) {
+      int a0 = mesh->bface_v[2 * bi], a1 = mesh->bface_v[2 * bi + 1];
+      long long k = (long long)std::min(a0, a1) * nv + std::max(a0, a1);
+      auto it = std::lower_bound(key.begin(), key.end(), std::make_pair(k, -1));
+      if (it == key.end() || it->first != k) { delete c; return fail(TSM_E_MESH, "bface lost in preprocessing"); }
+      const int bd = it->second;
+      for (int e = 0; e < 2; ++e)
+        c->bmap[2 * bi + e] = 8LL * bd + 4 * (g.bface[bd].v[1] == mesh->bface_v[2 * bi + e] ? 1 : 0);
+    }
+  }
   UP(blc, blc); UP(cwf, cwf); UP(cwb, cwb); UP(vbe_ptr, vbe_ptr); UP(vbe, vbe);
 #undef UP
   if (c->tiled) {
@@ -403,6 +435,40 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
 }
 
 extern "C" void tsm_destroy(tsm_ctx* c) { delete c; }
+
+extern "C" int tsm_set_bc_state(tsm_ctx* c, int s, const double* state) {
+  if (!c || !state || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  const int nb = c->G.nb;
+  std::vector<double> h(8 * (size_t)nb, 0.0);
+  for (int bi = 0; bi < nb; ++bi)
+    for (int e = 0; e < 2; ++e) {
+      const double* q = state + 8 * (size_t)bi + 4 * e;
+      const long long o = c->bmap[2 * bi + e];
+      const int type = c->h_btype[o / 8];
+      for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(q[k])) return fail(TSM_E_STATE, "non-finite boundary value at bface " + std::to_string(bi));
+      const bool needT = type == TSM_BC_INLET || type == TSM_BC_FARFIELD || type == TSM_BC_NOSLIP;
+      const bool needp = type == TSM_BC_INLET || type == TSM_BC_FARFIELD || type == TSM_BC_PRESSURE;
+      if ((needT && !(q[0] > 0.0)) || (needp && !(q[3] > 0.0)))
+        return fail(TSM_E_STATE, "non-positive boundary T or p at bface " + std::to_string(bi));
+      for (int k = 0; k < 4; ++k) h[o + k] = q[k];
+    }
+  if (c->G.bs_sample == 0 && c->S > 1) {  // samples stop sharing one block
+    double* all = nullptr;
+    int rc = c->alloc(&all, (size_t)c->S * 8 * nb);
+    if (rc) return rc;
+    for (int k = 0; k < c->S; ++k)
+      HIP_TRY(hipMemcpy(all + (size_t)k * 8 * nb, c->G.bstate, 8 * (size_t)nb * sizeof(double),
+                        hipMemcpyDeviceToDevice));
+    c->G.bstate = all;
+    c->G.bs_sample = 8LL * nb;
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }  // captured the old array
+  }
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(const_cast<double*>(c->G.bstate) + (size_t)s * c->G.bs_sample, h.data(),
+                    h.size() * sizeof(double), hipMemcpyHostToDevice));
+  return TSM_OK;
+}
 
 extern "C" int tsm_set_sample_material(tsm_ctx* c, int s, double gamma, double R) {
   if (!c || s < 0 || s >= c->S || !(gamma > 1.0) || !(R > 0.0)) return fail(TSM_E_ARG, "bad argument");

@@ -41,7 +41,8 @@ struct DevGeo {
   const int* bv;        // 2*nb (oriented)
   const double* bn;     // 2*nb outward edge normal
   const int* btype;     // nb
-  const double* bstate; // 4*nb (T,u,v,p) constant boundary state
+  const double* bstate; // boundary values (T,u,v,p) per bface end point: 8*nb per sample
+  long long bs_sample;  // doubles between samples' boundary values (0: all samples share)
   const int* pset;      // nv: periodic set id or -1
   const int* pptr;      // nsets+1
   const int* pv;        // members
@@ -68,6 +69,16 @@ struct ViscParams {
   int mu_model, visc_active, heat_active;
 };
 
+
+// Boundary values of sample s at end point e (0: bv[2b], 1: bv[2b+1]) of bface b:
+// the reference evaluates a boundary's expressions at each end point's
+// coordinates with the sample's random numbers (src/bc.h:411-562).
+__device__ __forceinline__ const double* bc_values(const DevGeo& G, int s, int b, int e) {
+  return G.bstate + (size_t)s * G.bs_sample + 8 * (size_t)b + 4 * e;
+}
+__device__ __forceinline__ int bc_end(const DevGeo& G, int b, int v) {
+  return G.bv[2 * b + 1] == v ? 1 : 0;
+}
 
 // Device-only boundary code: a noslip wall without a temperature of its own
 // (BoundaryCondition::adiabatic, src/bc.h:246-276).  Slip walls are adiabatic
@@ -215,6 +226,7 @@ struct StageArgs {
   int ns;          // Navier-Stokes model active
   int ns_samples;  // number of samples
   int ignore_done; // single-call API entry points compute regardless of the loop state
+  int sample;      // the sample the CSR helpers work on (set per thread block)
 };
 
 // LDS-staged tile kernels (Euler, SSP-RK3)

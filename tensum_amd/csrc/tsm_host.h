@@ -37,6 +37,7 @@ struct Geometry {
   std::vector<double> ccx, ccy;     // cell centroids
   std::vector<int> cell_slot_face;  // 3 per cell: face ids (reference slot order)
   std::vector<uint8_t> cell_has_nonper_bface;
+  int removed_faces = 0;            // empty Voronoi dual faces dropped in preprocessing
   std::vector<Face> face;
   std::vector<BFace> bface;
   std::vector<double> mcarea, dcarea;

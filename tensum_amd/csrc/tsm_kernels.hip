@@ -108,8 +108,8 @@ __device__ __forceinline__ void cell_gradient(const DevGeo& G, const StageArgs& 
       Ent q0 = E0, q1 = E1;
       if (bc_sets_state(type)) {
         Prim s0 = load_prim(A.W_in, off + v0), s1 = load_prim(A.W_in, off + v1);
-        bc_apply1(type, G.bstate + 4 * bf, nx, ny, s0);
-        bc_apply1(type, G.bstate + 4 * bf, nx, ny, s1);
+        bc_apply1(type, bc_values(G, A.sample, bf, 0), nx, ny, s0);
+        bc_apply1(type, bc_values(G, A.sample, bf, 1), nx, ny, s1);
         q0 = prim2ent(gas, s0);
         q1 = prim2ent(gas, s1);
       }
@@ -134,6 +134,7 @@ __global__ __launch_bounds__(256) void k_gradient(DevGeo G, StageArgs A) {
   if (v >= G.nv) return;
   StageArgs As = A;
   As.off = (size_t)s * G.nv;
+  As.sample = s;
   Gas gas{A.gamma[s], A.gasR[s]};
   int p = G.pset[v];
   int m0, m1;
@@ -220,11 +221,11 @@ __global__ __launch_bounds__(256) void k_flux(DevGeo G, StageArgs A) {
 // Boundary flux contribution of bface b at its vertex m (src/fv.cc:1100-1189)
 template <int FLUX, int REC>
 __device__ __forceinline__ Cons bface_flux(const DevGeo& G, const StageArgs& A, const Gas& gas,
-                                           int b, const Prim& wm, double xm, double ym) {
+                                           int b, int m, const Prim& wm, double xm, double ym) {
   int type = G.btype[b];
   double nx = G.bn[2 * b], ny = G.bn[2 * b + 1];
   Prim s0 = wm, s1 = wm;
-  bc_apply2(type, G.bstate + 4 * b, nx, ny, s0, s1);
+  bc_apply2(type, bc_values(G, A.sample, b, bc_end(G, b, m)), nx, ny, s0, s1);
   if (type == TSM_BC_FARFIELD) {
     double area = sqrt(nx * nx + ny * ny);
     double ia = 1.0 / area;
@@ -264,8 +265,9 @@ __device__ __forceinline__ Ent wall_cell_state(const DevGeo& G, const StageArgs&
     const int type = G.btype[b];
     const double nx = G.fn[2 * f], ny = G.fn[2 * f + 1];
     for (int k = 0; k < 2; ++k) {
-      Prim st = load_prim(A.W_in, A.off + G.fv[2 * f + k]);
-      bc_apply1(type, G.bstate + 4 * b, nx, ny, st);
+      const int vb = G.fv[2 * f + k];
+      Prim st = load_prim(A.W_in, A.off + vb);
+      bc_apply1(type, bc_values(G, A.sample, b, bc_end(G, b, vb)), nx, ny, st);
       const Ent e = prim2ent(gas, st);
       s.f += e.f; s.vx += e.vx; s.vy += e.vy; s.l += e.l;
     }
@@ -334,6 +336,7 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
   }
   StageArgs As = A;
   As.off = off;
+  As.sample = s;
   Gas gas{A.gamma[s], A.gasR[s]};
   double part[4] = {0, 0, 0, 0};
   if (v < G.nv) {
@@ -357,7 +360,7 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
           else { acc[0] -= f0; acc[1] -= f1; acc[2] -= f2; acc[3] -= f3; }
         } else {
           if (!have_wm) { wm = load_prim(A.W_in, off + m); have_wm = true; }
-          Cons Fb = bface_flux<FLUX, REC>(G, A, gas, (int)idx, wm, G.x[m], G.y[m]);
+          Cons Fb = bface_flux<FLUX, REC>(G, As, gas, (int)idx, m, wm, G.x[m], G.y[m]);
           acc[0] += Fb.m * 0.5; acc[1] += Fb.mx * 0.5; acc[2] += Fb.my * 0.5; acc[3] += Fb.e * 0.5;
         }
       }

@@ -138,11 +138,11 @@ __device__ __forceinline__ Cons pair_flux(const Gas& gas, const FluxOpts& fo, do
 // Boundary flux of bface b at a vertex with state wm (src/fv.cc:1100-1189)
 template <int FLUX, int REC>
 __device__ __forceinline__ Cons bflux(const DevGeo& G, const FluxOpts& fo, const Gas& gas, int b,
-                                      const Prim& wm) {
+                                      int s, int v, const Prim& wm) {
   int type = G.btype[b];
   double nx = G.bn[2 * b], ny = G.bn[2 * b + 1];
   Prim s0 = wm, s1 = wm;
-  bc_apply2(type, G.bstate + 4 * b, nx, ny, s0, s1);
+  bc_apply2(type, bc_values(G, s, b, bc_end(G, b, v)), nx, ny, s0, s1);
   if (type == TSM_BC_FARFIELD) {
     double area = sqrt(nx * nx + ny * ny);
     double ia = 1.0 / area;
@@ -165,7 +165,7 @@ __device__ __forceinline__ Cons bflux(const DevGeo& G, const FluxOpts& fo, const
 // non-periodic boundary face of the cell, n_b times the mean of the boundary
 // states' entropy variables minus the mean of the vertex ones.
 __device__ __forceinline__ void grad_bnd_correction(const DevGeo& G, const StageArgs& A,
-                                                    const Gas& gas, size_t off, int c,
+                                                    const Gas& gas, int s, size_t off, int c,
                                                     double* g) {
   for (int z = 0; z < 2; ++z) {
     const int bf = G.cbf[2 * c + z];
@@ -178,8 +178,8 @@ __device__ __forceinline__ void grad_bnd_correction(const DevGeo& G, const Stage
     Ent q0 = E0, q1 = E1;
     if (bc_sets_state(type)) {
       Prim s0 = w0, s1 = w1;
-      bc_apply1(type, G.bstate + 4 * bf, nx, ny, s0);
-      bc_apply1(type, G.bstate + 4 * bf, nx, ny, s1);
+      bc_apply1(type, bc_values(G, s, bf, 0), nx, ny, s0);
+      bc_apply1(type, bc_values(G, s, bf, 1), nx, ny, s1);
       q0 = prim2ent(gas, s0);
       q1 = prim2ent(gas, s1);
     }
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(512, PER ? 4 : 8) void k_grad_tile(DevGeo G, TileGe
       const double ed[4] = {SE[d], SE[MS + d], SE[2 * MS + d], SE[3 * MS + d]};
       double g[8];
       cell_grad(SX[a], SX[MS + a], SX[b], SX[MS + b], SX[d], SX[MS + d], ea, eb, ed, g);
-      if (PER && (e >> 30)) grad_bnd_correction(G, A, gas, off, TG.cgid[k], g);
+      if (PER && (e >> 30)) grad_bnd_correction(G, A, gas, s, off, TG.cgid[k], g);
 #pragma unroll
       for (int q = 0; q < 8; ++q) acc[q] += g[q];
     }
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_grad_bnd(DevGeo G, TileGeo TG, StageArg
       const double ea[4] = {Ea.f, Ea.vx, Ea.vy, Ea.l}, eb[4] = {Eb.f, Eb.vx, Eb.vy, Eb.l},
                    ed[4] = {Ed.f, Ed.vx, Ed.vy, Ed.l};
       cell_grad(G.x[va], G.y[va], G.x[vb], G.y[vb], G.x[vd], G.y[vd], ea, eb, ed, g);
-      if (TG.cent[k] >> 30) grad_bnd_correction(G, A, gas, off, c, g);
+      if (TG.cent[k] >> 30) grad_bnd_correction(G, A, gas, s, off, c, g);
     }
     for (int m = 0; m < GB_LANES; ++m) {  // every lane takes part in the shuffles
 #pragma unroll
@@ -486,7 +486,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
       const unsigned e = TG.gent[gk0 + k];
       if (e & 0x4000u) {
         const int b = TG.tbf[bb0 + (e & 0x3fffu)];
-        const Cons Fb = bflux<FLUX, REC>(G, A.fo, gas, b, w0);
+        const Cons Fb = bflux<FLUX, REC>(G, A.fo, gas, b, s, v, w0);
         acc[0] += Fb.m * 0.5; acc[1] += Fb.mx * 0.5; acc[2] += Fb.my * 0.5; acc[3] += Fb.e * 0.5;
       } else {
         gather_face(e, SF, MF, acc);
@@ -732,8 +732,8 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
           Ent q0 = E0, q1 = E1;
           if (bc_sets_state(type)) {
             Prim s0 = w0, s1 = w1;
-            bc_apply1(type, G.bstate + 4 * bf, nx, ny, s0);
-            bc_apply1(type, G.bstate + 4 * bf, nx, ny, s1);
+            bc_apply1(type, bc_values(G, s, bf, 0), nx, ny, s0);
+            bc_apply1(type, bc_values(G, s, bf, 1), nx, ny, s1);
             q0 = prim2ent(gas, s0);
             q1 = prim2ent(gas, s1);
           }
@@ -821,7 +821,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
       const unsigned e = TG.gent[gk0 + k];
       if (e & 0x4000u) {
         const int b = TG.tbf[bb0 + (e & 0x3fffu)];
-        const Cons Fb = bflux<FLUX, REC>(G, A.fo, gas, b, w0);
+        const Cons Fb = bflux<FLUX, REC>(G, A.fo, gas, b, s, v, w0);
         acc[0] += Fb.m * 0.5; acc[1] += Fb.mx * 0.5; acc[2] += Fb.my * 0.5; acc[3] += Fb.e * 0.5;
       } else {
         gather_face(e, SF, MF, acc);

@@ -53,6 +53,7 @@ def lib():
         L.orc_compute_dt.restype = C.c_double
         L.orc_compute_dt.argtypes = [C.c_void_p]
         L.orc_local_dt.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_set_bc_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_run.restype = C.c_long
         L.orc_run.argtypes = [C.c_void_p, C.c_long, C.c_void_p]
         L.orc_time.restype = C.c_double
@@ -95,6 +96,7 @@ class Oracle:
             raise RuntimeError("oracle: " + L.orc_error().decode())
         self._h = C.c_void_p(h)
         self.nv = mesh.n_vertex
+        self._bface_v = np.asarray(mesh.bface_v, dtype=np.int32)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -136,6 +138,12 @@ class Oracle:
 
     def compute_dt(self):
         return lib().orc_compute_dt(self._h)
+
+    def set_bc_state(self, values):
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        bv = np.ascontiguousarray(self._bface_v, dtype=np.int32)
+        if lib().orc_set_bc_state(self._h, _p(bv), _p(a)) != 0:
+            raise RuntimeError("oracle: " + lib().orc_error().decode())
 
     def local_dt(self):
         a = np.empty(self.nv)

@@ -144,3 +144,68 @@ def test_driver_steady_run_matches_oracle(tmp_path):
     i = lines.index("SCALARS pressure double 1")
     p = np.array([float(v) for v in lines[i + 2:i + 2 + mesh.n_vertex]])
     np.testing.assert_allclose(p, o.get_state()[:, 3], rtol=1e-10)
+
+
+def _uq_state(x, r, xc=0.3, Tl=1.0, Tr=0.8, ul=0.75, ur=0.0, pl=1.0, pr=0.1, dd=0.1):
+    """The UQ shock-tube example's expressions (examples/UQ/shocktube/param.in)
+    with pert(4,0,0,k) = r[k-1]."""
+    left = x < xc * (1 + dd * (r[0] - 0.5))
+    T_ = np.where(left, Tl * (1 + dd * (r[1] - 0.5)), Tr * (1 + dd * (r[2] - 0.5)))
+    u = np.where(left, ul * (1 + dd * (r[3] - 0.5)), ur)
+    p = np.where(left, pl * (1 + dd * (r[4] - 0.5)), pr * (1 + dd * (r[5] - 0.5)))
+    return np.stack([T_, u, 0 * x, p], 1)
+
+
+@pytest.mark.gpu
+def test_driver_uq_shocktube_example_matches_oracle(tmp_path):
+    """The reference's UQ example input (11 samples, random inlet values via
+    pert(4, ...) in the boundary expressions, Voronoi duals), with a shorter
+    final time, on a jittered square tagged as the example's mesh: inlet left,
+    outlet right, periodic top/bottom."""
+    n = 17
+    base = T.jittered_mesh(n, seed=11)
+    retag = {4: 100001, 2: 100002, 1: 100003, 3: 100004}
+    tags = np.array([retag[int(t)] for t in base.bface_tag], dtype=np.int32)
+    sets = [[i, (n - 1) * n + i] for i in range(n)]
+    ptr = np.cumsum([0] + [2] * n).astype(np.int32)
+    mesh = T.Mesh(base.x, base.y, tags, base.bface_v, base.cell_v, ptr,
+                  np.array(sum(sets, []), dtype=np.int32), {})
+    write_part_file(mesh, str(tmp_path), periodic_tags=(100003, 100004))
+    text = open(os.path.join(GOLD, "param_uq_shocktube.in")).read()
+    assert "final_time   0.2" in text
+    (tmp_path / "param.in").write_text(text.replace("final_time   0.2", "final_time   0.05"))
+    rng = np.random.default_rng(5)
+    rnd = rng.random((11, 6))
+    (tmp_path / "urnd_nos.txt").write_text(
+        "11\n" + "".join(f"{k} " + " ".join(f"{v:.17g}" for v in rnd[k]) + "\n" for k in range(11)))
+    r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    bv = mesh.bface_v.reshape(-1, 2)
+    # balance/ND is read only when switch_active is yes (src/parameter.cc:396-416):
+    # numerical dissipation stays on
+    prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.5, final_time=0.05,
+                   time_stamps=2, cell="voronoi", nd_active=True, prandtl=1.0)
+    differ = []
+    for k in range(11):
+        st = _uq_state(mesh.x, rnd[k])
+        bc = {100001: ("inlet", tuple(_uq_state(np.zeros(1), rnd[k])[0])), 100002: ("outlet",),
+              100003: ("periodic",), 100004: ("periodic",)}
+        om = T.Mesh(mesh.x, mesh.y, tags, mesh.bface_v, mesh.cell_v, ptr, mesh.periodic_v, bc)
+        o = O.Oracle(om, prm)
+        vals = np.zeros((tags.size, 2, 4))
+        for e in range(2):
+            vals[:, e, :] = np.where((tags == 100001)[:, None], _uq_state(mesh.x[bv[:, e]], rnd[k]), 0.0)
+        o.set_bc_state(vals)
+        o.set_state(st)
+        nsteps, log = o.run(10 ** 6, want_log=True)
+        sd = tmp_path / f"SAMPLE_{k}"
+        res = np.loadtxt(sd / "residual.dat")
+        assert res.shape[0] == nsteps
+        np.testing.assert_allclose(res[:, 1], log[:, 1], rtol=1e-11)
+        lines = (sd / "sol0002_0.vtk").read_text().splitlines()
+        i = lines.index("SCALARS pressure double 1")
+        p = np.array([float(v) for v in lines[i + 2:i + 2 + mesh.n_vertex]])
+        np.testing.assert_allclose(p, o.get_state()[:, 3], rtol=1e-10)
+        differ.append(p)
+    assert np.ptp([d.mean() for d in differ]) > 1e-3  # the samples really differ
