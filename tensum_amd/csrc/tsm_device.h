@@ -101,7 +101,7 @@ __device__ __forceinline__ void bc_apply1(int type, const double* bs, double nx,
                                           Prim& st) {
   switch (type) {
     case TSM_BC_SLIP: {
-      double im = 1.0 / sqrt(nx * nx + ny * ny);
+      double im = fdiv(1.0, fsqrt(nx * nx + ny * ny));
       double ux = nx * im, uy = ny * im;
       double fact = st.u * ux + st.v * uy;
       st.u += ux * -fact;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void bc_apply2(int type, const double* bs, double nx,
                                           Prim& s0, Prim& s1) {
   switch (type) {
     case TSM_BC_SLIP: {
-      double im = 1.0 / sqrt(nx * nx + ny * ny);
+      double im = fdiv(1.0, fsqrt(nx * nx + ny * ny));
       double ux = nx * im, uy = ny * im;
       double fact = s0.u * ux + s0.v * uy;
       s0.u += ux * -fact;
@@ -158,17 +158,17 @@ __device__ __forceinline__ Cons viscous_cell_flux(const Gas& gas, const ViscPara
                                                   const Ent& s, const double* dE, double nx,
                                                   double ny, bool adiabatic_wall = false) {
   const double R = gas.R;
-  double T = -1 / (R * s.l);
+  double T = fdiv(-1.0, R * s.l);
   double mu;
   if (vp.mu_model == 0) mu = vp.mu_ref;
-  else if (vp.mu_model == 1) mu = vp.mu_ref * pow(T, 1.5) / (T + vp.T_0);
-  else mu = vp.mu_ref * pow(T / vp.T_ref, vp.omega);
-  double Cp = gas.gamma * gas.R / (gas.gamma - 1.0);
-  double k = mu * Cp / vp.prandtl;
+  else if (vp.mu_model == 1) mu = fdiv(vp.mu_ref * pow(T, 1.5), T + vp.T_0);
+  else mu = vp.mu_ref * pow(fdiv(T, vp.T_ref), vp.omega);
+  double Cp = fdiv(gas.gamma * gas.R, gas.gamma - 1.0);
+  double k = fdiv(mu * Cp, vp.prandtl);
   if (!vp.visc_active) mu = 0.0;
   if (!vp.heat_active) k = 0.0;
-  double e1 = s.vx / s.l, e2 = s.vy / s.l, e4 = 1.0 / s.l;
-  double c1 = 4.0 * mu / 3.0, c2 = k / R, c3 = 2 * mu / 3.0;
+  double e1 = fdiv(s.vx, s.l), e2 = fdiv(s.vy, s.l), e4 = fdiv(1.0, s.l);
+  double c1 = fdiv(4.0 * mu, 3.0), c2 = fdiv(k, R), c3 = fdiv(2 * mu, 3.0);
   // dE components: [0,1] f, [2,3] vx, [4,5] vy, [6,7] l  (x, y)
   double d1x = dE[2], d1y = dE[3], d2x = dE[4], d2y = dE[5], d4x = dE[6], d4y = dE[7];
   double K11_1 = c1 * (-d1x + e1 * d4x);

@@ -144,8 +144,8 @@ __device__ __forceinline__ Cons bflux(const DevGeo& G, const FluxOpts& fo, const
   Prim s0 = wm, s1 = wm;
   bc_apply2(type, bc_values(G, s, b, bc_end(G, b, v)), nx, ny, s0, s1);
   if (type == TSM_BC_FARFIELD) {
-    double area = sqrt(nx * nx + ny * ny);
-    double ia = 1.0 / area;
+    double area = fsqrt(nx * nx + ny * ny);
+    double ia = fdiv(1.0, area);
     const double z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     constexpr bool TEC = FLUX == F_ROE || FLUX == F_RUS;
     Side L = make_side<TEC, FLUX == F_ROE>(gas, s0), R = make_side<TEC, FLUX == F_ROE>(gas, s1);
@@ -420,8 +420,8 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
     const unsigned lr = TG.f_lr[f0 + j];
     const int l = lr & 0xffff, rr = lr >> 16;
     const double nx = TG.f_n[2 * (size_t)(f0 + j)], ny = TG.f_n[2 * (size_t)(f0 + j) + 1];
-    const double area = sqrt(nx * nx + ny * ny);
-    const double ia = 1.0 / area;
+    const double area = fsqrt(nx * nx + ny * ny);
+    const double ia = fdiv(1.0, area);
     Side Ls, Rs;
     Ls.w = Prim{Q[l], Q[MS + l], Q[2 * MS + l], Q[3 * MS + l]};
     Rs.w = Prim{Q[rr], Q[MS + rr], Q[2 * MS + rr], Q[3 * MS + rr]};
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
         const double xd = Q[L::QX * MS + sd], yd = Q[(L::QX + 1) * MS + sd];
         double dE[8];
         cell_grad(xa, ya, xb, yb, xd, yd, ea, eb, ed, dE);
-        const double cf = 1.0 / (2.0 * G.carea[TG.cgid[k]]);
+        const double cf = fdiv(1.0, 2.0 * G.carea[TG.cgid[k]]);
 #pragma unroll
         for (int q = 0; q < 8; ++q) dE[q] *= cf;
         double nx, ny;
@@ -554,13 +554,13 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
       const double wt = G.weight[v];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double q = acc[k] / area;
+        const double q = fdiv(acc[k], area);
         part[k] = (q * q) * wt;
       }
     }
     const double a_rk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
     const double b_rk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
-    const double factor = (A.dtl ? A.dtl[off + v] : A.ctl[s].dt) / (1.0 * area);
+    const double factor = fdiv(A.dtl ? A.dtl[off + v] : A.ctl[s].dt, 1.0 * area);
     const double b = b_rk[r], a = a_rk[r], c = -factor * b_rk[r];
     Cons U = prim2con(gas, w0);
     U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
@@ -584,7 +584,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
         if (e & 0x4000u) {
           const int bf = TG.tbf[bb0 + (e & 0x3fffu)];
           const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
-          const double ar = sqrt(nx * nx + ny * ny);
+          const double ar = fsqrt(nx * nx + ny * ny);
           const double vn = w1.u * nx + w1.v * ny;
           lam += 0.5 * (fabs(vn) + cm * ar);
         } else {
@@ -606,8 +606,8 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
           const int sl = lr & 0xffff, sr = lr >> 16;
           const double dx = Q[L::QX * MS + sr] * 1.0 + Q[L::QX * MS + sl] * -1.0;
           const double dy = Q[(L::QX + 1) * MS + sr] * 1.0 + Q[(L::QX + 1) * MS + sl] * -1.0;
-          const double dl = sqrt(dx * dx + dy * dy);
-          lam += (mu * SN[2 * MF + j] / dl / rho) * 1.0;
+          const double dl = fsqrt(dx * dx + dy * dy);
+          lam += fdiv(fdiv(mu * SN[2 * MF + j], dl), rho) * 1.0;
         }
       }
     }
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
     }
     double dtv = 1.0e20;
     if (own) {
-      dtv = (P.time_step > 0.0) ? P.time_step : P.cfl * G.dcarea[v] / lam;
+      dtv = (P.time_step > 0.0) ? P.time_step : fdiv(P.cfl * G.dcarea[v], lam);
       if (A.dtl) A.dtl[off + v] = dtv;  // steady mode: read by this vertex only, next step
     }
     block_partials(part, dtv, wred, &arrive, A.partial, dtpart, (size_t)s * gridDim.x + t);
@@ -766,8 +766,8 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
     const unsigned lr = TG.f_lr[f0 + j];
     const int l = lr & 0xffff, rr = lr >> 16;
     const double nx = TG.f_n[2 * (size_t)(f0 + j)], ny = TG.f_n[2 * (size_t)(f0 + j) + 1];
-    const double area = sqrt(nx * nx + ny * ny);
-    const double ia = 1.0 / area;
+    const double area = fsqrt(nx * nx + ny * ny);
+    const double ia = fdiv(1.0, area);
     Side Ls, Rs;
     Ls.w = Prim{Q[l], Q[MS1 + l], Q[2 * MS1 + l], Q[3 * MS1 + l]};
     Rs.w = Prim{Q[rr], Q[MS1 + rr], Q[2 * MS1 + rr], Q[3 * MS1 + rr]};
@@ -832,13 +832,13 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
       const double wt = G.weight[v];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double q = acc[k] / area;
+        const double q = fdiv(acc[k], area);
         part[k] = (q * q) * wt;
       }
     }
     const double a_rk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
     const double b_rk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
-    const double factor = (A.dtl ? A.dtl[off + v] : A.ctl[s].dt) / (1.0 * area);
+    const double factor = fdiv(A.dtl ? A.dtl[off + v] : A.ctl[s].dt, 1.0 * area);
     const double b = b_rk[r], a = a_rk[r], c = -factor * b_rk[r];
     Cons U = prim2con(gas, w0);
     U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
@@ -862,7 +862,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
         if (e & 0x4000u) {
           const int bf = TG.tbf[bb0 + (e & 0x3fffu)];
           const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
-          const double ar = sqrt(nx * nx + ny * ny);
+          const double ar = fsqrt(nx * nx + ny * ny);
           const double vn = w1.u * nx + w1.v * ny;
           lam += 0.5 * (fabs(vn) + cm * ar);
         } else {
@@ -874,7 +874,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
   if (LAST) {
     double dtv = 1.0e20;
     if (own) {
-      dtv = (P.time_step > 0.0) ? P.time_step : P.cfl * G.dcarea[v] / lam;
+      dtv = (P.time_step > 0.0) ? P.time_step : fdiv(P.cfl * G.dcarea[v], lam);
       if (A.dtl) A.dtl[off + v] = dtv;  // steady mode: read by this vertex only, next step
     }
     block_partials(part, dtv, wred, &arrive, A.partial, dtpart, (size_t)s * gridDim.x + t);
