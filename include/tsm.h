@@ -53,7 +53,9 @@ typedef struct tsm_params {
   double tvb_M;       /* M_value of tvb_minmod */
   double ev_alpha, ev_beta;  /* flux_par alpha / beta */
   int nd_active, visc_active, heat_active;
-  int time_scheme;    /* 0 ssprk3, 1 rk4 */
+  int time_scheme;    /* 0 ssprk3, 1 rk4, 2 heuns: Heun's two-stage SSP scheme as
+                         src/fv.cc:1675-1686 means it (the reference's own update
+                         overwrites a term and gives NaN after one step) */
   int unsteady;       /* 1 unsteady, 0 steady */
   double cfl, time_step, final_time, min_residue;
   long max_iter;

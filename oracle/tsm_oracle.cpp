@@ -1080,9 +1080,16 @@ void update_solution(orc_solver* s, unsigned r) {
   for (int i = 0; i < s->nv; ++i) {
     double factor = s->dt[i] / (1.0 * s->dcarea[i]);
     Flx U;
-    if (M.p.time_scheme == 0) {
+    if (M.p.time_scheme == 0 || M.p.time_scheme == 2) {
+      // SSP-RK3, or Heun's method (src/fv.cc:1675-1686) as it is meant:
+      // U = (1 - r/2) U_r + (r/2) U_old - (1 - r/2) dt/A R.  The reference
+      // overwrites the first term (Vector::equ where sadd adds), which gives
+      // NaN after one step; this is the two-stage SSP scheme its
+      // coefficients describe.
+      const bool heun = M.p.time_scheme == 2;
       U = prim2con(M, s->W[i]);
-      double b = b_rk[r], a = a_rk[r], c = -factor * b_rk[r];
+      double b = heun ? (1.0 - r * 0.5) : b_rk[r], a = heun ? r * 0.5 : a_rk[r];
+      double c = -factor * b;
       U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
       const Flx &Uo = s->Uold[i], &R = s->Rs[i];
       U.m += Uo.m * a + R.m * c;
@@ -1204,7 +1211,7 @@ long orc_run(orc_solver* s, long nsteps, double* log) {
     for (int i = 0; i < s->nv; ++i) s->Uold[i] = prim2con(s->M, s->W[i]);
     compute_dt(s);
     if (P.liou_fix) compute_ssw(s);
-    unsigned n_rks = P.time_scheme == 0 ? 3 : 4;
+    unsigned n_rks = P.time_scheme == 0 ? 3 : (P.time_scheme == 1 ? 4 : 2);
     if (P.time_scheme == 1)
       for (auto& r : s->R2) r = Flx{0, 0, 0, 0};
     for (unsigned r = 0; r < n_rks; ++r) {

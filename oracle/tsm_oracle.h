@@ -32,7 +32,7 @@ typedef struct orc_params {
   double tvb_M;
   double ev_alpha, ev_beta;
   int nd_active, visc_active, heat_active;
-  int time_scheme;    /* 0 ssprk3, 1 rk4 */
+  int time_scheme;    /* 0 ssprk3, 1 rk4, 2 heuns (the two-stage SSP scheme) */
   int unsteady;       /* 1 unsteady, 0 steady */
   double cfl, time_step, final_time, min_residue;
   long max_iter;

@@ -102,7 +102,7 @@ class Params:
         p.reconstruct = RECONSTRUCT[self.reconstruct]
         p.tvb_M, p.ev_alpha, p.ev_beta = self.tvb_M, self.ev_alpha, self.ev_beta
         p.nd_active, p.visc_active, p.heat_active = int(self.nd_active), int(self.visc_active), int(self.heat_active)
-        p.time_scheme = {"ssprk3": 0, "rk4": 1}[self.time_scheme]
+        p.time_scheme = {"ssprk3": 0, "rk4": 1, "heuns": 2}[self.time_scheme]
         p.unsteady = 1 if self.time_mode == "unsteady" else 0
         p.cfl, p.time_step, p.final_time = self.cfl, self.time_step, self.final_time
         p.min_residue, p.max_iter, p.n_time_stamps = self.min_residue, self.max_iter, self.time_stamps

@@ -226,7 +226,6 @@ int main(int argc, char** argv) {
     }
     return 0;
   }
-  if (S.time_scheme == "heuns") { std::cerr << "ERROR --> heuns is not supported (it is broken in the reference)" << std::endl; return 1; }
   PartFile P;
   if (!read_part_file(loc + "/PARTITION/part_file_0.dat", P)) return 1;
   const int nv = (int)P.x.size();

@@ -114,8 +114,8 @@ int validate_params(const tsm_params& p) {
   if (!(p.gas_const > 0.0)) return fail(TSM_E_ARG, "gas_const must be > 0");
   if (p.flux < 0 || p.flux > 5) return fail(TSM_E_ARG, "unknown flux scheme");
   if (p.reconstruct < 0 || p.reconstruct > 6) return fail(TSM_E_ARG, "unknown reconstruction");
-  if (p.time_scheme != 0 && p.time_scheme != 1)
-    return fail(TSM_E_ARG, "time_scheme must be ssprk3 (0) or rk4 (1)");
+  if (p.time_scheme < 0 || p.time_scheme > 2)
+    return fail(TSM_E_ARG, "time_scheme must be ssprk3 (0), rk4 (1) or heuns (2)");
   if (p.unsteady != 0 && p.unsteady != 1) return fail(TSM_E_ARG, "unsteady must be 0 or 1");
   if (p.time_step == 0.0 && p.cfl == 0.0) return fail(TSM_E_ARG, "Either time_step or cfl must be non-zero");
   if (p.time_step > 0.0 && p.cfl > 0.0) return fail(TSM_E_ARG, "both time_step and cfl specified");
@@ -422,6 +422,7 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
                          params->prandtl, params->mu_model, params->visc_active,
                          params->heat_active};
   A.ns = params->model == 1;
+  A.heun = params->time_scheme == 2;
   A.ns_samples = S;
   A.ignore_done = 0;
   c->SP = tsm::StepParams{params->min_residue, final_time, params->time_step,
@@ -569,8 +570,8 @@ namespace {
 // dt_ready: the caller already ran the time-step pass (which also stores
 // U_old) and may have adjusted dt.
 void enqueue_step(tsm_ctx* c, int log_slot, bool dt_ready = false) {
-  const int nrk = c->P.time_scheme == 0 ? 3 : 4;
-  const int mode = c->P.time_scheme == 0 ? 1 : 2;
+  const int nrk = c->P.time_scheme == 0 ? 3 : (c->P.time_scheme == 1 ? 4 : 2);
+  const int mode = c->P.time_scheme == 1 ? 2 : 1;
   c->bind_state();
   if (!dt_ready) tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
   if (c->P.liou_fix) tsm::launch_ssw(c->G, c->A, c->S, c->st);
@@ -908,8 +909,8 @@ extern "C" int tsm_bench(tsm_ctx* c, int nsteps, double* us) {
     us[3] = 1e3 * acc[3] / nsteps;
   } else {
     c->csr_ready();
-    const int nrk = c->P.time_scheme == 0 ? 3 : 4;
-    const int mode = c->P.time_scheme == 0 ? 1 : 2;
+    const int nrk = c->P.time_scheme == 0 ? 3 : (c->P.time_scheme == 1 ? 4 : 2);
+    const int mode = c->P.time_scheme == 1 ? 2 : 1;
     for (int k = 0; k < nsteps; ++k) {
       c->bind_state();
       HIP_TRY(hipEventRecord(ev[0], c->st));

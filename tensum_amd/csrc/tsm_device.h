@@ -227,6 +227,7 @@ struct StageArgs {
   int ns_samples;  // number of samples
   int ignore_done; // single-call API entry points compute regardless of the loop state
   int sample;      // the sample the CSR helpers work on (set per thread block)
+  int heun;        // two-stage Heun (SSP-RK2) instead of SSP-RK3 in the SSP update
 };
 
 // LDS-staged tile kernels (Euler, SSP-RK3)

@@ -423,7 +423,10 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
         const double a_rk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
         const double b_rk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
         U = prim2con(gas, w0);
-        double b = b_rk[r], a = a_rk[r], c = -factor * b_rk[r];
+        // SSP-RK3, or Heun's method in the two-stage SSP form its coefficients
+        // describe (src/fv.cc:1675-1686 with sadd for the overwriting equ)
+        double b = A.heun ? (1.0 - r * 0.5) : b_rk[r], a = A.heun ? r * 0.5 : a_rk[r];
+        double c = -factor * b;
         U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
         U.m += Uo[0] * a + tot[0] * c;
         U.mx += Uo[1] * a + tot[1] * c;

@@ -136,6 +136,19 @@ def test_rk4_and_fixed_dt():
     assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-11
 
 
+@pytest.mark.parametrize("scheme", ["heuns", "rk4"])
+def test_heun_and_rk4_runs_match_oracle(scheme):
+    """Heun's two-stage scheme and RK4 (CSR kernels) against the oracle, logs included."""
+    m = riemann_mesh(33)
+    prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8,
+                   time_scheme=scheme)
+    s, o = _pair(m, prm, riemann(m.x, m.y))
+    _, lg = s.run(100, want_log=True)
+    _, lo = o.run(100, want_log=True)
+    assert np.allclose(lg[:, 1:], lo[:, 1:], rtol=1e-12, atol=0)
+    assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-10
+
+
 @pytest.mark.parametrize("path,flux,rec", [("auto", "kepec", "first"), ("csr", "kepec", "first"),
                                            ("auto", "kepes_tecno_roe", "minmod")])
 def test_navier_stokes_tgv(path, flux, rec):
