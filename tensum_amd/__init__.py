@@ -271,6 +271,19 @@ def host_tiling(mesh: Mesh, tile_vertices=128, cell="median", two_rings=False):
                      "two_rings", "halo2", "grad_cells", "max_grad_slots"), map(int, st)))
 
 
+def read_raw(path):
+    """(time, (n_vertex, 4) array of T, u, v, p) from a driver output file
+    written with `format raw`."""
+    with open(path, "rb") as f:
+        head = f.read(64)
+        if head[:8] != b"TSMRAW01":
+            raise TsmError(f"{path}: not a raw solution file")
+        nv, nf = np.frombuffer(head[8:24], dtype="<i8")
+        t = float(np.frombuffer(head[24:32], dtype="<f8")[0])
+        data = np.fromfile(f, dtype="<f8", count=int(nv * nf))
+    return t, data.reshape(int(nv), int(nf))
+
+
 def sobol(n, dim, seed=0, scramble=True, skip=0):
     """(n, dim) Sobol points in [0,1)^dim, Gray-code order, optionally Owen-scrambled."""
     out = np.empty((n, dim))

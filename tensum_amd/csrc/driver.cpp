@@ -14,6 +14,12 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <thread>
+#include <mutex>
+#include <functional>
+#include <deque>
+#include <condition_variable>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -169,6 +175,86 @@ void write_vtk(const std::string& file, const PartFile& P, const std::vector<dou
   vtk << "SCALARS  dual_area  double 1" << std::endl << "LOOKUP_TABLE default" << std::endl;
   for (int i = 0; i < nv; ++i) vtk << std::setprecision(12) << dcarea[i] << std::endl;
 }
+
+// Raw solution file (format raw): "TSMRAW01", int64 n_vertex, int64 n_fields
+// (4), double time, 32 bytes of field names "T u v p", then n_vertex x 4
+// little-endian doubles in part-file vertex order.
+bool write_raw(const std::string& file, const std::vector<double>& W, int nv, double time) {
+  std::ofstream f(file, std::ios::binary);
+  if (!f) return false;
+  char head[64] = {0};
+  std::memcpy(head, "TSMRAW01", 8);
+  const long long n = nv, nf = 4;
+  std::memcpy(head + 8, &n, 8);
+  std::memcpy(head + 16, &nf, 8);
+  std::memcpy(head + 24, &time, 8);
+  std::memcpy(head + 32, "T u v p", 7);
+  f.write(head, 64);
+  f.write(reinterpret_cast<const char*>(W.data()), (std::streamsize)(4 * (size_t)nv * sizeof(double)));
+  return (bool)f;
+}
+
+// Files are written on a thread of their own, in the order they are posted,
+// while the GPU carries on; a bounded queue keeps the copies held in memory
+// in check.
+class AsyncWriter {
+ public:
+  AsyncWriter() : th_([this] { loop(); }) {}
+  ~AsyncWriter() { finish(); }
+  void post(std::function<bool()> job) {
+    std::unique_lock<std::mutex> l(m_);
+    space_.wait(l, [this] { return q_.size() < 16; });
+    q_.push_back(std::move(job));
+    ready_.notify_one();
+  }
+  bool finish() {  // drains the queue; false if any write failed
+    {
+      std::lock_guard<std::mutex> l(m_);
+      done_ = true;
+      ready_.notify_one();
+    }
+    if (th_.joinable()) th_.join();
+    return ok_;
+  }
+
+ private:
+  void loop() {
+    for (;;) {
+      std::function<bool()> job;
+      {
+        std::unique_lock<std::mutex> l(m_);
+        ready_.wait(l, [this] { return done_ || !q_.empty(); });
+        if (q_.empty()) return;
+        job = std::move(q_.front());
+        q_.pop_front();
+        space_.notify_one();
+      }
+      if (!job()) ok_ = false;
+    }
+  }
+  std::mutex m_;
+  std::condition_variable ready_, space_;
+  std::deque<std::function<bool()>> q_;
+  bool done_ = false;
+  std::atomic<bool> ok_{true};
+  std::thread th_;
+};
+
+// Writer threads for many samples: a sample's files always go to the same
+// thread, so each sample's files (and master file lines) keep their order.
+class WriterPool {
+ public:
+  explicit WriterPool(int n) : w_(std::max(1, n)) {}
+  void post(int sample, std::function<bool()> job) { w_[sample % w_.size()].post(std::move(job)); }
+  bool finish() {
+    bool ok = true;
+    for (auto& w : w_) ok = w.finish() && ok;
+    return ok;
+  }
+
+ private:
+  std::deque<AsyncWriter> w_;
+};
 
 struct Moments {  // src/fv.cc:595-694 (primitive fields + density/mach/entropy)
   std::vector<std::vector<double>> fm, sm, var;  // per stamp: nv * 7
@@ -398,12 +484,20 @@ int main(int argc, char** argv) {
   const bool steady = S.time_mode == "steady";
   // a steady run lists only its final solution in the master file and keeps
   // overwriting sol0001 every write_frequency steps (src/fv.cc:2241-2245, :2318-2323, :2371-2377)
+  WriterPool writer(std::min(ns, std::max(1, std::min(8, (int)std::thread::hardware_concurrency() - 1))));
+  const bool raw = S.write_format == "raw";
   auto output = [&](int k, int counter, double t, bool master = true) {
     tsm_get_state(ctx, k, W.data());
-    tsm_gradients(ctx, k, dE.data());
-    std::string name = stamp_name("sol", counter);
-    write_vtk(dirs[k] + "/" + name + ".vtk", P, W, dE, dcarea, M, S, t);
-    if (master) std::ofstream(dirs[k] + "/master_file.visit", std::ios::app) << name << ".vtk" << std::endl;
+    if (!raw) tsm_gradients(ctx, k, dE.data());
+    const std::string name = stamp_name("sol", counter) + (raw ? ".raw" : ".vtk");
+    const std::string dir = dirs[k];
+    writer.post(k, [&P, &dcarea, &M, &S, raw, nv, dir, name, t, master, w = W,
+                 g = raw ? std::vector<double>() : dE]() {
+      const bool good = raw ? write_raw(dir + "/" + name, w, nv, t)
+                            : (write_vtk(dir + "/" + name, P, w, g, dcarea, M, S, t), true);
+      if (master) std::ofstream(dir + "/master_file.visit", std::ios::app) << name << std::endl;
+      return good;
+    });
   };
   // online statistics across samples at each stamp (Welford in sample order);
   // a steady run has two: the initial and the converged state
@@ -462,12 +556,17 @@ int main(int argc, char** argv) {
         }
         if (S.write_restart && (iter % S.restart_frequency == 0 || v[0] == S.p.final_time)) {
           tsm_get_state(ctx, k, W.data());
-          std::ofstream fo(dirs[k] + "/restart_" + std::to_string(head[k]) + "_0.dat");
-          for (int i = 0; i < nv; ++i)
-            fo << std::scientific << std::setprecision(15) << W[4 * i] << "  " << W[4 * i + 1] << "  "
-               << W[4 * i + 2] << "  " << 0.0 << "  " << W[4 * i + 3] << std::endl;
-          fo << iter << std::endl << counter << std::endl << v[0] << std::endl << 1.0 << std::endl;
-          std::ofstream(dirs[k] + "/LAST_RESTART_HEAD.txt") << head[k] << std::endl;
+          writer.post(k, [nv, dir = dirs[k], h = head[k], iter, counter, t = v[0], w = W]() {
+            std::ofstream fo(dir + "/restart_" + std::to_string(h) + "_0.dat");
+            for (int i = 0; i < nv; ++i)
+              fo << std::scientific << std::setprecision(15) << w[4 * i] << "  " << w[4 * i + 1] << "  "
+                 << w[4 * i + 2] << "  " << 0.0 << "  " << w[4 * i + 3] << std::endl;
+            fo << iter << std::endl << counter << std::endl << t << std::endl << 1.0 << std::endl;
+            const bool good = (bool)fo;
+            fo.close();
+            std::ofstream(dir + "/LAST_RESTART_HEAD.txt") << h << std::endl;
+            return good;
+          });
           head[k] = (head[k] + 1) % 2;
         }
       }
@@ -527,10 +626,23 @@ int main(int argc, char** argv) {
       std::vector<double> zero(8 * (size_t)nv, 0.0);
       tsm::Settings S0 = S;
       S0.write_variables.clear();
-      write_vtk(stamp_name("MEAN_ONLINE/mean", st) + ".vtk", P, mean, zero, dcarea, M, S0, st * S.p.final_time / S.n_time_stamps);
-      write_vtk(stamp_name("VARIANCE_ONLINE/var", st) + ".vtk", P, var, zero, dcarea, M, S0, st * S.p.final_time / S.n_time_stamps);
-      write_vtk(stamp_name("SECOND_MOMENT_ONLINE/sec_mom", st) + ".vtk", P, sm, zero, dcarea, M, S0, st * S.p.final_time / S.n_time_stamps);
+      const double ts = st * S.p.final_time / S.n_time_stamps;
+      if (raw) {
+        bool good = write_raw(stamp_name("MEAN_ONLINE/mean", st) + ".raw", mean, nv, ts) &&
+                    write_raw(stamp_name("VARIANCE_ONLINE/var", st) + ".raw", var, nv, ts) &&
+                    write_raw(stamp_name("SECOND_MOMENT_ONLINE/sec_mom", st) + ".raw", sm, nv, ts);
+        if (!good) { std::cerr << "ERROR --> cannot write the moment files" << std::endl; return 1; }
+      } else {
+        write_vtk(stamp_name("MEAN_ONLINE/mean", st) + ".vtk", P, mean, zero, dcarea, M, S0, ts);
+        write_vtk(stamp_name("VARIANCE_ONLINE/var", st) + ".vtk", P, var, zero, dcarea, M, S0, ts);
+        write_vtk(stamp_name("SECOND_MOMENT_ONLINE/sec_mom", st) + ".vtk", P, sm, zero, dcarea, M, S0, ts);
+      }
     }
+  }
+  if (!writer.finish()) {
+    std::cerr << "ERROR --> an output file could not be written" << std::endl;
+    tsm_destroy(ctx);
+    return 1;
   }
   tsm_destroy(ctx);
   return 0;

@@ -346,7 +346,9 @@ bool read_settings(const std::string& file, Settings& S, std::string& err) {
   BEGIN("output");
   ENTRY("format");
   R.word(S.write_format);
-  NEED(S.write_format == "vtk", "output format must be vtk");
+  // vtk (the reference's only format) or raw: little-endian FP64 (T,u,v,p) per
+  // vertex behind a 64-byte header, for runs whose ASCII output would dominate
+  NEED(S.write_format == "vtk" || S.write_format == "raw", "output format must be vtk or raw");
   ENTRY("frequency");
   long fr;
   NEED(R.integer(fr) && fr > 0, "frequency must be > 0");

@@ -209,3 +209,65 @@ def test_driver_uq_shocktube_example_matches_oracle(tmp_path):
         np.testing.assert_allclose(p, o.get_state()[:, 3], rtol=1e-10)
         differ.append(p)
     assert np.ptp([d.mean() for d in differ]) > 1e-3  # the samples really differ
+
+
+@pytest.mark.gpu
+def test_driver_raw_format_matches_oracle(tmp_path):
+    """format raw: FP64 states as the solver holds them, listed in the master
+    file like the VTK files; written from the driver's writer thread."""
+    mesh = vortex_mesh(17)
+    write_part_file(mesh, str(tmp_path), periodic_tags=(1, 2, 3, 4))
+    text = VORTEX_PARAM.format(final_time=1.0, stamps=2).replace("format                vtk",
+                                                                 "format                raw")
+    (tmp_path / "param.in").write_text(text)
+    r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    sd = tmp_path / "SAMPLE_0"
+    master = (sd / "master_file.visit").read_text().split()
+    assert master[2:] == [f"sol000{k}_0.raw" for k in range(3)]
+    o = O.Oracle(mesh, T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8,
+                                final_time=1.0, time_stamps=2))
+    o.set_state(vortex(mesh.x, mesh.y, box=None))
+    t0, w0 = T.read_raw(sd / "sol0000_0.raw")
+    # the driver evaluates the param.in expressions (pow, exp) itself: last-bit differences
+    assert t0 == 0.0 and np.allclose(w0, vortex(mesh.x, mesh.y, box=None), rtol=1e-14, atol=0)
+    o.run(10 ** 6)
+    t2, w2 = T.read_raw(sd / "sol0002_0.raw")
+    assert t2 == 1.0
+    assert rel_l2(w2, o.get_state()) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_driver_raw_moments_of_the_uq_example(tmp_path):
+    """use_online_stat_tool with format raw: the moment files equal the mean,
+    variance and second moment of the samples' own raw files."""
+    n = 17
+    base = T.jittered_mesh(n, seed=11)
+    retag = {4: 100001, 2: 100002, 1: 100003, 3: 100004}
+    tags = np.array([retag[int(t)] for t in base.bface_tag], dtype=np.int32)
+    ptr = np.arange(0, 2 * n + 1, 2, dtype=np.int32)
+    pv = np.array(sum([[i, (n - 1) * n + i] for i in range(n)], []), dtype=np.int32)
+    mesh = T.Mesh(base.x, base.y, tags, base.bface_v, base.cell_v, ptr, pv, {})
+    write_part_file(mesh, str(tmp_path), periodic_tags=(100003, 100004))
+    text = open(os.path.join(GOLD, "param_uq_shocktube.in")).read()
+    for a, b in (("final_time   0.2", "final_time   0.05"), ("format                vtk", "format                raw"),
+                 ("use_online_stat_tool  no", "use_online_stat_tool  yes")):
+        assert a in text
+        text = text.replace(a, b)
+    (tmp_path / "param.in").write_text(text)
+    rnd = np.random.default_rng(6).random((11, 6))
+    (tmp_path / "urnd_nos.txt").write_text(
+        "11\n" + "".join(f"{k} " + " ".join(f"{v:.17g}" for v in rnd[k]) + "\n" for k in range(11)))
+    r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for st in (1, 2):
+        x = np.stack([T.read_raw(tmp_path / f"SAMPLE_{k}" / f"sol000{st}_0.raw")[1] for k in range(11)])
+        _, mean = T.read_raw(tmp_path / "MEAN_ONLINE" / f"mean000{st}_0.raw")
+        _, var = T.read_raw(tmp_path / "VARIANCE_ONLINE" / f"var000{st}_0.raw")
+        _, sm = T.read_raw(tmp_path / "SECOND_MOMENT_ONLINE" / f"sec_mom000{st}_0.raw")
+        np.testing.assert_allclose(mean, x.mean(0), rtol=1e-13)
+        np.testing.assert_allclose(var, x.var(0, ddof=1), rtol=1e-8, atol=1e-14)
+        np.testing.assert_allclose(sm, (x * x).mean(0), rtol=1e-13)
+        assert var.max() > 1e-6
