@@ -75,6 +75,8 @@ typedef struct tsm_params {
                               instead of in a pass of their own */
   int graph_steps;    /* tiled path: capture this many steps in one HIP graph and
                          replay it (0: launch every kernel from the host) */
+  int host_preprocess; /* 1: build the dual-mesh geometry on the host; 0: on the
+                          device for median duals (the same arrays bit for bit) */
 } tsm_params;
 
 /* NPART = 1 part file content (readme/README.md:155-201), 0-based gids. */
@@ -217,6 +219,12 @@ int tsm_geometry(tsm_ctx* ctx, int* face_v, double* face_n, double* dcarea, doub
 int tsm_host_geometry(const tsm_mesh* mesh, int cell_type, int* face_v, double* face_n,
                       int* bface_v, double* bface_n, double* dcarea, double* mcarea,
                       int* n_face, int* n_bface);
+
+/* tsm_host_geometry's outputs built on the device (median duals: cell_type 0;
+   TSM_E_ARG for meshes the device builder leaves to the host). */
+int tsm_device_geometry(const tsm_mesh* mesh, int cell_type, int* face_v, double* face_n,
+                        int* bface_v, double* bface_n, double* dcarea, double* mcarea,
+                        int* n_face, int* n_bface);
 
 /* Host-only: builds and bounds-checks the vertex tiling of the tiled kernels
    (two_rings: with the second halo ring of the fused stage kernel, on meshes

@@ -40,7 +40,8 @@ class TsmParams(C.Structure):
                 ("n_time_stamps", C.c_int), ("cell_type", C.c_int), ("liou_fix", C.c_int),
                 ("kernel_path", C.c_int), ("tile_vertices", C.c_int),
                 ("block_grad", C.c_int), ("block_fu", C.c_int),
-                ("tile_fused_gradient", C.c_int), ("graph_steps", C.c_int)]
+                ("tile_fused_gradient", C.c_int), ("graph_steps", C.c_int),
+                ("host_preprocess", C.c_int)]
 
 
 class TsmMesh(C.Structure):
@@ -91,6 +92,7 @@ class Params:
     block_fu: int = 0              # threads per tile, flux/update kernel (0: auto)
     fused_gradient: bool = False   # gradients inside the stage kernel (two halo rings)
     graph_steps: int = 0           # steps per captured HIP graph (0: no graph)
+    preprocess: str = "auto"       # "auto" (device geometry for median duals) or "host"
 
     def to_c(self):
         p = TsmParams()
@@ -113,6 +115,7 @@ class Params:
         p.block_grad, p.block_fu = self.block_grad, self.block_fu
         p.tile_fused_gradient = int(self.fused_gradient)
         p.graph_steps = self.graph_steps
+        p.host_preprocess = {"auto": 0, "host": 1}[self.preprocess]
         return p
 
 
@@ -208,6 +211,7 @@ def lib():
                                         + [C.c_void_p] * 7)
         L.tsm_host_tiling.argtypes = [C.POINTER(TsmMesh), C.c_int, C.c_int, C.c_void_p]
         L.tsm_host_geometry.argtypes = [C.POINTER(TsmMesh), C.c_int] + [C.c_void_p] * 6 + [C.POINTER(C.c_int)] * 2
+        L.tsm_device_geometry.argtypes = L.tsm_host_geometry.argtypes
         _lib = L
     return _lib
 
@@ -245,19 +249,28 @@ def jittered_mesh(n, box=(0.0, 1.0, 0.0, 1.0), jitter=0.2, seed=0, periodic=Fals
 
 def host_geometry(mesh: Mesh, cell="median"):
     """Geometry of libtsm's host preprocessing (no device needed)."""
+    return _geometry(lib().tsm_host_geometry, mesh, cell)
+
+
+def device_geometry(mesh: Mesh, cell="median"):
+    """The same arrays built on the device (median duals)."""
+    return _geometry(lib().tsm_device_geometry, mesh, cell)
+
+
+def _geometry(fn, mesh, cell):
     L = lib()
     m, keep = mesh.to_c()
     nf, nb = C.c_int(), C.c_int()
-    _check(L.tsm_host_geometry(C.byref(m), {"median": 0, "voronoi": 1}[cell], None, None, None,
-                               None, None, None, C.byref(nf), C.byref(nb)))
+    _check(fn(C.byref(m), {"median": 0, "voronoi": 1}[cell], None, None, None, None, None, None,
+              C.byref(nf), C.byref(nb)))
     fv = np.empty(2 * nf.value, np.int32)
-    fn = np.empty(2 * nf.value)
+    fnv = np.empty(2 * nf.value)
     bvv = np.empty(2 * nb.value, np.int32)
     bn = np.empty(2 * nb.value)
     dc, mc = np.empty(mesh.n_vertex), np.empty(mesh.n_vertex)
-    _check(L.tsm_host_geometry(C.byref(m), {"median": 0, "voronoi": 1}[cell], _dptr(fv), _dptr(fn),
-                               _dptr(bvv), _dptr(bn), _dptr(dc), _dptr(mc), C.byref(nf), C.byref(nb)))
-    return dict(face_v=fv.reshape(-1, 2), face_n=fn.reshape(-1, 2), bface_v=bvv.reshape(-1, 2),
+    _check(fn(C.byref(m), {"median": 0, "voronoi": 1}[cell], _dptr(fv), _dptr(fnv), _dptr(bvv),
+              _dptr(bn), _dptr(dc), _dptr(mc), C.byref(nf), C.byref(nb)))
+    return dict(face_v=fv.reshape(-1, 2), face_n=fnv.reshape(-1, 2), bface_v=bvv.reshape(-1, 2),
                 bface_n=bn.reshape(-1, 2), dcarea=dc, mcarea=mc)
 
 

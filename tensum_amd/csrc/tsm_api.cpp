@@ -143,7 +143,12 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   c->P = *params;
   c->S = n_samples;
   std::string msg;
-  if (!tsm::build_geometry(*mesh, params->cell_type, c->geo, msg)) {
+  int grc = 1;  // median duals: geometry on the device unless the host is asked for
+  if (params->cell_type == 0 && !params->host_preprocess) {
+    grc = tsm::build_geometry_device(*mesh, c->geo, msg);
+    if (grc < 0) { delete c; return fail(grc, msg); }
+  }
+  if (grc == 1 && !tsm::build_geometry(*mesh, params->cell_type, c->geo, msg)) {
     delete c;
     return fail(TSM_E_MESH, msg);
   }

@@ -46,6 +46,10 @@ struct Geometry {
   std::vector<double> weight;              // 1/|P| for periodic vertices else 1
 };
 
+// Median-dual geometry built on the device (tsm_geom.hip), bit-identical to
+// build_geometry: 0 = built, 1 = left to the host builder, else TSM_E_*.
+int build_geometry_device(const tsm_mesh& m, Geometry& g, std::string& msg);
+
 // Vertex tiling for the LDS-staged kernels (see tiles.cpp).  Vertex ids in
 // every array are the new (tile-contiguous) ids.
 struct Tiling {
