@@ -124,9 +124,10 @@ struct Mat {
 };
 
 // Legacy ASCII VTK in the layout of Writer::output_vtk (src/writer.cc:129-328)
+using Attached = std::vector<std::pair<std::string, const std::vector<double>*>>;
 void write_vtk(const std::string& file, const PartFile& P, const std::vector<double>& W,
                const std::vector<double>& dE, const std::vector<double>& dcarea, const Mat& M,
-               const tsm::Settings& S, double time) {
+               const tsm::Settings& S, double time, const Attached& more = {}) {
   std::ofstream vtk(file);
   const int nv = (int)P.x.size(), nc = (int)P.cv.size() / 3;
   vtk << "# vtk DataFile Version 3.0" << std::endl << "tensum" << std::endl << "ASCII" << std::endl
@@ -174,6 +175,16 @@ void write_vtk(const std::string& file, const PartFile& P, const std::vector<dou
   }
   vtk << "SCALARS  dual_area  double 1" << std::endl << "LOOKUP_TABLE default" << std::endl;
   for (int i = 0; i < nv; ++i) vtk << std::setprecision(12) << dcarea[i] << std::endl;
+  for (auto& a : more) {  // further attached vertex data, in attach order
+    vtk << "SCALARS  " << a.first << "  double 1" << std::endl << "LOOKUP_TABLE default" << std::endl;
+    for (int i = 0; i < nv; ++i) vtk << std::setprecision(12) << (*a.second)[i] << std::endl;
+  }
+}
+
+// Vorticity from the entropy-variable gradients (src/material.h:622-627) in the
+// (f, vx, vy, l) x (x, y) layout.
+double vorticity(const Mat& M, const double* w, const double* g) {
+  return M.R * w[0] * (g[4] + w[2] * g[6] - g[3] - w[1] * g[7]);
 }
 
 // Raw solution file (format raw): "TSMRAW01", int64 n_vertex, int64 n_fields
@@ -504,10 +515,22 @@ int main(int argc, char** argv) {
   const int nst = steady ? 2 : S.n_time_stamps + 1;
   std::vector<std::vector<std::vector<double>>> snap;
   if (S.online_stat) snap.assign(nst, std::vector<std::vector<double>>(ns));
+  auto listed = [&](const char* n) {
+    return std::find(S.write_variables.begin(), S.write_variables.end(), n) != S.write_variables.end();
+  };
+  const bool has_vort = listed("vorticity");
+  std::vector<std::vector<std::vector<double>>> vsnap;  // per stamp and sample: vorticity
+  if (S.online_stat && has_vort) vsnap.assign(nst, std::vector<std::vector<double>>(ns));
   auto keep = [&](int k, int counter) {
     if (!S.online_stat) return;
     tsm_get_state(ctx, k, W.data());
     snap[counter][k] = W;
+    if (has_vort) {
+      tsm_gradients(ctx, k, dE.data());
+      std::vector<double> om(nv);
+      for (int i = 0; i < nv; ++i) om[i] = vorticity(M, &W[4 * i], &dE[8 * i]);
+      vsnap[counter][k] = std::move(om);
+    }
   };
   for (int k = 0; k < ns; ++k) {
     if (S.write_soln) output(k, 0, 0.0, !(steady && restart));
@@ -601,11 +624,33 @@ int main(int argc, char** argv) {
         ef << lab[q] << " (L1, L2, Linf): " << o[3 * q] << "   " << o[3 * q + 1] << "   " << o[3 * q + 2] << "\n";
     }
   }
-  // moments across samples, in sample order (src/fv.cc:595-694, src/reduce_moments.cc:115-141)
+  // moments across samples, in sample order (src/fv.cc:595-694, src/reduce_moments.cc:115-141):
+  // the primitive state, and density, Mach number, vorticity and entropy when
+  // listed in `variables`; their files (src/fv.cc:2027-2094) with master files,
+  // and the L1 norms of the mean and of the second moment
+  // (src/fv.cc:1884-1934, src/save_stat_norm.cc:14-100)
   if (S.online_stat) {
-    mkdirs("MEAN_ONLINE"); mkdirs("VARIANCE_ONLINE"); mkdirs("SECOND_MOMENT_ONLINE");
+    const char* dirs3[3] = {"MEAN_ONLINE", "VARIANCE_ONLINE", "SECOND_MOMENT_ONLINE"};
+    for (const char* d : dirs3) {
+      mkdirs(d);
+      std::ofstream(std::string(d) + "/master_file.visit") << "!NBLOCKS 1" << std::endl;
+    }
+    // derived fields in the writer's attach order
+    std::vector<std::string> dnames;
+    for (const char* n : {"density", "mach", "vorticity", "entropy"})
+      if (listed(n)) dnames.push_back(n);
+    auto derived = [&](const std::string& n, int st, int k, int i) {
+      const double* w = &snap[st][k][4 * (size_t)i];
+      if (n == "density") return M.rho(w);
+      if (n == "mach") return M.mach(w);
+      if (n == "entropy") return M.entropy(w);
+      return vsnap[st][k][i];
+    };
+    const size_t nd = dnames.size();
+    std::vector<std::vector<double>> l1_mean, l1_sm;  // per stamp: T u v w p, then the derived fields
     for (int st = 0; st < nst; ++st) {
       std::vector<double> mean(4 * (size_t)nv, 0.0), m2(4 * (size_t)nv, 0.0), s2(4 * (size_t)nv, 0.0);
+      std::vector<std::vector<double>> dm(nd, std::vector<double>(nv, 0.0)), dv = dm, ds = dm;
       int kk = 0;
       for (int k = 0; k < ns; ++k) {
         const auto& x = snap[st][k];
@@ -619,23 +664,83 @@ int main(int argc, char** argv) {
           s2[i] += x[i] * x[i];
           m2[i] += (x[i] - Mk1) * (x[i] - Mk);
         }
+        for (size_t q = 0; q < nd; ++q)
+          for (int i = 0; i < nv; ++i) {
+            const double val = derived(dnames[q], st, k, i);
+            if (kk == 1) { dm[q][i] = val; ds[q][i] = val * val; dv[q][i] = 0.0; continue; }
+            const double sMk1 = dm[q][i], sMk = sMk1 + (val - sMk1) / kk;
+            dm[q][i] = sMk;
+            ds[q][i] += val * val;
+            dv[q][i] += (val - sMk1) * (val - sMk);
+          }
       }
       if (kk < 2) continue;
+      const double fact3 = 1.0 / ((double)kk - 1.0), fact4 = 1.0 / (double)kk;
       std::vector<double> var(m2.size()), sm(s2.size());
-      for (size_t i = 0; i < m2.size(); ++i) { var[i] = m2[i] / (kk - 1); sm[i] = s2[i] / kk; }
+      for (size_t i = 0; i < m2.size(); ++i) { var[i] = m2[i] * fact3; sm[i] = s2[i] * fact4; }
+      for (size_t q = 0; q < nd; ++q)
+        for (int i = 0; i < nv; ++i) { dv[q][i] *= fact3; ds[q][i] *= fact4; }
+      // L1 norms: sum over vertices of dual area x |value| (one partition)
+      std::vector<double> a(5 + nd, 0.0), b(5 + nd, 0.0);
+      for (int i = 0; i < nv; ++i) {
+        const double area = dcarea[i];
+        const int col[5] = {0, 1, 2, -1, 3};
+        for (int c = 0; c < 5; ++c) {
+          a[c] += col[c] < 0 ? 0.0 : area * std::abs(mean[4 * (size_t)i + col[c]]) / 1.0;
+          b[c] += col[c] < 0 ? 0.0 : area * std::abs(sm[4 * (size_t)i + col[c]]) / 1.0;
+        }
+        for (size_t q = 0; q < nd; ++q) {
+          a[5 + q] += area * std::abs(dm[q][i]) / 1.0;
+          b[5 + q] += area * std::abs(ds[q][i]) / 1.0;
+        }
+      }
+      l1_mean.push_back(a);
+      l1_sm.push_back(b);
       std::vector<double> zero(8 * (size_t)nv, 0.0);
       tsm::Settings S0 = S;
       S0.write_variables.clear();
-      const double ts = st * S.p.final_time / S.n_time_stamps;
-      if (raw) {
-        bool good = write_raw(stamp_name("MEAN_ONLINE/mean", st) + ".raw", mean, nv, ts) &&
-                    write_raw(stamp_name("VARIANCE_ONLINE/var", st) + ".raw", var, nv, ts) &&
-                    write_raw(stamp_name("SECOND_MOMENT_ONLINE/sec_mom", st) + ".raw", sm, nv, ts);
-        if (!good) { std::cerr << "ERROR --> cannot write the moment files" << std::endl; return 1; }
-      } else {
-        write_vtk(stamp_name("MEAN_ONLINE/mean", st) + ".vtk", P, mean, zero, dcarea, M, S0, ts);
-        write_vtk(stamp_name("VARIANCE_ONLINE/var", st) + ".vtk", P, var, zero, dcarea, M, S0, ts);
-        write_vtk(stamp_name("SECOND_MOMENT_ONLINE/sec_mom", st) + ".vtk", P, sm, zero, dcarea, M, S0, ts);
+      const double ts = steady ? 0.0 : st * (S.p.final_time / S.n_time_stamps);
+      const std::string names[3] = {stamp_name("mean", st), stamp_name("var", st), stamp_name("sec_mom", st)};
+      const std::vector<double>* fields[3] = {&mean, &var, &sm};
+      const std::vector<std::vector<double>>* dfields[3] = {&dm, &dv, &ds};
+      for (int f = 0; f < 3; ++f) {
+        const std::string file = std::string(dirs3[f]) + "/" + names[f] + (raw ? ".raw" : ".vtk");
+        if (raw) {
+          if (!write_raw(file, *fields[f], nv, ts)) {
+            std::cerr << "ERROR --> cannot write " << file << std::endl;
+            return 1;
+          }
+        } else {
+          Attached more;
+          for (size_t q = 0; q < nd; ++q) more.push_back({dnames[q], &(*dfields[f])[q]});
+          write_vtk(file, P, *fields[f], zero, dcarea, M, S0, ts, more);
+        }
+        std::ofstream(std::string(dirs3[f]) + "/master_file.visit", std::ios::app)
+            << names[f] << (raw ? ".raw" : ".vtk") << std::endl;
+      }
+    }
+    // The reference names this directory but never sets its name (src/fv.h:47),
+    // so its two files go to the file system root; here they go to
+    // STAT_L1_ONLINE.  The "var" file holds the second moment's norm, as there.
+    if (!l1_mean.empty()) {
+      mkdirs("STAT_L1_ONLINE");
+      const char* fn[2] = {"STAT_L1_ONLINE/mean_l1_norm.dat", "STAT_L1_ONLINE/var_l1_norm.dat"};
+      const std::vector<std::vector<double>>* rows[2] = {&l1_mean, &l1_sm};
+      for (int f = 0; f < 2; ++f) {
+        std::ofstream o(fn[f]);
+        o << " TIME TEMPERATURE U V W PRESSURE";
+        for (const char* n : {"density", "mach", "vorticity", "entropy"})  // the reference's header order
+          if (listed(n)) o << " " << (std::string(n) == "density" ? "DENSITY" : std::string(n) == "mach" ? "MACH" : std::string(n) == "vorticity" ? "VORTICITY" : "ENTROPY");
+        o << "\n";
+        for (size_t t = 0; t < rows[f]->size(); ++t) {
+          const auto& r = (*rows[f])[t];
+          o << t;
+          for (int c = 0; c < 5; ++c) o << " " << r[c];
+          for (const char* n : {"density", "mach", "vorticity", "entropy"})
+            for (size_t q = 0; q < nd; ++q)
+              if (dnames[q] == n) o << " " << r[5 + q];
+          o << "\n";
+        }
       }
     }
   }

@@ -271,3 +271,64 @@ def test_driver_raw_moments_of_the_uq_example(tmp_path):
         np.testing.assert_allclose(var, x.var(0, ddof=1), rtol=1e-8, atol=1e-14)
         np.testing.assert_allclose(sm, (x * x).mean(0), rtol=1e-13)
         assert var.max() > 1e-6
+
+
+def _vtk_scalars(path, name, n):
+    lines = open(path).read().splitlines()
+    for tag in (f"SCALARS {name} double 1", f"SCALARS  {name}  double 1"):
+        if tag in lines:
+            i = lines.index(tag)
+            return np.array([float(v) for v in lines[i + 2:i + 2 + n]])
+    raise KeyError(name)
+
+
+@pytest.mark.gpu
+def test_driver_vtk_moments_and_l1_norms_of_the_uq_example(tmp_path):
+    """use_online_stat_tool with VTK output (src/fv.cc:2027-2094): moment files
+    with the listed derived fields (density) attached after dual_area, their
+    master files, and the L1 norms of the mean and of the second moment
+    (src/fv.cc:1884-1934) in STAT_L1_ONLINE."""
+    n = 17
+    base = T.jittered_mesh(n, seed=11)
+    retag = {4: 100001, 2: 100002, 1: 100003, 3: 100004}
+    tags = np.array([retag[int(t)] for t in base.bface_tag], dtype=np.int32)
+    ptr = np.arange(0, 2 * n + 1, 2, dtype=np.int32)
+    pv = np.array(sum([[i, (n - 1) * n + i] for i in range(n)], []), dtype=np.int32)
+    mesh = T.Mesh(base.x, base.y, tags, base.bface_v, base.cell_v, ptr, pv, {})
+    write_part_file(mesh, str(tmp_path), periodic_tags=(100003, 100004))
+    text = open(os.path.join(GOLD, "param_uq_shocktube.in")).read()
+    for a, b in (("final_time   0.2", "final_time   0.05"), ("use_online_stat_tool  no", "use_online_stat_tool  yes")):
+        text = text.replace(a, b)
+    (tmp_path / "param.in").write_text(text)
+    rnd = np.random.default_rng(7).random((11, 6))
+    (tmp_path / "urnd_nos.txt").write_text(
+        "11\n" + "".join(f"{k} " + " ".join(f"{v:.17g}" for v in rnd[k]) + "\n" for k in range(11)))
+    r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    nv = mesh.n_vertex
+    for d, stem in (("MEAN_ONLINE", "mean"), ("VARIANCE_ONLINE", "var"), ("SECOND_MOMENT_ONLINE", "sec_mom")):
+        assert (tmp_path / d / "master_file.visit").read_text().split() == \
+            ["!NBLOCKS", "1"] + [f"{stem}000{k}_0.vtk" for k in range(3)]
+    l1m = np.loadtxt(tmp_path / "STAT_L1_ONLINE" / "mean_l1_norm.dat", skiprows=1)
+    l1s = np.loadtxt(tmp_path / "STAT_L1_ONLINE" / "var_l1_norm.dat", skiprows=1)
+    assert open(tmp_path / "STAT_L1_ONLINE" / "mean_l1_norm.dat").readline().split() == \
+        ["TIME", "TEMPERATURE", "U", "V", "W", "PRESSURE", "DENSITY"]
+    assert l1m.shape == (3, 7) and l1s.shape == (3, 7)
+    area = _vtk_scalars(tmp_path / "SAMPLE_0" / "sol0000_0.vtk", "dual_area", nv)
+    for st in (1, 2):
+        rho = np.stack([_vtk_scalars(tmp_path / f"SAMPLE_{k}" / f"sol000{st}_0.vtk", "density", nv)
+                        for k in range(11)])
+        p = np.stack([_vtk_scalars(tmp_path / f"SAMPLE_{k}" / f"sol000{st}_0.vtk", "pressure", nv)
+                      for k in range(11)])
+        mf = tmp_path / "MEAN_ONLINE" / f"mean000{st}_0.vtk"
+        vf = tmp_path / "VARIANCE_ONLINE" / f"var000{st}_0.vtk"
+        sf = tmp_path / "SECOND_MOMENT_ONLINE" / f"sec_mom000{st}_0.vtk"
+        np.testing.assert_allclose(_vtk_scalars(mf, "density", nv), rho.mean(0), rtol=1e-10)
+        np.testing.assert_allclose(_vtk_scalars(vf, "density", nv), rho.var(0, ddof=1), rtol=1e-6, atol=1e-12)
+        np.testing.assert_allclose(_vtk_scalars(sf, "density", nv), (rho * rho).mean(0), rtol=1e-10)
+        np.testing.assert_allclose(_vtk_scalars(mf, "pressure", nv), p.mean(0), rtol=1e-10)
+        np.testing.assert_allclose(l1m[st, 5], (area * np.abs(p.mean(0))).sum(), rtol=1e-5)
+        np.testing.assert_allclose(l1m[st, 6], (area * np.abs(rho.mean(0))).sum(), rtol=1e-5)
+        np.testing.assert_allclose(l1s[st, 6], (area * (rho * rho).mean(0)).sum(), rtol=1e-5)
+        assert l1m[st, 4] == 0.0
