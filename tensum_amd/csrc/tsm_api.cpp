@@ -168,9 +168,8 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     return fail(TSM_E_MESH, "mesh too large for 30-bit face ids");
   }
   // ---- path and vertex numbering -----------------------------------------
-  // tiled stage kernels: SSP-RK3 without the Liou fix, Euler, or Navier-Stokes
-  // without wall cells (those run on the CSR kernels)
-  c->tiled = params->kernel_path == 0 && params->time_scheme == 0 && !params->liou_fix && !wall_cells;
+  // tiled stage kernels: SSP-RK3 without the Liou fix, Euler or Navier-Stokes
+  c->tiled = params->kernel_path == 0 && params->time_scheme == 0 && !params->liou_fix;
   std::vector<int> perm(nv), inv(nv);
   if (c->tiled) {
     int tb = params->tile_vertices > 0 ? params->tile_vertices : 128;
@@ -427,6 +426,7 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
                          params->prandtl, params->mu_model, params->visc_active,
                          params->heat_active};
   A.ns = params->model == 1;
+  A.ns_walls = wall_cells;
   A.heun = params->time_scheme == 2;
   A.ns_samples = S;
   A.ignore_done = 0;

@@ -146,6 +146,36 @@ __device__ __forceinline__ void bc_apply2(int type, const double* bs, double nx,
   }
 }
 
+// Mean entropy variables of a wall cell c (src/fv.cc:1245-1266): the one-state
+// boundary states of the ends of the cell's boundary edges (slot order, each
+// edge's ends in face order, a slip edge projected with its half dual face
+// normal), summed from zero and scaled by 0.5 / (number of edges).  W: the
+// stage's array-of-structs state, off: the sample's first vertex.
+__device__ __forceinline__ Ent wall_cell_state(const DevGeo& G, const double* W, size_t off,
+                                               int s, const Gas& gas, int c) {
+  Ent e{0.0, 0.0, 0.0, 0.0};
+  int n = 0;
+  for (int t = 0; t < 2; ++t) {
+    const int f = G.cwf[2 * c + t];
+    if (f < 0) break;
+    const int b = G.cwb[2 * c + t];
+    const int type = G.btype[b];
+    const double nx = G.fn[2 * f], ny = G.fn[2 * f + 1];
+    for (int k = 0; k < 2; ++k) {
+      const int vb = G.fv[2 * f + k];
+      const double4 q = reinterpret_cast<const double4*>(W)[off + vb];
+      Prim st{q.x, q.y, q.z, q.w};
+      bc_apply1(type, bc_values(G, s, b, bc_end(G, b, vb)), nx, ny, st);
+      const Ent x = prim2ent(gas, st);
+      e.f += x.f; e.vx += x.vx; e.vy += x.vy; e.l += x.l;
+    }
+    ++n;
+  }
+  const double sc = 0.5 * 1.0 / n;
+  e.f *= sc; e.vx *= sc; e.vy *= sc; e.l *= sc;
+  return e;
+}
+
 // Viscous flux of a cell delivered to one of its vertices (src/material.cc:
 // 158-230): s = the mean of the cell's three entropy-variable states, dE =
 // the cell gradient of the entropy variables (x, y per component), (nx, ny) =
@@ -224,6 +254,7 @@ struct StageArgs {
   FluxOpts fo;
   ViscParams vp;
   int ns;          // Navier-Stokes model active
+  int ns_walls;    // ... on a mesh with wall cells (cells with a non-periodic bface)
   int ns_samples;  // number of samples
   int ignore_done; // single-call API entry points compute regardless of the loop state
   int sample;      // the sample the CSR helpers work on (set per thread block)

@@ -250,34 +250,6 @@ __device__ __forceinline__ Cons bface_flux(const DevGeo& G, const StageArgs& A, 
   return inviscid_bflux(gas, s0, s1, nx, ny);
 }
 
-// Mean entropy variables of a wall cell (src/fv.cc:1245-1266): the one-state
-// boundary states of the ends of the cell's boundary edges (slot order, each
-// edge's ends in face order, a slip edge projected with its half dual face
-// normal), summed from zero and scaled by 0.5 / (number of edges).
-__device__ __forceinline__ Ent wall_cell_state(const DevGeo& G, const StageArgs& A, const Gas& gas,
-                                               int c) {
-  Ent s{0.0, 0.0, 0.0, 0.0};
-  int n = 0;
-  for (int t = 0; t < 2; ++t) {
-    const int f = G.cwf[2 * c + t];
-    if (f < 0) break;
-    const int b = G.cwb[2 * c + t];
-    const int type = G.btype[b];
-    const double nx = G.fn[2 * f], ny = G.fn[2 * f + 1];
-    for (int k = 0; k < 2; ++k) {
-      const int vb = G.fv[2 * f + k];
-      Prim st = load_prim(A.W_in, A.off + vb);
-      bc_apply1(type, bc_values(G, A.sample, b, bc_end(G, b, vb)), nx, ny, st);
-      const Ent e = prim2ent(gas, st);
-      s.f += e.f; s.vx += e.vx; s.vy += e.vy; s.l += e.l;
-    }
-    ++n;
-  }
-  const double sc = 0.5 * 1.0 / n;
-  s.f *= sc; s.vx *= sc; s.vy *= sc; s.l *= sc;
-  return s;
-}
-
 // Viscous flux of cell c delivered to its local vertex slot j
 // (src/fv.cc:1197-1232, src/material.cc:158-230); a wall cell uses its wall
 // state and, for a vertex on its bface b, also returns the wall flux through b
@@ -288,7 +260,7 @@ __device__ __forceinline__ Cons viscous_part(const DevGeo& G, const StageArgs& A
   int a = G.cv[3 * c], b = G.cv[3 * c + 1], d = G.cv[3 * c + 2];
   Ent s;
   if (G.cnb[c]) {
-    s = wall_cell_state(G, A, gas, c);
+    s = wall_cell_state(G, A.W_in, A.off, A.sample, gas, c);
   } else {
     Ent ea = load_ent(A.E_in, off + a), eb = load_ent(A.E_in, off + b),
         ed = load_ent(A.E_in, off + d);

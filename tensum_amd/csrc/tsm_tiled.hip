@@ -359,10 +359,12 @@ constexpr int FPT = 2;
 #define TSM_FU_WAVES 3
 #endif
 
-template <int FLUX, int REC, int LAST, bool NS>
+// NS: 0 Euler; 1 Navier-Stokes; 2 Navier-Stokes on a mesh with wall cells
+// (their terms need registers the other meshes should not pay for)
+template <int FLUX, int REC, int LAST, int NS>
 __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs A, StepParams P,
                                                  const double* Wn, double* dtpart, int r) {
-  using L = SlotLayout<FLUX, REC, NS>;
+  using L = SlotLayout<FLUX, REC, (NS != 0)>;
   extern __shared__ double lds[];
   __shared__ double wred[8 * 5];
   __shared__ int arrive;
@@ -492,10 +494,13 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
         gather_face(e, SF, MF, acc);
       }
     }
-    if (NS) {  // viscous cell fluxes after the faces, in cell order (src/fv.cc:1197-1232)
+    if (NS) {  // viscous cell fluxes after the faces: interior cells in cell order
+               // (src/fv.cc:1197-1232), then wall cells once per bface in bface order
+               // (src/fv.cc:1236-1312)
       const double fact = 1.0 / 3.0;
       for (int k = TG.cptr[v]; k < TG.cptr[v + 1]; ++k) {
         const unsigned e = TG.cent[k];
+        if (NS == 2 && (e >> 30)) continue;  // wall cell
         const int sa = e & 1023, sb = (e >> 10) & 1023, sd = (e >> 20) & 1023;
         const int j = (sa == tid) ? 0 : ((sb == tid) ? 1 : 2);
         double ea[4], eb[4], ed[4];
@@ -524,6 +529,47 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
         const Cons Fv = viscous_cell_flux(gas, A.vp, sm, dE, nx, ny);
         const double cm = -(0.5 * 1.0);
         acc[0] += Fv.m * cm; acc[1] += Fv.mx * cm; acc[2] += Fv.my * cm; acc[3] += Fv.e * cm;
+      }
+      if (NS == 2)
+      for (int q = G.vbe_ptr[v]; q < G.vbe_ptr[v + 1]; ++q) {
+        const unsigned ev = (unsigned)G.vbe[q];
+        const int b = (int)(ev & 0x7fffffffu);
+        const int cg = G.blc[b];
+        int k = TG.cptr[v];  // the cell's entry among the vertex's cells (always present)
+        const int kend = TG.cptr[v + 1] - 1;
+        while (k < kend && TG.cgid[k] != cg) ++k;
+        const unsigned e = TG.cent[k];
+        const int sa = e & 1023, sb = (e >> 10) & 1023, sd = (e >> 20) & 1023;
+        const int j = (sa == tid) ? 0 : ((sb == tid) ? 1 : 2);
+        double ea[4], eb[4], ed[4];
+#pragma unroll
+        for (int z = 0; z < 4; ++z) {
+          ea[z] = Q[(L::QE + z) * MS + sa]; eb[z] = Q[(L::QE + z) * MS + sb];
+          ed[z] = Q[(L::QE + z) * MS + sd];
+        }
+        const double xa = Q[L::QX * MS + sa], ya = Q[(L::QX + 1) * MS + sa];
+        const double xb = Q[L::QX * MS + sb], yb = Q[(L::QX + 1) * MS + sb];
+        const double xd = Q[L::QX * MS + sd], yd = Q[(L::QX + 1) * MS + sd];
+        double dE[8];
+        cell_grad(xa, ya, xb, yb, xd, yd, ea, eb, ed, dE);
+        grad_bnd_correction(G, A, gas, s, off, cg, dE);
+        const double cf = fdiv(1.0, 2.0 * G.carea[cg]);
+#pragma unroll
+        for (int z = 0; z < 8; ++z) dE[z] *= cf;
+        double nx, ny;
+        if (j == 0) { nx = (yb - yd); ny = -(xb - xd); }
+        else if (j == 1) { nx = (yd - ya); ny = -(xd - xa); }
+        else { nx = (ya - yb); ny = -(xa - xb); }
+        const Ent sw = wall_cell_state(G, A.W_in, off, s, gas, cg);
+        const Cons Fv = viscous_cell_flux(gas, A.vp, sw, dE, nx, ny);
+        const double cm = -(0.5 * 1.0);
+        acc[0] += Fv.m * cm; acc[1] += Fv.mx * cm; acc[2] += Fv.my * cm; acc[3] += Fv.e * cm;
+        if (ev >> 31) {  // the vertex is on bface b: the viscous wall flux
+          const Cons Fw = viscous_cell_flux(gas, A.vp, sw, dE, G.bn[2 * b], G.bn[2 * b + 1],
+                                            bc_adiabatic(G.btype[b]));
+          const double cw = 0.5 * 1.0;
+          acc[0] += Fw.m * cw; acc[1] += Fw.mx * cw; acc[2] += Fw.my * cw; acc[3] += Fw.e * cw;
+        }
       }
     }
   }
@@ -997,14 +1043,18 @@ void fu_launch(const DevGeo& G, const TileGeo& T, const StageArgs& A, const Step
     (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kern, grid, blk, lds, st, G, T, A, P, Wn, dtpart, r);
   };
-  if (A.ns) {
+  if (A.ns && A.ns_walls) {
     const size_t lds = fu_lds<FLUX, REC, true>(T, r == 2);
-    if (r == 2) go(k_fu_tile<FLUX, REC, 1, true>, lds);
-    else go(k_fu_tile<FLUX, REC, 0, true>, lds);
+    if (r == 2) go(k_fu_tile<FLUX, REC, 1, 2>, lds);
+    else go(k_fu_tile<FLUX, REC, 0, 2>, lds);
+  } else if (A.ns) {
+    const size_t lds = fu_lds<FLUX, REC, true>(T, r == 2);
+    if (r == 2) go(k_fu_tile<FLUX, REC, 1, 1>, lds);
+    else go(k_fu_tile<FLUX, REC, 0, 1>, lds);
   } else {
     const size_t lds = fu_lds<FLUX, REC, false>(T, r == 2);
-    if (r == 2) go(k_fu_tile<FLUX, REC, 1, false>, lds);
-    else go(k_fu_tile<FLUX, REC, 0, false>, lds);
+    if (r == 2) go(k_fu_tile<FLUX, REC, 1, 0>, lds);
+    else go(k_fu_tile<FLUX, REC, 0, 0>, lds);
   }
 }
 template <int FLUX>
