@@ -169,7 +169,7 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   }
   // ---- path and vertex numbering -----------------------------------------
   // tiled stage kernels: SSP-RK3 without the Liou fix, Euler or Navier-Stokes
-  c->tiled = params->kernel_path == 0 && params->time_scheme == 0 && !params->liou_fix;
+  c->tiled = params->kernel_path == 0 && !params->liou_fix;
   std::vector<int> perm(nv), inv(nv);
   if (c->tiled) {
     int tb = params->tile_vertices > 0 ? params->tile_vertices : 128;
@@ -179,7 +179,8 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
       return fail(TSM_E_ARG, "graph_steps must be in [0, 4096]");
     }
     const bool grad = (params->flux == 0 || params->flux == 1) && params->reconstruct != 0;
-    const bool ring2 = grad && params->tile_fused_gradient && params->model == 0;
+    const bool ring2 = grad && params->tile_fused_gradient && params->model == 0 &&
+                       params->time_scheme == 0;
     if (!tsm::build_tiles(g, tb, c->tiling, msg, ring2) || !tsm::validate_tiles(g, c->tiling, msg)) {
       delete c;
       return fail(TSM_E_MESH, "tiling: " + msg);
@@ -428,6 +429,7 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   A.ns = params->model == 1;
   A.ns_walls = wall_cells;
   A.heun = params->time_scheme == 2;
+  A.scheme = params->time_scheme;
   A.ns_samples = S;
   A.ignore_done = 0;
   c->SP = tsm::StepParams{params->min_residue, final_time, params->time_step,
@@ -598,10 +600,15 @@ void enqueue_tiled_step(tsm_ctx* c, int log_slot, hipEvent_t* ev = nullptr) {
   double* Wn = c->W[c->cur];
   double* Wa = c->W[(c->cur + 1) % 3];
   double* Wb = c->W[(c->cur + 2) % 3];
-  const double* ins[3] = {Wn, Wa, Wb};
-  double* outs[3] = {Wa, Wb, Wn};
+  // stage buffers: every stage reads W^n for U_old, the last one writes W^{n+1}
+  // over it in place; SSP-RK3 and RK4 rotate through two scratch buffers
+  const int nrk = c->P.time_scheme == 0 ? 3 : (c->P.time_scheme == 1 ? 4 : 2);
+  const double* ins[4] = {Wn, Wa, Wb, Wa};
+  double* outs[4] = {Wa, Wb, Wa, Wn};
+  if (nrk == 3) outs[2] = Wn;
+  if (nrk == 2) outs[1] = Wn;
   int e = 0;
-  for (int r = 0; r < 3; ++r) {
+  for (int r = 0; r < nrk; ++r) {
     c->A.W_in = ins[r];
     c->A.W_out = outs[r];
     if (ev) (void)hipEventRecord(ev[e++], c->st);
@@ -613,7 +620,7 @@ void enqueue_tiled_step(tsm_ctx* c, int log_slot, hipEvent_t* ev = nullptr) {
     }
     if (ev) (void)hipEventRecord(ev[e++], c->st);
     tsm::launch_tile_stage(c->P.flux, c->P.reconstruct, c->G, c->TG, c->A, c->SP, Wn, c->dtpart,
-                           c->S, r, c->st);
+                           c->S, r, r == nrk - 1, c->st);
   }
   if (ev) (void)hipEventRecord(ev[e++], c->st);
   tsm::launch_step_final_tiled(c->G, c->A, c->SP, c->S, c->TG.ntiles, c->dtpart, log_slot, c->st);
@@ -899,18 +906,19 @@ extern "C" int tsm_bench(tsm_ctx* c, int nsteps, double* us) {
   if (c->tiled) {
     c->bind_state();
     tsm::launch_dt(c->G, c->A, c->SP, c->S, c->st);
+    const int nrk = c->P.time_scheme == 0 ? 3 : (c->P.time_scheme == 1 ? 4 : 2);
     for (int k = 0; k < nsteps; ++k) {
       enqueue_tiled_step(c, -1, ev);
-      HIP_TRY(hipEventSynchronize(ev[7]));
-      for (int r = 0; r < 3; ++r) {
+      HIP_TRY(hipEventSynchronize(ev[2 * nrk + 1]));
+      for (int r = 0; r < nrk; ++r) {
         HIP_TRY(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1])); acc[1] += ms;
         HIP_TRY(hipEventElapsedTime(&ms, ev[2 * r + 1], ev[2 * r + 2])); acc[2] += ms;
       }
-      HIP_TRY(hipEventElapsedTime(&ms, ev[6], ev[7])); acc[3] += ms;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[2 * nrk], ev[2 * nrk + 1])); acc[3] += ms;
     }
     us[0] = 0.0;
-    us[1] = 1e3 * acc[1] / (3.0 * nsteps);
-    us[2] = 1e3 * acc[2] / (3.0 * nsteps);
+    us[1] = 1e3 * acc[1] / ((double)nrk * nsteps);
+    us[2] = 1e3 * acc[2] / ((double)nrk * nsteps);
     us[3] = 1e3 * acc[3] / nsteps;
   } else {
     c->csr_ready();

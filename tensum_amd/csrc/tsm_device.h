@@ -259,6 +259,7 @@ struct StageArgs {
   int ignore_done; // single-call API entry points compute regardless of the loop state
   int sample;      // the sample the CSR helpers work on (set per thread block)
   int heun;        // two-stage Heun (SSP-RK2) instead of SSP-RK3 in the SSP update
+  int scheme;      // 0 SSP-RK3, 1 RK4, 2 Heun (the tiled stage kernel's update)
 };
 
 // LDS-staged tile kernels (Euler, SSP-RK3)
@@ -275,7 +276,7 @@ void launch_tile_gradient(const DevGeo& G, const TileGeo& T, const StageArgs& A,
                           hipStream_t st);
 void launch_tile_stage(int flux, int rec, const DevGeo& G, const TileGeo& T, const StageArgs& A,
                        const StepParams& P, const double* Wn, double* dtpart, int S, int r,
-                       hipStream_t st);
+                       bool last, hipStream_t st);
 void launch_step_final_tiled(const DevGeo& G, const StageArgs& A, const StepParams& P, int S,
                              int nblk, const double* dtpart, int log_slot, hipStream_t st);
 

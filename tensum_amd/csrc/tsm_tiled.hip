@@ -604,19 +604,40 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
         part[k] = (q * q) * wt;
       }
     }
-    const double a_rk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
-    const double b_rk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
     const double factor = fdiv(A.dtl ? A.dtl[off + v] : A.ctl[s].dt, 1.0 * area);
-    const double b = b_rk[r], a = a_rk[r], c = -factor * b_rk[r];
-    Cons U = prim2con(gas, w0);
-    U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
-    if (r == 0) {  // a = 0: U_old * 0 contributes an exact zero
-      U.m += 0.0 + acc[0] * c; U.mx += 0.0 + acc[1] * c;
-      U.my += 0.0 + acc[2] * c; U.e += 0.0 + acc[3] * c;
-    } else {
+    Cons U;
+    if (A.scheme == 1) {  // RK4 (src/fv.cc:1661-1673): U_old plus a multiple of R, or of
+                          // the stages' weighted sum R2 in the last stage
+      const double rks[4] = {1.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
+      double* R2 = A.R2 + 4 * (off + v);
+      const double ir = 1.0 / rks[r];
+      double q[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double r2 = (r == 0 ? 0.0 : R2[k]) + acc[k] * ir;
+        R2[k] = r2;
+        q[k] = (r < 3) ? acc[k] : r2;
+      }
+      const double c = (r < 3) ? -factor * rks[r < 3 ? r + 1 : 3] : -factor / 6.0;
       const Cons Uo = prim2con(gas, ld_prim(Wn, off + v));
-      U.m += Uo.m * a + acc[0] * c; U.mx += Uo.mx * a + acc[1] * c;
-      U.my += Uo.my * a + acc[2] * c; U.e += Uo.e * a + acc[3] * c;
+      U.m = Uo.m * 1.0 + q[0] * c; U.mx = Uo.mx * 1.0 + q[1] * c;
+      U.my = Uo.my * 1.0 + q[2] * c; U.e = Uo.e * 1.0 + q[3] * c;
+    } else {  // SSP-RK3, or Heun's two-stage SSP form (src/fv.cc:1640-1651, :1675-1686)
+      const double a_rk[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0};
+      const double b_rk[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0};
+      const bool heun = A.scheme == 2;
+      const double b = heun ? (1.0 - r * 0.5) : b_rk[r], a = heun ? r * 0.5 : a_rk[r];
+      const double c = -factor * b;
+      U = prim2con(gas, w0);
+      U.m *= b; U.mx *= b; U.my *= b; U.e *= b;
+      if (r == 0) {  // a = 0: U_old * 0 contributes an exact zero
+        U.m += 0.0 + acc[0] * c; U.mx += 0.0 + acc[1] * c;
+        U.my += 0.0 + acc[2] * c; U.e += 0.0 + acc[3] * c;
+      } else {
+        const Cons Uo = prim2con(gas, ld_prim(Wn, off + v));
+        U.m += Uo.m * a + acc[0] * c; U.mx += Uo.mx * a + acc[1] * c;
+        U.my += Uo.my * a + acc[2] * c; U.e += Uo.e * a + acc[3] * c;
+      }
     }
     w1 = con2prim(gas, U);
     reinterpret_cast<double4*>(A.W_out)[off + v] = make_double4(w1.T, w1.u, w1.v, w1.p);
@@ -1022,12 +1043,12 @@ size_t fused_lds(const TileGeo& T) {
 }
 template <int FLUX, int REC>
 void fu_launch(const DevGeo& G, const TileGeo& T, const StageArgs& A, const StepParams& P,
-               const double* Wn, double* dtpart, int S, int r, hipStream_t st) {
+               const double* Wn, double* dtpart, int S, int r, bool last, hipStream_t st) {
   constexpr bool GRAD = SlotLayout<FLUX, REC>::GRAD;
-  if (GRAD && T.fused) {
+  if (GRAD && T.fused) {  // SSP-RK3 only (the host builds two-ring tiles for it alone)
     const size_t lds = fused_lds<FLUX, REC>(T);
     dim3 grid(T.ntiles, S), blk(tile_fu_block(T));
-    if (r == 2) {
+    if (last) {
       auto k = k_stage_fused<FLUX, REC, 1>;
       (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL(k, grid, blk, lds, st, G, T, A, P, Wn, dtpart, r);
@@ -1044,30 +1065,30 @@ void fu_launch(const DevGeo& G, const TileGeo& T, const StageArgs& A, const Step
     hipLaunchKernelGGL(kern, grid, blk, lds, st, G, T, A, P, Wn, dtpart, r);
   };
   if (A.ns && A.ns_walls) {
-    const size_t lds = fu_lds<FLUX, REC, true>(T, r == 2);
-    if (r == 2) go(k_fu_tile<FLUX, REC, 1, 2>, lds);
+    const size_t lds = fu_lds<FLUX, REC, true>(T, last);
+    if (last) go(k_fu_tile<FLUX, REC, 1, 2>, lds);
     else go(k_fu_tile<FLUX, REC, 0, 2>, lds);
   } else if (A.ns) {
-    const size_t lds = fu_lds<FLUX, REC, true>(T, r == 2);
-    if (r == 2) go(k_fu_tile<FLUX, REC, 1, 1>, lds);
+    const size_t lds = fu_lds<FLUX, REC, true>(T, last);
+    if (last) go(k_fu_tile<FLUX, REC, 1, 1>, lds);
     else go(k_fu_tile<FLUX, REC, 0, 1>, lds);
   } else {
-    const size_t lds = fu_lds<FLUX, REC, false>(T, r == 2);
-    if (r == 2) go(k_fu_tile<FLUX, REC, 1, 0>, lds);
+    const size_t lds = fu_lds<FLUX, REC, false>(T, last);
+    if (last) go(k_fu_tile<FLUX, REC, 1, 0>, lds);
     else go(k_fu_tile<FLUX, REC, 0, 0>, lds);
   }
 }
 template <int FLUX>
 void fu_rec(int rec, const DevGeo& G, const TileGeo& T, const StageArgs& A, const StepParams& P,
-            const double* Wn, double* dtpart, int S, int r, hipStream_t st) {
+            const double* Wn, double* dtpart, int S, int r, bool last, hipStream_t st) {
   switch (rec) {
-    case R_FIRST: fu_launch<FLUX, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, st); break;
-    case R_SECOND: fu_launch<FLUX, R_SECOND>(G, T, A, P, Wn, dtpart, S, r, st); break;
-    case R_MUSCL: fu_launch<FLUX, R_MUSCL>(G, T, A, P, Wn, dtpart, S, r, st); break;
-    case R_ALBADA: fu_launch<FLUX, R_ALBADA>(G, T, A, P, Wn, dtpart, S, r, st); break;
-    case R_MINMOD: fu_launch<FLUX, R_MINMOD>(G, T, A, P, Wn, dtpart, S, r, st); break;
-    case R_TVB: fu_launch<FLUX, R_TVB>(G, T, A, P, Wn, dtpart, S, r, st); break;
-    default: fu_launch<FLUX, R_ENO2>(G, T, A, P, Wn, dtpart, S, r, st); break;
+    case R_FIRST: fu_launch<FLUX, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, last, st); break;
+    case R_SECOND: fu_launch<FLUX, R_SECOND>(G, T, A, P, Wn, dtpart, S, r, last, st); break;
+    case R_MUSCL: fu_launch<FLUX, R_MUSCL>(G, T, A, P, Wn, dtpart, S, r, last, st); break;
+    case R_ALBADA: fu_launch<FLUX, R_ALBADA>(G, T, A, P, Wn, dtpart, S, r, last, st); break;
+    case R_MINMOD: fu_launch<FLUX, R_MINMOD>(G, T, A, P, Wn, dtpart, S, r, last, st); break;
+    case R_TVB: fu_launch<FLUX, R_TVB>(G, T, A, P, Wn, dtpart, S, r, last, st); break;
+    default: fu_launch<FLUX, R_ENO2>(G, T, A, P, Wn, dtpart, S, r, last, st); break;
   }
 }
 }  // namespace
@@ -1129,14 +1150,14 @@ void launch_tile_gradient_bnd(const DevGeo& G, const TileGeo& T, const StageArgs
 
 void launch_tile_stage(int flux, int rec, const DevGeo& G, const TileGeo& T, const StageArgs& A,
                        const StepParams& P, const double* Wn, double* dtpart, int S, int r,
-                       hipStream_t st) {
+                       bool last, hipStream_t st) {
   switch (flux) {
-    case F_ROE: fu_rec<F_ROE>(rec, G, T, A, P, Wn, dtpart, S, r, st); break;
-    case F_RUS: fu_rec<F_RUS>(rec, G, T, A, P, Wn, dtpart, S, r, st); break;
-    case F_KEPEC: fu_launch<F_KEPEC, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, st); break;
-    case F_ROEEC: fu_launch<F_ROEEC, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, st); break;
-    case F_KEP: fu_launch<F_KEP, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, st); break;
-    default: fu_launch<F_AVG, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, st); break;
+    case F_ROE: fu_rec<F_ROE>(rec, G, T, A, P, Wn, dtpart, S, r, last, st); break;
+    case F_RUS: fu_rec<F_RUS>(rec, G, T, A, P, Wn, dtpart, S, r, last, st); break;
+    case F_KEPEC: fu_launch<F_KEPEC, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, last, st); break;
+    case F_ROEEC: fu_launch<F_ROEEC, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, last, st); break;
+    case F_KEP: fu_launch<F_KEP, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, last, st); break;
+    default: fu_launch<F_AVG, R_FIRST>(G, T, A, P, Wn, dtpart, S, r, last, st); break;
   }
 }
 

@@ -136,17 +136,34 @@ def test_rk4_and_fixed_dt():
     assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-11
 
 
+@pytest.mark.parametrize("path", ["auto", "csr"])
 @pytest.mark.parametrize("scheme", ["heuns", "rk4"])
-def test_heun_and_rk4_runs_match_oracle(scheme):
-    """Heun's two-stage scheme and RK4 (CSR kernels) against the oracle, logs included."""
+def test_heun_and_rk4_runs_match_oracle(scheme, path):
+    """Heun's two-stage scheme and RK4 (tiled and CSR kernels) against the
+    oracle, logs included."""
     m = riemann_mesh(33)
     prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8,
-                   time_scheme=scheme)
+                   time_scheme=scheme, kernel_path=path)
     s, o = _pair(m, prm, riemann(m.x, m.y))
     _, lg = s.run(100, want_log=True)
     _, lo = o.run(100, want_log=True)
     assert np.allclose(lg[:, 1:], lo[:, 1:], rtol=1e-12, atol=0)
     assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-10
+
+
+@pytest.mark.parametrize("graph", [0, 3])
+def test_rk4_periodic_vortex_tiled_matches_oracle(graph):
+    """RK4 with KEPEC on the periodic vortex (the scheme and flux of the
+    reference's isentropic-vortex example) on the tiled kernels, eager and
+    replayed from HIP graphs."""
+    m = vortex_mesh(33)
+    prm = T.Params(flux="kepec", reconstruct="first", cfl=0.4, final_time=10.0, time_scheme="rk4",
+                   graph_steps=graph)
+    s, o = _pair(m, prm, vortex(m.x, m.y))
+    for chunk in (7, 13):
+        s.run(chunk)
+        o.run(chunk)
+        assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-10
 
 
 @pytest.mark.parametrize("path,flux,rec", [("auto", "kepec", "first"), ("csr", "kepec", "first"),
