@@ -129,6 +129,16 @@ int tsm_set_bc_state(tsm_ctx* ctx, int sample, const double* state);
 int tsm_set_state(tsm_ctx* ctx, int sample, const double* prim);
 int tsm_get_state(tsm_ctx* ctx, int sample, double* prim);
 
+/* Force integral on the current state (FiniteVolume::compute_forces,
+   src/force.cc:118-217): over the boundary faces whose tag is one of tags[ntags]
+   (in bface order), out[0..1] = sum 0.5 (p0 + p1) n_b and, for Navier-Stokes,
+   out[2..3] = -sum tau n_b with tau from the mean of the end points' velocity
+   gradients (vertex gradients of the current state).  force_index is the
+   force's position in the param.in list: as the reference does, the end-point
+   states get the boundary condition of bface number force_index. */
+int tsm_forces(tsm_ctx* ctx, int sample, int force_index, const int* tags, int ntags,
+               double* out);
+
 /* Residual of the current state (FiniteVolume::compute_residual, src/fv.cc:1467). R: 4*n_vertex. */
 int tsm_residual(tsm_ctx* ctx, int sample, double* R);
 /* Vertex entropy-variable gradients (FiniteVolume::compute_gradients, src/fv.cc:699).

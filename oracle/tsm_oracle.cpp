@@ -1240,6 +1240,54 @@ long orc_run(orc_solver* s, long nsteps, double* log) {
 }
 double orc_time(const orc_solver* s) { return s->elapsed; }
 
+// src/force.cc:118-217: pressure and viscous forces on the boundary faces whose
+// tag is in tags, in bface order.  The reference takes the boundary condition
+// of bface[force_index] (its loop index) rather than of the face summed; kept.
+void orc_forces(orc_solver* s, int force_index, const int* tags, int ntags, double* out) {
+  const Mat& M = s->M;
+  if (M.p.model == 1) compute_gradients(s);
+  const Face& bi = s->bface[force_index];
+  const BC& b = s->bcof(bi.type);
+  V2 inv{0.0, 0.0}, vis{0.0, 0.0};
+  for (size_t f = 0; f < s->bface.size(); ++f) {
+    const Face& bf = s->bface[f];
+    bool in = false;
+    for (int k = 0; k < ntags; ++k) in = in || tags[k] == bf.type;
+    if (!in) continue;
+    const int v0 = bf.v[0], v1 = bf.v[1];
+    Prim st[2] = {s->W[v0], s->W[v1]};
+    bc_apply1(b, s->bstate_at(force_index, v0), bi, st[0]);
+    bc_apply1(b, s->bstate_at(force_index, v1), bi, st[1]);
+    const Ent e0 = prim2ent(M, st[0]), e1 = prim2ent(M, st[1]);
+    const double p = 0.5 * (st[0].p + st[1].p);
+    inv.x += bf.normal.x * p;
+    inv.y += bf.normal.y * p;
+    if (M.p.model == 1) {
+      const double T = (st[0].T + st[1].T) / 2.0;
+      const double mu = M.viscosity(T);
+      const double f0 = 1.0 / e0.l, f1 = 1.0 / e1.l;
+      const V2* g0 = &s->dE[(size_t)v0 * 4];
+      const V2* g1 = &s->dE[(size_t)v1 * 4];
+      // gradU_j = -f dE[vx] + f (vx/l) dE[l], then the mean of the two ends
+      const double a0 = f0 * e0.vx / e0.l, a1 = f1 * e1.vx / e1.l;
+      const double b0 = f0 * e0.vy / e0.l, b1 = f1 * e1.vy / e1.l;
+      V2 gu0{g0[1].x * -f0 + g0[3].x * a0, g0[1].y * -f0 + g0[3].y * a0};
+      V2 gu1{g1[1].x * -f1 + g1[3].x * a1, g1[1].y * -f1 + g1[3].y * a1};
+      V2 gv0{g0[2].x * -f0 + g0[3].x * b0, g0[2].y * -f0 + g0[3].y * b0};
+      V2 gv1{g1[2].x * -f1 + g1[3].x * b1, g1[2].y * -f1 + g1[3].y * b1};
+      V2 gu{gu0.x * 0.5 + gu1.x * 0.5, gu0.y * 0.5 + gu1.y * 0.5};
+      V2 gv{gv0.x * 0.5 + gv1.x * 0.5, gv0.y * 0.5 + gv1.y * 0.5};
+      const double div = gu.x + gv.y;
+      const double sxx = 2.0 * mu * (gu.x - (1.0 / 3.0) * div);
+      const double syy = 2.0 * mu * (gv.y - (1.0 / 3.0) * div);
+      const double sxy = mu * (gu.y + gv.x);
+      vis.x += -(sxx * bf.normal.x + sxy * bf.normal.y);
+      vis.y += -(sxy * bf.normal.x + syy * bf.normal.y);
+    }
+  }
+  out[0] = inv.x; out[1] = inv.y; out[2] = vis.x; out[3] = vis.y;
+}
+
 int orc_set_bc_state(orc_solver* s, const int* bface_v, const double* state) {
   // state: per input bface b and end point k (bface_v[2b+k]) the values (T,u,v,p)
   std::map<std::pair<int, int>, int> in;

@@ -92,6 +92,9 @@ void orc_local_dt(const orc_solver* s, double* dt);
    done.  log (optional) receives 4 doubles per step: iter, dt, t, res_total */
 long orc_run(orc_solver* s, long nsteps, double* log);
 double orc_time(const orc_solver* s);
+/* pressure (out[0..1]) and viscous (out[2..3]) force on the boundary faces with
+   one of the tags, current state (src/force.cc:118-217) */
+void orc_forces(orc_solver* s, int force_index, const int* tags, int ntags, double* out);
 /* boundary states of this run per input bface b and end point k (the vertex
    bface_v[2b+k]): state[8b+4k..] = (T,u,v,p), the boundary expressions
    evaluated there; replaces the tags' constant states.  0 or -1. */

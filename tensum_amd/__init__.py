@@ -193,6 +193,7 @@ def lib():
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.tsm_residual_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_set_bc_state.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.tsm_forces.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_error_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.tsm_global.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tsm_geometry_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -399,6 +400,14 @@ class Solver:
         if a.size != 8 * self.n_bface:
             raise TsmError(f"expected {8 * self.n_bface} boundary values, got {a.size}")
         _check(lib().tsm_set_bc_state(self._h, sample, _dptr(a)))
+
+    def forces(self, tags, sample=0, force_index=0):
+        """(pressure x, y, viscous x, y) force on the boundary faces with these tags."""
+        t = np.ascontiguousarray(tags, dtype=np.int32)
+        out = np.zeros(4)
+        _check(lib().tsm_forces(self._h, sample, int(force_index), t.ctypes.data_as(C.c_void_p),
+                                t.size, _dptr(out)))
+        return out
 
     def residual_norms(self, sample=0):
         """Last step's residual norms (mass, x/y momentum, energy) before

@@ -446,7 +446,14 @@ int main(int argc, char** argv) {
   std::vector<double> dcarea(nv), mcarea(nv);
   tsm_geometry(ctx, nullptr, nullptr, dcarea.data(), mcarea.data());
   std::vector<std::string> dirs(ns);
-  std::vector<std::ofstream> res_f(ns), glob_f(ns);
+  std::vector<std::ofstream> res_f(ns), glob_f(ns), force_f(ns);
+  // every force's tags must name boundary faces of the mesh (src/force.cc:25-106)
+  for (auto& fo : S.forces)
+    for (int t : fo.second)
+      if (std::find(P.btag.begin(), P.btag.end(), t) == P.btag.end()) {
+        std::cerr << "Force: " << fo.first << " has face type " << t << " but it is not present in the grid" << std::endl;
+        return 1;
+      }
   std::vector<int> head(ns, 0);
   std::vector<double> W(4 * (size_t)nv), dE(8 * (size_t)nv);
   for (int k = 0; k < ns; ++k) {
@@ -490,6 +497,7 @@ int main(int argc, char** argv) {
     if (tsm_set_state(ctx, k, W.data()) != TSM_OK) { std::cerr << "ERROR --> " << tsm_last_error() << std::endl; return 1; }
     if (!restart) std::ofstream(dirs[k] + "/master_file.visit") << "!NBLOCKS 1" << std::endl;
     if (S.write_log) res_f[k].open(dirs[k] + "/residual.dat", std::ios::app);
+    if (!S.forces.empty()) force_f[k].open(dirs[k] + "/force.dat", std::ios::app);
     if (S.has_global) glob_f[k].open(dirs[k] + "/global.dat", std::ios::app);
   }
   const bool steady = S.time_mode == "steady";
@@ -563,6 +571,20 @@ int main(int argc, char** argv) {
         } else if (S.write_log) {
           res_f[k] << std::setw(10) << (long)L[0] << "  " << std::scientific << std::setprecision(12)
                    << L[1] << "  " << L[2] << "  " << L[3] << std::endl;
+        }
+        if (!S.forces.empty()) {  // src/force.cc:118-217, after each step
+          std::ofstream& ff = force_f[k];
+          ff << std::setw(6) << (long)L[0] << " " << std::scientific << std::setw(15);
+          ff << L[2] << std::setw(15);
+          for (size_t i = 0; i < S.forces.size(); ++i) {
+            double fo[4];
+            const auto& tags = S.forces[i].second;
+            if (tsm_forces(ctx, k, (int)i, tags.data(), (int)tags.size(), fo) != TSM_OK) {
+              std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
+              return 1;
+            }
+            ff << fo[0] << "  " << fo[1] << "  " << fo[2] << "  " << fo[3] << std::endl;
+          }
         }
         if (S.has_global) {
           double ke, ent;

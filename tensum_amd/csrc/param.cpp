@@ -329,16 +329,22 @@ bool read_settings(const std::string& file, Settings& S, std::string& err) {
     while (!R.eos()) R.word(w);
   }
 
-  // integrals (src/parameter.cc:612-637): force integrals are not computed
+  // integrals (src/parameter.cc:630-653): force blocks, a name and boundary tags
   BEGIN("integrals");
   while (!R.eos()) {
     R.word(w);
     if (w == "force") {
       ENTRY("{");
-      R.word(w);
-      while (!R.eos()) R.word(w);
-      err = "force integrals are not supported";
-      return false;
+      std::string name;
+      R.word(name);
+      std::vector<int> tags;
+      while (!R.eos()) {
+        long t;
+        NEED(R.integer(t), "bad boundary tag in force " + name);
+        tags.push_back((int)t);
+      }
+      NEED(!tags.empty(), "force " + name + " has no boundary tags");
+      S.forces.push_back({name, tags});
     }
   }
 

@@ -38,6 +38,8 @@ struct Settings {
   int n_time_stamps = 1;
   bool online_stat = false;
   std::vector<std::string> write_variables;
+  // integrals/force blocks: name and boundary tags (src/parameter.cc:630-653)
+  std::vector<std::pair<std::string, std::vector<int>>> forces;
   std::vector<int> write_surfaces;
   bool write_restart = false;
   unsigned restart_frequency = 0;

@@ -52,6 +52,9 @@ struct tsm_ctx {
   // input bface b, end point k (bface_v[2b+k]) -> offset of its boundary
   // values in a sample's block of G.bstate; device boundary codes
   std::vector<long long> bmap;
+  double* force_buf = nullptr;  // per-face force terms and the sum (tsm_forces)
+  int* force_sel = nullptr;
+  size_t force_cap = 0;
   std::vector<int> h_btype;
   double* red = nullptr;   // reduction partials for global / error norms
   size_t red_cap = 0;
@@ -543,6 +546,38 @@ extern "C" int tsm_gradients(tsm_ctx* c, int s, double* dE) {
   c->A.ignore_done = 0;
   HIP_TRY(hipGetLastError());
   return download_vec(c, c->A.dE, 8, s, dE);
+}
+
+extern "C" int tsm_forces(tsm_ctx* c, int s, int force_index, const int* tags, int ntags,
+                          double* out) {
+  const int nb = c ? c->G.nb : 0;
+  if (!c || !out || s < 0 || s >= c->S || ntags < 1 || !tags || force_index < 0 ||
+      force_index >= nb)
+    return fail(TSM_E_ARG, "bad argument");
+  std::vector<int> sel;
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < ntags; ++k)
+      if (c->geo.bface[b].tag == tags[k]) { sel.push_back(b); break; }
+  if (sel.empty()) return fail(TSM_E_ARG, "no boundary face has the force's tags");
+  if (c->force_cap < sel.size()) {
+    int rc = c->alloc(&c->force_buf, 4 * sel.size() + 4);
+    if (rc) return rc;
+    rc = c->alloc(&c->force_sel, sel.size());
+    if (rc) return rc;
+    c->force_cap = sel.size();
+  }
+  c->csr_ready();
+  HIP_TRY(hipMemcpyAsync(c->force_sel, sel.data(), sel.size() * sizeof(int), hipMemcpyHostToDevice,
+                         c->st));
+  c->A.ignore_done = 1;
+  if (c->P.model == 1) tsm::launch_gradient(c->G, c->A, c->S, c->st);
+  c->A.ignore_done = 0;
+  tsm::launch_forces(c->G, c->A, s, force_index, c->force_sel, (int)sel.size(), c->P.model == 1,
+                     c->force_buf + 4, c->force_buf, c->st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, c->force_buf, 4 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  return TSM_OK;
 }
 
 extern "C" int tsm_residual(tsm_ctx* c, int s, double* R) {

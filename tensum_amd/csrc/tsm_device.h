@@ -280,6 +280,8 @@ void launch_tile_stage(int flux, int rec, const DevGeo& G, const TileGeo& T, con
 void launch_step_final_tiled(const DevGeo& G, const StageArgs& A, const StepParams& P, int S,
                              int nblk, const double* dtpart, int log_slot, hipStream_t st);
 
+void launch_forces(const DevGeo& G, const StageArgs& A, int s, int bq, const int* sel, int nsel,
+                   int ns_model, double* terms, double* out, hipStream_t st);
 void launch_flux_update(int flux, int rec, const DevGeo& G, const StageArgs& A, int S, int r,
                         int last, int mode, hipStream_t st, bool do_flux, bool do_update);
 void launch_gradient(const DevGeo& G, const StageArgs& A, int S, hipStream_t st);

@@ -576,6 +576,74 @@ __global__ __launch_bounds__(256) void k_step_final(StageArgs A, StepParams P, i
   }
 }
 
+// Forces on a set of boundary faces (src/force.cc:118-217): per selected bface
+// (ascending), the pressure force 0.5 (p0 + p1) n_b and, for Navier-Stokes, the
+// viscous stress of the mean of the end points' velocity gradients (from the
+// vertex gradients of the entropy variables) against n_b.  The boundary
+// condition applied to the end-point states is that of bface bq (the
+// reference's loop index).  One block: the lanes form the per-face terms, lane 0
+// adds them in bface order.
+__global__ __launch_bounds__(256) void k_forces(DevGeo G, StageArgs A, int s, int bq,
+                                                const int* sel, int nsel, int ns_model,
+                                                double* terms, double* out) {
+  const size_t off = (size_t)s * G.nv;
+  const Gas gas{A.gamma[s], A.gasR[s]};
+  const int type = G.btype[bq];
+  const double qx = G.bn[2 * bq], qy = G.bn[2 * bq + 1];
+  for (int j = threadIdx.x; j < nsel; j += blockDim.x) {
+    const int b = sel[j];
+    const int v0 = G.bv[2 * b], v1 = G.bv[2 * b + 1];
+    const double nx = G.bn[2 * b], ny = G.bn[2 * b + 1];
+    Prim s0 = load_prim(A.W_in, off + v0), s1 = load_prim(A.W_in, off + v1);
+    bc_apply1(type, bc_values(G, s, bq, bc_end(G, bq, v0)), qx, qy, s0);
+    bc_apply1(type, bc_values(G, s, bq, bc_end(G, bq, v1)), qx, qy, s1);
+    const double p = 0.5 * (s0.p + s1.p);
+    double* t = terms + 4 * (size_t)j;
+    t[0] = nx * p;
+    t[1] = ny * p;
+    t[2] = t[3] = 0.0;
+    if (ns_model) {
+      const Ent e0 = prim2ent(gas, s0), e1 = prim2ent(gas, s1);
+      const double T = (s0.T + s1.T) / 2.0;
+      double mu;
+      if (A.vp.mu_model == 0) mu = A.vp.mu_ref;
+      else if (A.vp.mu_model == 1) mu = fdiv(A.vp.mu_ref * pow(T, 1.5), T + A.vp.T_0);
+      else mu = A.vp.mu_ref * pow(fdiv(T, A.vp.T_ref), A.vp.omega);
+      const double f0 = fdiv(1.0, e0.l), f1 = fdiv(1.0, e1.l);
+      const double* g0 = A.dE + 8 * (off + v0);
+      const double* g1 = A.dE + 8 * (off + v1);
+      const double a0 = fdiv(f0 * e0.vx, e0.l), a1 = fdiv(f1 * e1.vx, e1.l);
+      const double b0 = fdiv(f0 * e0.vy, e0.l), b1 = fdiv(f1 * e1.vy, e1.l);
+      // (f, vx, vy, l) x (x, y): vx at 2-3, vy at 4-5, l at 6-7
+      const double gu0x = g0[2] * -f0 + g0[6] * a0, gu0y = g0[3] * -f0 + g0[7] * a0;
+      const double gu1x = g1[2] * -f1 + g1[6] * a1, gu1y = g1[3] * -f1 + g1[7] * a1;
+      const double gv0x = g0[4] * -f0 + g0[6] * b0, gv0y = g0[5] * -f0 + g0[7] * b0;
+      const double gv1x = g1[4] * -f1 + g1[6] * b1, gv1y = g1[5] * -f1 + g1[7] * b1;
+      const double gux = gu0x * 0.5 + gu1x * 0.5, guy = gu0y * 0.5 + gu1y * 0.5;
+      const double gvx = gv0x * 0.5 + gv1x * 0.5, gvy = gv0y * 0.5 + gv1y * 0.5;
+      const double div = gux + gvy;
+      const double sxx = 2.0 * mu * (gux - (1.0 / 3.0) * div);
+      const double syy = 2.0 * mu * (gvy - (1.0 / 3.0) * div);
+      const double sxy = mu * (guy + gvx);
+      t[2] = -(sxx * nx + sxy * ny);
+      t[3] = -(sxy * nx + syy * ny);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double f[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < nsel; ++j)
+      for (int k = 0; k < 4; ++k) f[k] += terms[4 * (size_t)j + k];
+    for (int k = 0; k < 4; ++k) out[k] = f[k];
+  }
+}
+
+void launch_forces(const DevGeo& G, const StageArgs& A, int s, int bq, const int* sel, int nsel,
+                   int ns_model, double* terms, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, st, G, A, s, bq, sel, nsel, ns_model, terms,
+                     out);
+}
+
 // Liou carbuncle sensor (src/compute_ssw.cc:7-39)
 __global__ void k_ssw_zero(DevGeo G, StageArgs A) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

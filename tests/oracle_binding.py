@@ -54,6 +54,7 @@ def lib():
         L.orc_compute_dt.argtypes = [C.c_void_p]
         L.orc_local_dt.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_set_bc_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_forces.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_run.restype = C.c_long
         L.orc_run.argtypes = [C.c_void_p, C.c_long, C.c_void_p]
         L.orc_time.restype = C.c_double
@@ -144,6 +145,12 @@ class Oracle:
         bv = np.ascontiguousarray(self._bface_v, dtype=np.int32)
         if lib().orc_set_bc_state(self._h, _p(bv), _p(a)) != 0:
             raise RuntimeError("oracle: " + lib().orc_error().decode())
+
+    def forces(self, tags, force_index=0):
+        t = np.ascontiguousarray(tags, dtype=np.int32)
+        out = np.empty(4)
+        lib().orc_forces(self._h, int(force_index), _p(t), t.size, _p(out))
+        return out
 
     def local_dt(self):
         a = np.empty(self.nv)

@@ -332,3 +332,59 @@ def test_driver_vtk_moments_and_l1_norms_of_the_uq_example(tmp_path):
         np.testing.assert_allclose(l1m[st, 6], (area * np.abs(rho.mean(0))).sum(), rtol=1e-5)
         np.testing.assert_allclose(l1s[st, 6], (area * (rho * rho).mean(0)).sum(), rtol=1e-5)
         assert l1m[st, 4] == 0.0
+
+
+CHANNEL_BC = """boundary
+{
+   1 3
+   {
+      type         slip
+   }
+   4
+   {
+      type         inlet
+      temperature  1.0
+      xvelocity    0.5
+      yvelocity    0.0
+      zvelocity    0.0
+      pressure     1.0
+   }
+   2
+   {
+      type         outlet
+   }
+}"""
+
+
+@pytest.mark.gpu
+def test_driver_force_file_matches_oracle(tmp_path):
+    """integrals/force blocks: force.dat after every step (src/force.cc:118-217),
+    one line per force, the first with the step and time."""
+    import re
+    mesh = T.jittered_mesh(17, seed=13)
+    write_part_file(mesh, str(tmp_path))
+    text = VORTEX_PARAM.format(final_time=0.05, stamps=1)
+    text = re.sub(r"boundary\n\{.*?\n\}\nexact_soln", CHANNEL_BC + "\nexact_soln", text, flags=re.S)
+    text = text.replace("integrals\n{\n}", "integrals\n{\n   force\n   {\n      walls\n      1\n      3\n   }\n"
+                        "   force\n   {\n      inflow\n      4\n   }\n}")
+    text = text.replace("find_error            yes", "find_error            no")
+    assert "force" in text and "type         inlet" in text
+    (tmp_path / "param.in").write_text(text)
+    r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = (tmp_path / "SAMPLE_0" / "force.dat").read_text().splitlines()
+    bc = {1: ("slip",), 3: ("slip",), 4: ("inlet", (1.0, 0.5, 0.0, 1.0)), 2: ("outlet",)}
+    om = T.Mesh(mesh.x, mesh.y, mesh.bface_tag, mesh.bface_v, mesh.cell_v, mesh.periodic_ptr,
+                mesh.periodic_v, bc)
+    o = O.Oracle(om, T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8, final_time=0.05))
+    o.set_state(vortex(mesh.x, mesh.y, box=None))
+    n = 0
+    while o.run(1) == 1:
+        a, b = lines[2 * n].split(), lines[2 * n + 1].split()
+        assert int(a[0]) == n + 1
+        assert abs(float(a[1]) - o.time()) <= 1e-5 * o.time()
+        np.testing.assert_allclose([float(v) for v in a[2:]], o.forces([1, 3], 0), rtol=2e-5, atol=1e-9)
+        np.testing.assert_allclose([float(v) for v in b], o.forces([4], 1), rtol=2e-5, atol=1e-9)
+        n += 1
+    assert n > 3 and len(lines) == 2 * n
