@@ -139,6 +139,32 @@ int tsm_get_state(tsm_ctx* ctx, int sample, double* prim);
 int tsm_forces(tsm_ctx* ctx, int sample, int force_index, const int* tags, int ntags,
                double* out);
 
+/* The reference's outputs after a step read the gradient array dE as the loop
+   body left it: the gradients of the last Runge-Kutta stage's input state
+   (compute_residual recomputes them every stage, src/fv.cc:1476, :2295-2304),
+   unless a Navier-Stokes force evaluation recomputed them on the current state
+   (src/force.cc:124).  tsm_keep_stage_state(ctx, 1) makes every step keep each
+   sample's last-stage input state (4 doubles per vertex and sample, one more
+   store per vertex in the last stage); before any step, and after
+   tsm_set_state, the kept state is the current one.  Call it before the steps
+   whose outputs need it; 0 stops keeping. */
+int tsm_keep_stage_state(tsm_ctx* ctx, int on);
+/* Gradients of the kept last-stage input state, the reference's dE after a
+   step: the vorticity output and moments (src/writer.cc:241-255,
+   src/fv.cc:624-679) use them.  dE: 8*n_vertex as in tsm_gradients.
+   TSM_E_ARG if the state is not kept. */
+int tsm_stage_gradients(tsm_ctx* ctx, int sample, double* dE);
+/* Surface quantities on the boundary faces whose tag is one of tags[ntags], in
+   bface order (FiniteVolume::output_surface_sf / output_surface_hf,
+   src/output_skin_friction.cc:14-96, src/output_heatflux.cc:14-92): the end
+   points' states after the face's own boundary condition; out[2j] = skin
+   friction f x n (f the viscous stress on the unit normal n), out[2j+1] =
+   heat flux -k gradT.n, k = mu cp / Pr, from the end points' mean velocity and
+   temperature gradients.  stage = 1 takes the gradients of the kept
+   last-stage state (tsm_stage_gradients), 0 those of the current state.
+   out: 2 doubles per selected face. */
+int tsm_surface(tsm_ctx* ctx, int sample, int stage, const int* tags, int ntags, double* out);
+
 /* Residual of the current state (FiniteVolume::compute_residual, src/fv.cc:1467). R: 4*n_vertex. */
 int tsm_residual(tsm_ctx* ctx, int sample, double* R);
 /* Vertex entropy-variable gradients (FiniteVolume::compute_gradients, src/fv.cc:699).

@@ -459,6 +459,9 @@ struct orc_solver {
   std::vector<Flx> Uold, Rs, R2;
   std::vector<double> dt, ssw;
   std::vector<V2> dE;       // nv*4
+  // dE is not yet that of the current set-up: the reference's loop computes
+  // the gradients of the initial state before its first output (src/fv.cc:2238)
+  bool dE_stale = true;
   std::vector<V2> dEc;      // nc*4
   double dt_global = 0, elapsed = 0, res_total = 1e20, res0 = 0;
   long iter = 0;
@@ -814,6 +817,7 @@ void compute_gradients(orc_solver* s) {
   const Mat& M = s->M;
   int nv = s->nv, nc = s->nc;
   s->dE.assign((size_t)nv * 4, V2{0.0, 0.0});
+  s->dE_stale = false;
   s->dEc.assign((size_t)nc * 4, V2{0.0, 0.0});
   for (int i = 0; i < nc; ++i) {
     const Cell& c = s->cell[i];
@@ -1170,6 +1174,7 @@ void orc_set_state(orc_solver* s, const double* w) {
   }
   s->elapsed = 0; s->iter = 0; s->counter = 0; s->res_total = 1e20; s->res0 = 0;
   s->reached = false;
+  s->dE_stale = true;
 }
 void orc_get_state(const orc_solver* s, double* w) {
   for (int i = 0; i < s->nv; ++i) {
@@ -1240,6 +1245,33 @@ long orc_run(orc_solver* s, long nsteps, double* log) {
 }
 double orc_time(const orc_solver* s) { return s->elapsed; }
 
+namespace {
+// Mean over a boundary face's two end points of the velocity and temperature
+// gradients formed from the entropy-variable gradients dE and the end points'
+// entropy variables: gradU = -dE[vx]/l + (vx/l^2) dE[l] (and likewise v),
+// gradT = dE[l] / (R l^2) (src/force.cc:160-175, src/output_skin_friction.cc:
+// 64-74, src/output_heatflux.cc:69-76).
+struct WallGrad { V2 gu, gv, gT; };
+WallGrad wall_gradients(const orc_solver* s, int v0, int v1, const Ent& e0, const Ent& e1) {
+  const double f0 = 1.0 / e0.l, f1 = 1.0 / e1.l;
+  const V2* g0 = &s->dE[(size_t)v0 * 4];
+  const V2* g1 = &s->dE[(size_t)v1 * 4];
+  const double a0 = f0 * e0.vx / e0.l, a1 = f1 * e1.vx / e1.l;
+  const double b0 = f0 * e0.vy / e0.l, b1 = f1 * e1.vy / e1.l;
+  V2 gu0{g0[1].x * -f0 + g0[3].x * a0, g0[1].y * -f0 + g0[3].y * a0};
+  V2 gu1{g1[1].x * -f1 + g1[3].x * a1, g1[1].y * -f1 + g1[3].y * a1};
+  V2 gv0{g0[2].x * -f0 + g0[3].x * b0, g0[2].y * -f0 + g0[3].y * b0};
+  V2 gv1{g1[2].x * -f1 + g1[3].x * b1, g1[2].y * -f1 + g1[3].y * b1};
+  const double R = s->M.p.gas_const;
+  const double t0 = 1.0 / (R * std::pow(e0.l, 2.0)), t1 = 1.0 / (R * std::pow(e1.l, 2.0));
+  WallGrad w;
+  w.gu = V2{gu0.x * 0.5 + gu1.x * 0.5, gu0.y * 0.5 + gu1.y * 0.5};
+  w.gv = V2{gv0.x * 0.5 + gv1.x * 0.5, gv0.y * 0.5 + gv1.y * 0.5};
+  w.gT = V2{(g0[3].x * t0) * 0.5 + (g1[3].x * t1) * 0.5, (g0[3].y * t0) * 0.5 + (g1[3].y * t1) * 0.5};
+  return w;
+}
+}  // namespace
+
 // src/force.cc:118-217: pressure and viscous forces on the boundary faces whose
 // tag is in tags, in bface order.  The reference takes the boundary condition
 // of bface[force_index] (its loop index) rather than of the face summed; kept.
@@ -1265,18 +1297,8 @@ void orc_forces(orc_solver* s, int force_index, const int* tags, int ntags, doub
     if (M.p.model == 1) {
       const double T = (st[0].T + st[1].T) / 2.0;
       const double mu = M.viscosity(T);
-      const double f0 = 1.0 / e0.l, f1 = 1.0 / e1.l;
-      const V2* g0 = &s->dE[(size_t)v0 * 4];
-      const V2* g1 = &s->dE[(size_t)v1 * 4];
-      // gradU_j = -f dE[vx] + f (vx/l) dE[l], then the mean of the two ends
-      const double a0 = f0 * e0.vx / e0.l, a1 = f1 * e1.vx / e1.l;
-      const double b0 = f0 * e0.vy / e0.l, b1 = f1 * e1.vy / e1.l;
-      V2 gu0{g0[1].x * -f0 + g0[3].x * a0, g0[1].y * -f0 + g0[3].y * a0};
-      V2 gu1{g1[1].x * -f1 + g1[3].x * a1, g1[1].y * -f1 + g1[3].y * a1};
-      V2 gv0{g0[2].x * -f0 + g0[3].x * b0, g0[2].y * -f0 + g0[3].y * b0};
-      V2 gv1{g1[2].x * -f1 + g1[3].x * b1, g1[2].y * -f1 + g1[3].y * b1};
-      V2 gu{gu0.x * 0.5 + gu1.x * 0.5, gu0.y * 0.5 + gu1.y * 0.5};
-      V2 gv{gv0.x * 0.5 + gv1.x * 0.5, gv0.y * 0.5 + gv1.y * 0.5};
+      const WallGrad w = wall_gradients(s, v0, v1, e0, e1);
+      const V2 gu = w.gu, gv = w.gv;
       const double div = gu.x + gv.y;
       const double sxx = 2.0 * mu * (gu.x - (1.0 / 3.0) * div);
       const double syy = 2.0 * mu * (gv.y - (1.0 / 3.0) * div);
@@ -1288,6 +1310,56 @@ void orc_forces(orc_solver* s, int force_index, const int* tags, int ntags, doub
   out[0] = inv.x; out[1] = inv.y; out[2] = vis.x; out[3] = vis.y;
 }
 
+// The entropy-variable gradients the reference's dE holds: after a step those
+// its last stage computed, of that stage's input state (src/fv.cc:1476, :2295-2304);
+// before any step, of the current state.
+void orc_stage_gradients(orc_solver* s, double* out) {
+  if (s->dE_stale) compute_gradients(s);
+  for (int i = 0; i < s->nv; ++i)
+    for (int k = 0; k < 4; ++k) {
+      out[8 * i + 2 * k] = s->dE[(size_t)i * 4 + k].x;
+      out[8 * i + 2 * k + 1] = s->dE[(size_t)i * 4 + k].y;
+    }
+}
+
+// src/output_skin_friction.cc:43-95 and src/output_heatflux.cc:43-90: for each
+// boundary face whose tag is in tags, in bface order, the skin friction
+// cf = f x n (f the viscous stress on the unit normal n) and the heat flux
+// -k gradT.n, from the end points' states after the face's own boundary
+// condition and the gradients dE holds.
+void orc_surface(orc_solver* s, const int* tags, int ntags, double* out) {
+  const Mat& M = s->M;
+  if (s->dE_stale) compute_gradients(s);
+  int j = 0;
+  for (size_t f = 0; f < s->bface.size(); ++f) {
+    const Face& bf = s->bface[f];
+    bool in = false;
+    for (int k = 0; k < ntags; ++k) in = in || tags[k] == bf.type;
+    if (!in) continue;
+    const int v0 = bf.v[0], v1 = bf.v[1];
+    const BC& b = s->bcof(bf.type);
+    Prim st[2] = {s->W[v0], s->W[v1]};
+    bc_apply1(b, s->bstate_at((int)f, v0), bf, st[0]);
+    bc_apply1(b, s->bstate_at((int)f, v1), bf, st[1]);
+    const Ent e0 = prim2ent(M, st[0]), e1 = prim2ent(M, st[1]);
+    const double T = (st[0].T + st[1].T) / 2.0;
+    const double mu = M.viscosity(T);
+    const double kc = mu * M.Cp / M.p.prandtl;
+    const WallGrad w = wall_gradients(s, v0, v1, e0, e1);
+    const double div = w.gu.x + w.gv.y;
+    const double sxx = 2.0 * mu * (w.gu.x - (1.0 / 3.0) * div);
+    const double syy = 2.0 * mu * (w.gv.y - (1.0 / 3.0) * div);
+    const double sxy = mu * (w.gu.y + w.gv.x);
+    const double im = 1.0 / bf.measure;
+    const V2 n{bf.normal.x * im, bf.normal.y * im};
+    const double fx = sxx * n.x + sxy * n.y;
+    const double fy = sxy * n.x + syy * n.y;
+    out[2 * j] = fx * n.y - fy * n.x;
+    out[2 * j + 1] = (w.gT.x * n.x + w.gT.y * n.y) * -kc;
+    ++j;
+  }
+}
+
 int orc_set_bc_state(orc_solver* s, const int* bface_v, const double* state) {
   // state: per input bface b and end point k (bface_v[2b+k]) the values (T,u,v,p)
   std::map<std::pair<int, int>, int> in;
@@ -1295,6 +1367,7 @@ int orc_set_bc_state(orc_solver* s, const int* bface_v, const double* state) {
   for (int b = 0; b < nb; ++b)
     in[{std::min(bface_v[2 * b], bface_v[2 * b + 1]), std::max(bface_v[2 * b], bface_v[2 * b + 1])}] = b;
   s->bst.assign(2 * (size_t)nb, Prim{0, 0, 0, 0});
+  s->dE_stale = true;
   for (int bi = 0; bi < nb; ++bi) {
     const Face& f = s->bface[bi];
     auto it = in.find({std::min(f.v[0], f.v[1]), std::max(f.v[0], f.v[1])});

@@ -95,6 +95,12 @@ double orc_time(const orc_solver* s);
 /* pressure (out[0..1]) and viscous (out[2..3]) force on the boundary faces with
    one of the tags, current state (src/force.cc:118-217) */
 void orc_forces(orc_solver* s, int force_index, const int* tags, int ntags, double* out);
+/* The entropy-variable gradients the reference's dE holds: after a step those of
+   its last stage's input state; before any step those of the current state. */
+void orc_stage_gradients(orc_solver* s, double* out);
+/* Skin friction and heat flux (2 per bface whose tag is listed, bface order)
+   from the gradients dE holds (src/output_skin_friction.cc, src/output_heatflux.cc). */
+void orc_surface(orc_solver* s, const int* tags, int ntags, double* out);
 /* boundary states of this run per input bface b and end point k (the vertex
    bface_v[2b+k]): state[8b+4k..] = (T,u,v,p), the boundary expressions
    evaluated there; replaces the tags' constant states.  0 or -1. */

@@ -194,6 +194,9 @@ def lib():
         L.tsm_residual_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_set_bc_state.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_forces.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+        L.tsm_keep_stage_state.argtypes = [C.c_void_p, C.c_int]
+        L.tsm_stage_gradients.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.tsm_surface.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_error_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.tsm_global.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.tsm_geometry_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -335,6 +338,7 @@ class Solver:
         self._h = h
         self.nv = mesh.n_vertex
         self.n_bface = int(mesh.bface_tag.size)
+        self._bface_tag = np.asarray(mesh.bface_tag, dtype=np.int32)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -407,6 +411,27 @@ class Solver:
         out = np.zeros(4)
         _check(lib().tsm_forces(self._h, sample, int(force_index), t.ctypes.data_as(C.c_void_p),
                                 t.size, _dptr(out)))
+        return out
+
+    def keep_stage_state(self, on=True):
+        """Keep each step's last-stage input state, whose gradients the
+        reference's outputs after a step use (stage_gradients, surface)."""
+        _check(lib().tsm_keep_stage_state(self._h, int(bool(on))))
+
+    def stage_gradients(self, sample=0):
+        """(n_vertex, 8) gradients of the kept last-stage input state."""
+        a = np.empty((self.nv, 8))
+        _check(lib().tsm_stage_gradients(self._h, sample, _dptr(a)))
+        return a
+
+    def surface(self, tags, sample=0, stage=True):
+        """(n, 2) skin friction and heat flux of the boundary faces with these
+        tags, in bface order; gradients of the kept last-stage state (stage)
+        or of the current state."""
+        t = np.ascontiguousarray(tags, dtype=np.int32)
+        out = np.zeros((int(np.isin(self._bface_tag, t).sum()), 2))
+        _check(lib().tsm_surface(self._h, sample, int(bool(stage)), t.ctypes.data_as(C.c_void_p),
+                                 t.size, _dptr(out)))
         return out
 
     def residual_norms(self, sample=0):

@@ -505,9 +505,80 @@ int main(int argc, char** argv) {
   // overwriting sol0001 every write_frequency steps (src/fv.cc:2241-2245, :2318-2323, :2371-2377)
   WriterPool writer(std::min(ns, std::max(1, std::min(8, (int)std::thread::hardware_concurrency() - 1))));
   const bool raw = S.write_format == "raw";
+  auto listed = [&](const char* n) {
+    return std::find(S.write_variables.begin(), S.write_variables.end(), n) != S.write_variables.end();
+  };
+  // The outputs after a step read the gradient array as the loop body left it:
+  // the gradients of the last stage's input state (src/fv.cc:1476, :2295-2304),
+  // or of the current state where a Navier-Stokes force evaluation recomputed
+  // them (src/force.cc:124).  Before any step both are those of the initial state.
+  const bool ns_model = S.p.model == 1;
+  const bool fresh_grad = ns_model && !S.forces.empty();
+  const bool surf_ns = ns_model && !S.write_surfaces.empty();
+  const bool kept = !fresh_grad && (listed("vorticity") || surf_ns);
+  if (kept && tsm_keep_stage_state(ctx, 1) != TSM_OK) {
+    std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
+    return 1;
+  }
+  auto out_gradients = [&](int k, double* g) {
+    return kept ? tsm_stage_gradients(ctx, k, g) : tsm_gradients(ctx, k, g);
+  };
+  // surfaces (src/writer.cc:336-377, src/output_skin_friction.cc, src/output_heatflux.cc):
+  // the reference's bface order is the part file's, its vertex order the
+  // preprocessed one
+  std::vector<int> bfv;
+  if (!S.write_surfaces.empty()) {
+    int nf = 0, nbq = 0;
+    bfv.assign(2 * (size_t)nbf, 0);
+    if (tsm_host_geometry(&m, S.p.cell_type, nullptr, nullptr, bfv.data(), nullptr, nullptr, nullptr,
+                          &nf, &nbq) != TSM_OK || nbq != nbf) {
+      std::cerr << "ERROR --> surfaces: " << tsm_last_error() << std::endl;
+      return 1;
+    }
+  }
+  auto surfaces = [&](int k, int counter) {
+    if (S.write_surfaces.empty()) return;
+    const std::string idx = stamp_name("", counter);
+    std::vector<std::vector<double>> sq(S.write_surfaces.size());
+    for (size_t i = 0; i < S.write_surfaces.size() && ns_model; ++i) {
+      const int tag = S.write_surfaces[i];
+      sq[i].assign(2 * (size_t)std::count(P.btag.begin(), P.btag.end(), tag), 0.0);
+      if (tsm_surface(ctx, k, kept ? 1 : 0, &tag, 1, sq[i].data()) != TSM_OK)
+        std::cerr << "ERROR --> surfaces: " << tsm_last_error() << std::endl;
+    }
+    writer.post(k, [&P, &S, &bfv, ns_model, idx, dir = dirs[k], w = W, sq]() {
+      bool good = true;
+      for (size_t i = 0; i < S.write_surfaces.size(); ++i) {
+        const int tag = S.write_surfaces[i];
+        const std::string sfx = idx + "_" + std::to_string(tag) + ".dat";
+        std::ofstream fv(dir + "/v" + sfx);
+        for (size_t b = 0; b < P.btag.size(); ++b) {
+          if (P.btag[b] != tag) continue;
+          const int v = bfv[2 * b];
+          const double* q = &w[4 * (size_t)v];
+          fv << P.x[v] << "  " << P.y[v] << "  " << q[3] << "  " << q[0] << "  " << q[1] << "  " << q[2]
+             << "  " << 0.0 << "  " << std::endl;
+        }
+        good = good && (bool)fv;
+        if (!ns_model) continue;
+        std::ofstream ff(dir + "/f" + sfx), fq(dir + "/q" + sfx);
+        size_t j = 0;
+        for (size_t b = 0; b < P.btag.size(); ++b) {
+          if (P.btag[b] != tag) continue;
+          const int v0 = bfv[2 * b], v1 = bfv[2 * b + 1];
+          const double cx = P.x[v0] * 0.5 + P.x[v1] * 0.5, cy = P.y[v0] * 0.5 + P.y[v1] * 0.5;
+          ff << cx << "  " << cy << "  " << sq[i][2 * j] << "  " << std::endl;
+          fq << cx << "  " << cy << "  " << sq[i][2 * j + 1] << "  " << std::endl;
+          ++j;
+        }
+        good = good && (bool)ff && (bool)fq;
+      }
+      return good;
+    });
+  };
   auto output = [&](int k, int counter, double t, bool master = true) {
     tsm_get_state(ctx, k, W.data());
-    if (!raw) tsm_gradients(ctx, k, dE.data());
+    if (!raw) out_gradients(k, dE.data());
     const std::string name = stamp_name("sol", counter) + (raw ? ".raw" : ".vtk");
     const std::string dir = dirs[k];
     writer.post(k, [&P, &dcarea, &M, &S, raw, nv, dir, name, t, master, w = W,
@@ -517,15 +588,13 @@ int main(int argc, char** argv) {
       if (master) std::ofstream(dir + "/master_file.visit", std::ios::app) << name << std::endl;
       return good;
     });
+    surfaces(k, counter);
   };
   // online statistics across samples at each stamp (Welford in sample order);
   // a steady run has two: the initial and the converged state
   const int nst = steady ? 2 : S.n_time_stamps + 1;
   std::vector<std::vector<std::vector<double>>> snap;
   if (S.online_stat) snap.assign(nst, std::vector<std::vector<double>>(ns));
-  auto listed = [&](const char* n) {
-    return std::find(S.write_variables.begin(), S.write_variables.end(), n) != S.write_variables.end();
-  };
   const bool has_vort = listed("vorticity");
   std::vector<std::vector<std::vector<double>>> vsnap;  // per stamp and sample: vorticity
   if (S.online_stat && has_vort) vsnap.assign(nst, std::vector<std::vector<double>>(ns));
@@ -534,7 +603,7 @@ int main(int argc, char** argv) {
     tsm_get_state(ctx, k, W.data());
     snap[counter][k] = W;
     if (has_vort) {
-      tsm_gradients(ctx, k, dE.data());
+      out_gradients(k, dE.data());
       std::vector<double> om(nv);
       for (int i = 0; i < nv; ++i) om[i] = vorticity(M, &W[4 * i], &dE[8 * i]);
       vsnap[counter][k] = std::move(om);

@@ -55,6 +55,7 @@ struct tsm_ctx {
   double* force_buf = nullptr;  // per-face force terms and the sum (tsm_forces)
   int* force_sel = nullptr;
   size_t force_cap = 0;
+  double* W_keep = nullptr;     // last stage's input states (tsm_keep_stage_state)
   std::vector<int> h_btype;
   double* red = nullptr;   // reduction partials for global / error norms
   size_t red_cap = 0;
@@ -511,6 +512,9 @@ extern "C" int tsm_set_state(tsm_ctx* c, int s, const double* prim) {
   c->bind_state();
   tsm::launch_prim2ent(c->G, c->A, s, c->st);
   HIP_TRY(hipGetLastError());
+  if (c->A.W_keep)  // before any step the kept state is the current one
+    HIP_TRY(hipMemcpyAsync(c->A.W_keep + 4 * off, c->W[c->cur] + 4 * off, aos.size() * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->st));
   tsm::Ctl q{};
   q.res_total = 1e20; q.counter = 1; q.done = 0;
   HIP_TRY(hipMemcpyAsync(c->A.ctl + s, &q, sizeof(q), hipMemcpyHostToDevice, c->st));
@@ -576,6 +580,81 @@ extern "C" int tsm_forces(tsm_ctx* c, int s, int force_index, const int* tags, i
                      c->force_buf + 4, c->force_buf, c->st);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(out, c->force_buf, 4 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  return TSM_OK;
+}
+
+extern "C" int tsm_keep_stage_state(tsm_ctx* c, int on) {
+  if (!c) return fail(TSM_E_ARG, "bad argument");
+  if (on && !c->A.W_keep) {
+    if (!c->W_keep) {
+      int rc = c->alloc(&c->W_keep, 4 * c->A.cs);
+      if (rc) return rc;
+    }
+    c->csr_ready();
+    HIP_TRY(hipMemcpyAsync(c->W_keep, c->W[c->cur], 4 * c->A.cs * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->st));
+    c->A.W_keep = c->W_keep;
+  } else if (!on) {
+    c->A.W_keep = nullptr;
+  }
+  // a captured step graph holds the stage kernels' arguments
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  HIP_TRY(hipStreamSynchronize(c->st));
+  return TSM_OK;
+}
+
+namespace {
+// Vertex gradients of sample s into its slice of A.dE: of the kept last-stage
+// input state (stage) or of the current state.
+int sample_gradients(tsm_ctx* c, int s, bool stage) {
+  if (stage && !c->A.W_keep)
+    return fail(TSM_E_ARG, "the last stage's state is not kept (tsm_keep_stage_state)");
+  c->csr_ready();
+  tsm::StageArgs K = c->A;
+  K.ignore_done = 1;
+  if (stage) {  // entropy variables of the kept state in the scratch entropy buffer
+    K.W_in = c->A.W_keep;
+    K.E_in = c->E[c->cur ^ 1];
+    tsm::launch_prim2ent(c->G, K, s, c->st);
+  }
+  tsm::launch_gradient_one(c->G, K, s, c->st);
+  HIP_TRY(hipGetLastError());
+  return TSM_OK;
+}
+}  // namespace
+
+extern "C" int tsm_stage_gradients(tsm_ctx* c, int s, double* dE) {
+  if (!c || !dE || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  int rc = sample_gradients(c, s, true);
+  if (rc) return rc;
+  return download_vec(c, c->A.dE, 8, s, dE);
+}
+
+extern "C" int tsm_surface(tsm_ctx* c, int s, int stage, const int* tags, int ntags, double* out) {
+  const int nb = c ? c->G.nb : 0;
+  if (!c || !out || s < 0 || s >= c->S || ntags < 0 || (ntags > 0 && !tags))
+    return fail(TSM_E_ARG, "bad argument");
+  std::vector<int> sel;
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < ntags; ++k)
+      if (c->geo.bface[b].tag == tags[k]) { sel.push_back(b); break; }
+  if (sel.empty()) return TSM_OK;
+  if (c->force_cap < sel.size()) {
+    int rc = c->alloc(&c->force_buf, 4 * sel.size() + 4);
+    if (rc) return rc;
+    rc = c->alloc(&c->force_sel, sel.size());
+    if (rc) return rc;
+    c->force_cap = sel.size();
+  }
+  int rc = sample_gradients(c, s, stage != 0);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(c->force_sel, sel.data(), sel.size() * sizeof(int), hipMemcpyHostToDevice,
+                         c->st));
+  tsm::launch_surface(c->G, c->A, s, c->force_sel, (int)sel.size(), c->force_buf, c->st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, c->force_buf, 2 * sel.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         c->st));
   HIP_TRY(hipStreamSynchronize(c->st));
   return TSM_OK;
 }

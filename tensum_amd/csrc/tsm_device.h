@@ -247,6 +247,7 @@ struct StageArgs {
   double* ssw;
   double* partial; // per-block reduction scratch
   double* dtl;     // per-vertex local time step (steady mode), null in unsteady mode
+  double* W_keep;  // last stage's input state of each sample's latest step, null unless kept
   Ctl* ctl;
   const double* gamma; const double* gasR;
   const double* tstamps;
@@ -285,6 +286,10 @@ void launch_forces(const DevGeo& G, const StageArgs& A, int s, int bq, const int
 void launch_flux_update(int flux, int rec, const DevGeo& G, const StageArgs& A, int S, int r,
                         int last, int mode, hipStream_t st, bool do_flux, bool do_update);
 void launch_gradient(const DevGeo& G, const StageArgs& A, int S, hipStream_t st);
+// vertex gradients of sample s alone (into the sample's slice of A.dE)
+void launch_gradient_one(const DevGeo& G, const StageArgs& A, int s, hipStream_t st);
+void launch_surface(const DevGeo& G, const StageArgs& A, int s, const int* sel, int nsel,
+                    double* out, hipStream_t st);
 void launch_dt(const DevGeo& G, const StageArgs& A, const StepParams& P, int S, hipStream_t st);
 void launch_step_final(const DevGeo& G, const StageArgs& A, const StepParams& P, int S,
                        int log_slot, hipStream_t st);

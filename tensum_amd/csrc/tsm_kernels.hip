@@ -127,8 +127,8 @@ __device__ __forceinline__ void cell_gradient(const DevGeo& G, const StageArgs& 
 // ---------------------------------------------------------------------------
 // K1: vertex gradients (src/fv.cc:699-1059, src/reduce_gradient_periodic.cc)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gradient(DevGeo G, StageArgs A) {
-  const int s = blockIdx.y;
+__global__ __launch_bounds__(256) void k_gradient(DevGeo G, StageArgs A, int s_only) {
+  const int s = s_only >= 0 ? s_only : (int)blockIdx.y;
   if (!A.ignore_done && A.ctl[s].done) return;
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= G.nv) return;
@@ -424,6 +424,7 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
       Prim w1 = con2prim(gas, U);
       store_prim(A.W_out, off + v, w1);
       store_ent(A.E_out, off + v, prim2ent(gas, w1));
+      if (last && A.W_keep) store_prim(A.W_keep, off + v, w0);
     }
   }
   if (MODE != 0 && last) {
@@ -583,6 +584,41 @@ __global__ __launch_bounds__(256) void k_step_final(StageArgs A, StepParams P, i
 // condition applied to the end-point states is that of bface bq (the
 // reference's loop index).  One block: the lanes form the per-face terms, lane 0
 // adds them in bface order.
+// Mean over a boundary face's two end points of the velocity gradients, and
+// with HEAT of the temperature gradient, formed from the entropy-variable
+// gradients g (f, vx, vy, l) x (x, y) and the end points' entropy variables:
+// gradU = -g[vx]/l + (vx/l^2) g[l] (likewise v), gradT = g[l] / (R l^2)
+// (src/force.cc:160-175, src/output_skin_friction.cc:64-74,
+// src/output_heatflux.cc:69-76).
+struct WallGrad { double gux, guy, gvx, gvy, gTx, gTy; };
+template <bool HEAT>
+__device__ WallGrad wall_gradients(const Gas& gas, const Ent& e0, const Ent& e1, const double* g0,
+                                   const double* g1) {
+  const double f0 = fdiv(1.0, e0.l), f1 = fdiv(1.0, e1.l);
+  const double a0 = fdiv(f0 * e0.vx, e0.l), a1 = fdiv(f1 * e1.vx, e1.l);
+  const double b0 = fdiv(f0 * e0.vy, e0.l), b1 = fdiv(f1 * e1.vy, e1.l);
+  const double gu0x = g0[2] * -f0 + g0[6] * a0, gu0y = g0[3] * -f0 + g0[7] * a0;
+  const double gu1x = g1[2] * -f1 + g1[6] * a1, gu1y = g1[3] * -f1 + g1[7] * a1;
+  const double gv0x = g0[4] * -f0 + g0[6] * b0, gv0y = g0[5] * -f0 + g0[7] * b0;
+  const double gv1x = g1[4] * -f1 + g1[6] * b1, gv1y = g1[5] * -f1 + g1[7] * b1;
+  WallGrad w;
+  w.gux = gu0x * 0.5 + gu1x * 0.5; w.guy = gu0y * 0.5 + gu1y * 0.5;
+  w.gvx = gv0x * 0.5 + gv1x * 0.5; w.gvy = gv0y * 0.5 + gv1y * 0.5;
+  w.gTx = w.gTy = 0.0;
+  if (HEAT) {
+    const double t0 = fdiv(1.0, gas.R * (e0.l * e0.l)), t1 = fdiv(1.0, gas.R * (e1.l * e1.l));
+    w.gTx = (g0[6] * t0) * 0.5 + (g1[6] * t1) * 0.5;
+    w.gTy = (g0[7] * t0) * 0.5 + (g1[7] * t1) * 0.5;
+  }
+  return w;
+}
+
+__device__ double wall_viscosity(const ViscParams& vp, double T) {  // src/material.h:551-568
+  if (vp.mu_model == 0) return vp.mu_ref;
+  if (vp.mu_model == 1) return fdiv(vp.mu_ref * pow(T, 1.5), T + vp.T_0);
+  return vp.mu_ref * pow(fdiv(T, vp.T_ref), vp.omega);
+}
+
 __global__ __launch_bounds__(256) void k_forces(DevGeo G, StageArgs A, int s, int bq,
                                                 const int* sel, int nsel, int ns_model,
                                                 double* terms, double* out) {
@@ -605,22 +641,9 @@ __global__ __launch_bounds__(256) void k_forces(DevGeo G, StageArgs A, int s, in
     if (ns_model) {
       const Ent e0 = prim2ent(gas, s0), e1 = prim2ent(gas, s1);
       const double T = (s0.T + s1.T) / 2.0;
-      double mu;
-      if (A.vp.mu_model == 0) mu = A.vp.mu_ref;
-      else if (A.vp.mu_model == 1) mu = fdiv(A.vp.mu_ref * pow(T, 1.5), T + A.vp.T_0);
-      else mu = A.vp.mu_ref * pow(fdiv(T, A.vp.T_ref), A.vp.omega);
-      const double f0 = fdiv(1.0, e0.l), f1 = fdiv(1.0, e1.l);
-      const double* g0 = A.dE + 8 * (off + v0);
-      const double* g1 = A.dE + 8 * (off + v1);
-      const double a0 = fdiv(f0 * e0.vx, e0.l), a1 = fdiv(f1 * e1.vx, e1.l);
-      const double b0 = fdiv(f0 * e0.vy, e0.l), b1 = fdiv(f1 * e1.vy, e1.l);
-      // (f, vx, vy, l) x (x, y): vx at 2-3, vy at 4-5, l at 6-7
-      const double gu0x = g0[2] * -f0 + g0[6] * a0, gu0y = g0[3] * -f0 + g0[7] * a0;
-      const double gu1x = g1[2] * -f1 + g1[6] * a1, gu1y = g1[3] * -f1 + g1[7] * a1;
-      const double gv0x = g0[4] * -f0 + g0[6] * b0, gv0y = g0[5] * -f0 + g0[7] * b0;
-      const double gv1x = g1[4] * -f1 + g1[6] * b1, gv1y = g1[5] * -f1 + g1[7] * b1;
-      const double gux = gu0x * 0.5 + gu1x * 0.5, guy = gu0y * 0.5 + gu1y * 0.5;
-      const double gvx = gv0x * 0.5 + gv1x * 0.5, gvy = gv0y * 0.5 + gv1y * 0.5;
+      const double mu = wall_viscosity(A.vp, T);
+      const WallGrad w = wall_gradients<false>(gas, e0, e1, A.dE + 8 * (off + v0), A.dE + 8 * (off + v1));
+      const double gux = w.gux, guy = w.guy, gvx = w.gvx, gvy = w.gvy;
       const double div = gux + gvy;
       const double sxx = 2.0 * mu * (gux - (1.0 / 3.0) * div);
       const double syy = 2.0 * mu * (gvy - (1.0 / 3.0) * div);
@@ -642,6 +665,48 @@ void launch_forces(const DevGeo& G, const StageArgs& A, int s, int bq, const int
                    int ns_model, double* terms, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_forces, dim3(1), dim3(256), 0, st, G, A, s, bq, sel, nsel, ns_model, terms,
                      out);
+}
+
+// Skin friction and heat flux per selected boundary face (src/output_skin_friction.cc:
+// 43-95, src/output_heatflux.cc:43-90): the end points' states after the face's
+// own boundary condition, the gradients in A.dE, the unit normal n; out[2j] =
+// f x n with f the viscous stress on n, out[2j+1] = -k gradT.n, k = mu cp / Pr.
+__global__ __launch_bounds__(256) void k_surface(DevGeo G, StageArgs A, int s, const int* sel,
+                                                 int nsel, double* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nsel) return;
+  const size_t off = (size_t)s * G.nv;
+  const Gas gas{A.gamma[s], A.gasR[s]};
+  const int b = sel[j];
+  const int type = G.btype[b];
+  const int v0 = G.bv[2 * b], v1 = G.bv[2 * b + 1];
+  const double bx = G.bn[2 * b], by = G.bn[2 * b + 1];
+  Prim s0 = load_prim(A.W_in, off + v0), s1 = load_prim(A.W_in, off + v1);
+  bc_apply1(type, bc_values(G, s, b, bc_end(G, b, v0)), bx, by, s0);
+  bc_apply1(type, bc_values(G, s, b, bc_end(G, b, v1)), bx, by, s1);
+  const Ent e0 = prim2ent(gas, s0), e1 = prim2ent(gas, s1);
+  const double T = (s0.T + s1.T) / 2.0;
+  const double mu = wall_viscosity(A.vp, T);
+  const double cp = fdiv(gas.gamma * gas.R, gas.gamma - 1.0);
+  const double kc = fdiv(mu * cp, A.vp.prandtl);
+  const WallGrad w = wall_gradients<true>(gas, e0, e1, A.dE + 8 * (off + v0), A.dE + 8 * (off + v1));
+  const double div = w.gux + w.gvy;
+  const double sxx = 2.0 * mu * (w.gux - (1.0 / 3.0) * div);
+  const double syy = 2.0 * mu * (w.gvy - (1.0 / 3.0) * div);
+  const double sxy = mu * (w.guy + w.gvx);
+  const double im = fdiv(1.0, fsqrt(bx * bx + by * by));
+  const double nx = bx * im, ny = by * im;
+  const double fx = sxx * nx + sxy * ny;
+  const double fy = sxy * nx + syy * ny;
+  out[2 * j] = fx * ny - fy * nx;
+  out[2 * j + 1] = (w.gTx * nx + w.gTy * ny) * -kc;
+}
+
+void launch_surface(const DevGeo& G, const StageArgs& A, int s, const int* sel, int nsel,
+                    double* out, hipStream_t st) {
+  if (nsel > 0)
+    hipLaunchKernelGGL(k_surface, dim3((nsel + 255) / 256), dim3(256), 0, st, G, A, s, sel, nsel,
+                       out);
 }
 
 // Liou carbuncle sensor (src/compute_ssw.cc:7-39)
@@ -794,7 +859,10 @@ void launch_flux_update(int flux, int rec, const DevGeo& G, const StageArgs& A, 
 }
 
 void launch_gradient(const DevGeo& G, const StageArgs& A, int S, hipStream_t st) {
-  hipLaunchKernelGGL(k_gradient, dim3((G.nv + 255) / 256, S), dim3(256), 0, st, G, A);
+  hipLaunchKernelGGL(k_gradient, dim3((G.nv + 255) / 256, S), dim3(256), 0, st, G, A, -1);
+}
+void launch_gradient_one(const DevGeo& G, const StageArgs& A, int s, hipStream_t st) {
+  hipLaunchKernelGGL(k_gradient, dim3((G.nv + 255) / 256, 1), dim3(256), 0, st, G, A, s);
 }
 void launch_dt(const DevGeo& G, const StageArgs& A, const StepParams& P, int S, hipStream_t st) {
   int nb = (G.nv + 255) / 256;

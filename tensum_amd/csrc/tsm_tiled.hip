@@ -467,7 +467,11 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
   const int bb0 = TG.bbeg[t];
   double acc[4] = {0, 0, 0, 0};
   Prim w0{1, 0, 0, 1};
-  if (own) w0 = Prim{Q[tid], Q[MS + tid], Q[2 * MS + tid], Q[3 * MS + tid]};
+  if (own) {
+    w0 = Prim{Q[tid], Q[MS + tid], Q[2 * MS + tid], Q[3 * MS + tid]};
+    if (LAST && A.W_keep)  // the last stage's input state, kept for the caller
+      reinterpret_cast<double4*>(A.W_keep)[off + v] = make_double4(w0.T, w0.u, w0.v, w0.p);
+  }
   const bool per = TG.tper[t] != 0;
   if (per) __syncthreads();  // owned states read: SR may overwrite the primitive rows
   // A vertex's gather entries: its faces, then its boundary faces.  The first
@@ -877,6 +881,8 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
   int gk0 = 0, gne = 0, gnf = 0;
   if (own) {
     const Prim w0 = Prim{Q[tid], Q[MS1 + tid], Q[2 * MS1 + tid], Q[3 * MS1 + tid]};
+    if (LAST && A.W_keep)  // the last stage's input state, kept for the caller
+      reinterpret_cast<double4*>(A.W_keep)[off + v] = make_double4(w0.T, w0.u, w0.v, w0.p);
     gk0 = TG.gptr[v];
     gne = TG.gptr[v + 1] - gk0;
 #pragma unroll

@@ -55,6 +55,8 @@ def lib():
         L.orc_local_dt.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_set_bc_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_forces.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_stage_gradients.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_surface.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_run.restype = C.c_long
         L.orc_run.argtypes = [C.c_void_p, C.c_long, C.c_void_p]
         L.orc_time.restype = C.c_double
@@ -98,6 +100,7 @@ class Oracle:
         self._h = C.c_void_p(h)
         self.nv = mesh.n_vertex
         self._bface_v = np.asarray(mesh.bface_v, dtype=np.int32)
+        self._bface_tag = np.asarray(mesh.bface_tag, dtype=np.int32)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -150,6 +153,19 @@ class Oracle:
         t = np.ascontiguousarray(tags, dtype=np.int32)
         out = np.empty(4)
         lib().orc_forces(self._h, int(force_index), _p(t), t.size, _p(out))
+        return out
+
+    def stage_gradients(self):
+        a = np.empty((self.nv, 8))
+        lib().orc_stage_gradients(self._h, _p(a))
+        return a
+
+    def surface(self, tags):
+        """(n, 2): skin friction and heat flux of each bface whose tag is
+        listed, in the reference's bface order."""
+        t = np.ascontiguousarray(tags, dtype=np.int32)
+        out = np.empty((int(np.isin(self._bface_tag, t).sum()), 2))
+        lib().orc_surface(self._h, _p(t), t.size, _p(out))
         return out
 
     def local_dt(self):

@@ -388,3 +388,108 @@ def test_driver_force_file_matches_oracle(tmp_path):
         np.testing.assert_allclose([float(v) for v in b], o.forces([4], 1), rtol=2e-5, atol=1e-9)
         n += 1
     assert n > 3 and len(lines) == 2 * n
+
+
+NS_WALL_BC = """boundary
+{
+   1
+   {
+      type         noslip
+      temperature  1.2
+      xvelocity    0.0
+      yvelocity    0.0
+      zvelocity    0.0
+   }
+   3
+   {
+      type         noslip
+      xvelocity    0.1
+      yvelocity    0.0
+      zvelocity    0.0
+   }
+   4
+   {
+      type         inlet
+      temperature  1.0
+      xvelocity    0.5
+      yvelocity    0.0
+      zvelocity    0.0
+      pressure     1.0
+   }
+   2
+   {
+      type         outlet
+   }
+}"""
+
+
+def _omega(W, g, R=1.0):
+    """Vorticity dv/dx - du/dy by the chain rule from the gradients of the
+    entropy variables (f, vx, vy, l) x (x, y), with u = -vx/l, v = -vy/l and
+    -1/l = R T."""
+    Tm, u, v = W[:, 0], W[:, 1], W[:, 2]
+    return R * Tm * (g[:, 4] + v * g[:, 6] - g[:, 3] - u * g[:, 7])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("with_force", [False, True])
+def test_driver_surfaces_and_vorticity_match_oracle(tmp_path, with_force):
+    """output/surfaces on Navier-Stokes walls: v<stamp>_<tag>.dat (first end
+    point's x, y, p, T, u, v, w), f... (face centre, skin friction) and q...
+    (face centre, heat flux) at every solution output (src/writer.cc:336-377,
+    src/fv.cc:1993-1997), and the vorticity field, all from the gradients of
+    the last stage's input state, or of the current state when a force block
+    made the loop recompute them (src/force.cc:124)."""
+    import re
+    mesh = T.jittered_mesh(17, seed=13)
+    write_part_file(mesh, str(tmp_path))
+    text = VORTEX_PARAM.format(final_time=0.05, stamps=1)
+    text = re.sub(r"boundary\n\{.*?\n\}\nexact_soln", NS_WALL_BC + "\nexact_soln", text, flags=re.S)
+    for a, b in (("model       euler", "model       ns"), ("mu_ref   0.0", "mu_ref   0.01"),
+                 ("find_error            yes", "find_error            no"),
+                 ("   surfaces\n   {\n   }", "   surfaces\n   {\n      1\n      3\n   }")):
+        assert a in text
+        text = text.replace(a, b)
+    if with_force:
+        text = text.replace("integrals\n{\n}", "integrals\n{\n   force\n   {\n      walls\n      1\n"
+                            "      3\n   }\n}")
+    (tmp_path / "param.in").write_text(text)
+    r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    sd = tmp_path / "SAMPLE_0"
+    bc = {1: ("noslip", (1.2, 0.0, 0.0, 1.0)), 3: ("noslip", (1.0, 0.1, 0.0, 1.0), True),
+          4: ("inlet", (1.0, 0.5, 0.0, 1.0)), 2: ("outlet",)}
+    om = T.Mesh(mesh.x, mesh.y, mesh.bface_tag, mesh.bface_v, mesh.cell_v, mesh.periodic_ptr,
+                mesh.periodic_v, bc)
+    prm = T.Params(model="ns", mu_ref=0.01, flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8,
+                   final_time=0.05)
+    o = O.Oracle(om, prm)
+    o.set_state(vortex(mesh.x, mesh.y, box=None))
+    bv = o.geometry()["bface_v"]
+    bt = np.asarray(mesh.bface_tag)
+    for stamp in (0, 1):
+        if stamp:
+            n = o.run(1000)
+            assert n > 2
+            if with_force:
+                o.forces([1, 3], 0)
+        W = o.get_state()
+        g = o.stage_gradients()
+        om_vtk = _vtk_scalars(sd / f"sol000{stamp}_0.vtk", "vorticity", mesh.n_vertex)
+        np.testing.assert_allclose(om_vtk, _omega(W, g), rtol=1e-10, atol=1e-11)
+        for tag in (1, 3):
+            sel = np.flatnonzero(bt == tag)
+            v = np.loadtxt(sd / f"v000{stamp}_0_{tag}.dat")
+            v0 = bv[sel, 0]
+            np.testing.assert_allclose(v[:, :2], np.stack([mesh.x[v0], mesh.y[v0]], 1), rtol=1e-5)
+            np.testing.assert_allclose(v[:, 2:6], W[v0][:, [3, 0, 1, 2]], rtol=1e-5, atol=1e-9)
+            assert np.all(v[:, 6] == 0.0)
+            cx = 0.5 * (mesh.x[bv[sel, 0]] + mesh.x[bv[sel, 1]])
+            cy = 0.5 * (mesh.y[bv[sel, 0]] + mesh.y[bv[sel, 1]])
+            sq = o.surface([tag])
+            for pre, col in (("f", 0), ("q", 1)):
+                a = np.loadtxt(sd / f"{pre}000{stamp}_0_{tag}.dat")
+                np.testing.assert_allclose(a[:, :2], np.stack([cx, cy], 1), rtol=1e-5, atol=1e-9)
+                np.testing.assert_allclose(a[:, 2], sq[:, col], rtol=2e-5,
+                                           atol=1e-9 * np.abs(sq[:, col]).max())
