@@ -77,6 +77,11 @@ typedef struct tsm_params {
                          replay it (0: launch every kernel from the host) */
   int host_preprocess; /* 1: build the dual-mesh geometry on the host; 0: on the
                           device for median duals (the same arrays bit for bit) */
+  int strong_bc;      /* bc_scheme (src/parameter.h:31-34): 0 weak; 1 strong: after
+                         every stage update each bface, in bface order, imposes its
+                         boundary condition on its end points' state (src/fv.cc:
+                         1694-1704) and the last stage zeroes the momentum residual
+                         of noslip end points (src/fv.cc:1805-1819); CSR kernels */
 } tsm_params;
 
 /* NPART = 1 part file content (readme/README.md:155-201), 0-based gids. */

@@ -1115,6 +1115,28 @@ void update_solution(orc_solver* s, unsigned r) {
   }
 }
 
+// bc_scheme strong, src/fv.cc:1805-1819: before the last stage's residual norm
+// and update, the momentum residual of both end points of every noslip bface
+// is set to zero.
+void strong_zero_noslip(orc_solver* s) {
+  for (const Face& f : s->bface) {
+    if (s->bcof(f.type).type != BC_NOSLIP) continue;
+    for (int e = 0; e < 2; ++e) { s->Rs[f.v[e]].mx = 0; s->Rs[f.v[e]].my = 0; }
+  }
+}
+
+// bc_scheme strong, src/fv.cc:1694-1704: after each stage's update every bface,
+// in bface order, imposes its boundary condition on both end points' primitive
+// state; the entropy variables keep the values the update gave them.
+void strong_apply(orc_solver* s) {
+  for (size_t b = 0; b < s->bface.size(); ++b) {
+    const Face& f = s->bface[b];
+    const BC& bc = s->bcof(f.type);
+    bc_apply1(bc, s->bstate_at((int)b, f.v[0]), f, s->W[f.v[0]]);
+    bc_apply1(bc, s->bstate_at((int)b, f.v[1]), f, s->W[f.v[1]]);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1222,6 +1244,7 @@ long orc_run(orc_solver* s, long nsteps, double* log) {
     for (unsigned r = 0; r < n_rks; ++r) {
       if (!compute_residual(s)) return -1;
       if (r == n_rks - 1) {
+        if (P.strong_bc) strong_zero_noslip(s);
         double tot = residual_norm(s);
         if (s->iter == 0) {
           s->res0 = tot;
@@ -1230,6 +1253,7 @@ long orc_run(orc_solver* s, long nsteps, double* log) {
         s->res_total = tot / s->res0;
       }
       update_solution(s, r);
+      if (P.strong_bc) strong_apply(s);
     }
     ++s->iter;
     if (!P.unsteady) term_iter = s->iter;

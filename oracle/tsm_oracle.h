@@ -39,6 +39,7 @@ typedef struct orc_params {
   int n_time_stamps;
   int cell_type;      /* 0 median, 1 voronoi */
   int liou_fix;
+  int strong_bc;      /* bc_scheme: 0 weak, 1 strong */
 } orc_params;
 
 typedef struct orc_mesh {

@@ -41,7 +41,7 @@ class TsmParams(C.Structure):
                 ("kernel_path", C.c_int), ("tile_vertices", C.c_int),
                 ("block_grad", C.c_int), ("block_fu", C.c_int),
                 ("tile_fused_gradient", C.c_int), ("graph_steps", C.c_int),
-                ("host_preprocess", C.c_int)]
+                ("host_preprocess", C.c_int), ("strong_bc", C.c_int)]
 
 
 class TsmMesh(C.Structure):
@@ -86,6 +86,7 @@ class Params:
     time_stamps: int = 1
     cell: str = "median"
     liou_fix: bool = False
+    bc_scheme: str = "weak"        # "weak" or "strong" (boundary values imposed after each stage)
     kernel_path: str = "auto"      # "auto" (tiled where supported) or "csr"
     tile_vertices: int = 0
     block_grad: int = 0            # threads per tile, gradient kernel (0: auto)
@@ -116,6 +117,7 @@ class Params:
         p.tile_fused_gradient = int(self.fused_gradient)
         p.graph_steps = self.graph_steps
         p.host_preprocess = {"auto": 0, "host": 1}[self.preprocess]
+        p.strong_bc = {"weak": 0, "strong": 1}[self.bc_scheme]
         return p
 
 

@@ -158,7 +158,7 @@ bool read_settings(const std::string& file, Settings& S, std::string& err) {
   ENTRY("bc_scheme");
   R.word(w);
   NEED(w == "weak" || w == "strong", "read_numeric: unknown bc scheme " + w);
-  NEED(w == "weak", "bc_scheme strong is not supported (only weak boundary conditions)");
+  p.strong_bc = w == "strong";
   ENTRY("liou_fix");
   bool liou = false;
   if (!yesno(R, "liou_fix", liou, err)) return false;

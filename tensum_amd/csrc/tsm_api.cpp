@@ -121,6 +121,7 @@ int validate_params(const tsm_params& p) {
   if (p.time_scheme < 0 || p.time_scheme > 2)
     return fail(TSM_E_ARG, "time_scheme must be ssprk3 (0), rk4 (1) or heuns (2)");
   if (p.unsteady != 0 && p.unsteady != 1) return fail(TSM_E_ARG, "unsteady must be 0 or 1");
+  if (p.strong_bc != 0 && p.strong_bc != 1) return fail(TSM_E_ARG, "strong_bc must be 0 (weak) or 1 (strong)");
   if (p.time_step == 0.0 && p.cfl == 0.0) return fail(TSM_E_ARG, "Either time_step or cfl must be non-zero");
   if (p.time_step > 0.0 && p.cfl > 0.0) return fail(TSM_E_ARG, "both time_step and cfl specified");
   if (p.unsteady && !(p.final_time > 0.0))
@@ -173,7 +174,7 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   }
   // ---- path and vertex numbering -----------------------------------------
   // tiled stage kernels: SSP-RK3 without the Liou fix, Euler or Navier-Stokes
-  c->tiled = params->kernel_path == 0 && !params->liou_fix;
+  c->tiled = params->kernel_path == 0 && !params->liou_fix && !params->strong_bc;
   std::vector<int> perm(nv), inv(nv);
   if (c->tiled) {
     int tb = params->tile_vertices > 0 ? params->tile_vertices : 128;
@@ -357,6 +358,27 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     }
   }
   UP(blc, blc); UP(cwf, cwf); UP(cwb, cwb); UP(vbe_ptr, vbe_ptr); UP(vbe, vbe);
+  if (params->strong_bc) {  // boundary vertices with their bfaces in bface order
+    std::vector<std::vector<int>> vb(nv);
+    std::vector<unsigned char> noslip(nv, 0);
+    for (int b = 0; b < nb; ++b)
+      for (int e = 0; e < 2; ++e) {
+        const int v = bvv[2 * b + e];
+        vb[v].push_back(b);
+        if (btype[b] == TSM_BC_NOSLIP || btype[b] == tsm::BC_NOSLIP_ADIABATIC) noslip[v] = 1;
+      }
+    std::vector<int> sbv, sb_ptr(1, 0), sb;
+    for (int v = 0; v < nv; ++v) {
+      if (vb[v].empty()) continue;
+      sbv.push_back(v);
+      sb.insert(sb.end(), vb[v].begin(), vb[v].end());
+      sb_ptr.push_back((int)sb.size());
+    }
+    G.strong = 1;
+    G.nsbv = (int)sbv.size();
+    if (sbv.empty()) { sbv.push_back(0); sb.push_back(0); }
+    UP(sbv, sbv); UP(sb_ptr, sb_ptr); UP(sb, sb); UP(snoslip, noslip);
+  }
 #undef UP
   if (c->tiled) {
     const tsm::Tiling& T = c->tiling;
@@ -701,6 +723,7 @@ void enqueue_step(tsm_ctx* c, int log_slot, bool dt_ready = false) {
     tsm::launch_gradient(c->G, c->A, c->S, c->st);
     tsm::launch_flux_update(c->P.flux, c->P.reconstruct, c->G, c->A, c->S, r, r == nrk - 1, mode,
                             c->st, true, true);
+    if (c->P.strong_bc) tsm::launch_strong_bc(c->G, c->A, c->S, c->st);
     c->cur ^= 1;
   }
   c->bind_state();
@@ -1055,6 +1078,7 @@ extern "C" int tsm_bench(tsm_ctx* c, int nsteps, double* us) {
         HIP_TRY(hipEventRecord(ev[e++], c->st));
         tsm::launch_flux_update(c->P.flux, c->P.reconstruct, c->G, c->A, c->S, r, r == nrk - 1,
                                 mode, c->st, false, true);
+        if (c->P.strong_bc) tsm::launch_strong_bc(c->G, c->A, c->S, c->st);
         c->cur ^= 1;
       }
       c->bind_state();

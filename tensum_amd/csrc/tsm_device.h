@@ -46,6 +46,15 @@ struct DevGeo {
   const int* pset;      // nv: periodic set id or -1
   const int* pptr;      // nsets+1
   const int* pv;        // members
+  // strong boundary conditions (null when weak): the vertices on bfaces, each
+  // with its bfaces in bface order, and per vertex whether it ends a noslip bface
+  int strong;           // bc_scheme strong: stored entropy variables of boundary
+                        // vertices may predate the imposed boundary values
+  int nsbv;
+  const int* sbv;
+  const int* sb_ptr;    // nsbv+1
+  const int* sb;
+  const unsigned char* snoslip;  // nv
 };
 
 // Tile arrays of the LDS-staged kernels (device pointers, new vertex ids).
@@ -290,6 +299,8 @@ void launch_gradient(const DevGeo& G, const StageArgs& A, int S, hipStream_t st)
 void launch_gradient_one(const DevGeo& G, const StageArgs& A, int s, hipStream_t st);
 void launch_surface(const DevGeo& G, const StageArgs& A, int s, const int* sel, int nsel,
                     double* out, hipStream_t st);
+// strong boundary conditions on the stage output A.W_out of every live sample
+void launch_strong_bc(const DevGeo& G, const StageArgs& A, int S, hipStream_t st);
 void launch_dt(const DevGeo& G, const StageArgs& A, const StepParams& P, int S, hipStream_t st);
 void launch_step_final(const DevGeo& G, const StageArgs& A, const StepParams& P, int S,
                        int log_slot, hipStream_t st);

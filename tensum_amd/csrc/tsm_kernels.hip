@@ -106,7 +106,9 @@ __device__ __forceinline__ void cell_gradient(const DevGeo& G, const StageArgs& 
       int type = G.btype[bf];
       Ent E0 = load_ent(A.E_in, off + v0), E1 = load_ent(A.E_in, off + v1);
       Ent q0 = E0, q1 = E1;
-      if (bc_sets_state(type)) {
+      // a boundary that leaves the state as it is contributes nothing, unless
+      // strong boundary values made the stored entropy variables stale
+      if (bc_sets_state(type) || G.strong) {
         Prim s0 = load_prim(A.W_in, off + v0), s1 = load_prim(A.W_in, off + v1);
         bc_apply1(type, bc_values(G, A.sample, bf, 0), nx, ny, s0);
         bc_apply1(type, bc_values(G, A.sample, bf, 1), nx, ny, s1);
@@ -371,6 +373,9 @@ __global__ __launch_bounds__(256) void k_update(DevGeo G, StageArgs A, int r, in
         for (int k = 0; k < 4; ++k) tot[k] += acc[k] * 1.0;
       }
     }
+    // bc_scheme strong: the last stage zeroes the momentum residual of noslip
+    // end points before its norm and update (src/fv.cc:1805-1819)
+    if (MODE != 0 && last && G.snoslip && G.snoslip[v]) { tot[1] = 0.0; tot[2] = 0.0; }
     if (MODE == 0) {
       double* R = A.R + 4 * (off + v);
 #pragma unroll
@@ -707,6 +712,30 @@ void launch_surface(const DevGeo& G, const StageArgs& A, int s, const int* sel, 
   if (nsel > 0)
     hipLaunchKernelGGL(k_surface, dim3((nsel + 255) / 256), dim3(256), 0, st, G, A, s, sel, nsel,
                        out);
+}
+
+// bc_scheme strong (src/fv.cc:1694-1704): after a stage's update each bface, in
+// bface order, imposes its boundary condition on its end points' state; one
+// thread per boundary vertex applies that vertex's bfaces in that order.  The
+// entropy variables keep the update's values, as the reference's do.
+__global__ __launch_bounds__(256) void k_strong_bc(DevGeo G, StageArgs A) {
+  const int s = blockIdx.y;
+  if (A.ctl[s].done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.nsbv) return;
+  const int v = G.sbv[i];
+  const size_t off = (size_t)s * G.nv;
+  Prim w = load_prim(A.W_out, off + v);
+  for (int e = G.sb_ptr[i]; e < G.sb_ptr[i + 1]; ++e) {
+    const int b = G.sb[e];
+    bc_apply1(G.btype[b], bc_values(G, s, b, bc_end(G, b, v)), G.bn[2 * b], G.bn[2 * b + 1], w);
+  }
+  store_prim(A.W_out, off + v, w);
+}
+
+void launch_strong_bc(const DevGeo& G, const StageArgs& A, int S, hipStream_t st) {
+  if (G.nsbv > 0)
+    hipLaunchKernelGGL(k_strong_bc, dim3((G.nsbv + 255) / 256, S), dim3(256), 0, st, G, A);
 }
 
 // Liou carbuncle sensor (src/compute_ssw.cc:7-39)

@@ -25,7 +25,8 @@ class OrcParams(C.Structure):
                 ("time_scheme", C.c_int), ("unsteady", C.c_int),
                 ("cfl", C.c_double), ("time_step", C.c_double), ("final_time", C.c_double),
                 ("min_residue", C.c_double), ("max_iter", C.c_long),
-                ("n_time_stamps", C.c_int), ("cell_type", C.c_int), ("liou_fix", C.c_int)]
+                ("n_time_stamps", C.c_int), ("cell_type", C.c_int), ("liou_fix", C.c_int),
+                ("strong_bc", C.c_int)]
 
 
 _lib = None

@@ -493,3 +493,40 @@ def test_driver_surfaces_and_vorticity_match_oracle(tmp_path, with_force):
                 np.testing.assert_allclose(a[:, :2], np.stack([cx, cy], 1), rtol=1e-5, atol=1e-9)
                 np.testing.assert_allclose(a[:, 2], sq[:, col], rtol=2e-5,
                                            atol=1e-9 * np.abs(sq[:, col]).max())
+
+
+@pytest.mark.gpu
+def test_driver_strong_boundary_conditions_match_oracle(tmp_path):
+    """bc_scheme strong in param.in (src/parameter.cc:146-156) on Navier-Stokes
+    walls: the residual log and the final state against the oracle."""
+    import re
+    mesh = T.jittered_mesh(17, seed=13)
+    write_part_file(mesh, str(tmp_path))
+    text = VORTEX_PARAM.format(final_time=0.05, stamps=1)
+    text = re.sub(r"boundary\n\{.*?\n\}\nexact_soln", NS_WALL_BC + "\nexact_soln", text, flags=re.S)
+    for a, b in (("model       euler", "model       ns"), ("mu_ref   0.0", "mu_ref   0.01"),
+                 ("find_error            yes", "find_error            no"),
+                 ("bc_scheme    weak", "bc_scheme    strong"), ("format                vtk",
+                                                               "format                raw")):
+        assert a in text
+        text = text.replace(a, b)
+    (tmp_path / "param.in").write_text(text)
+    r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    sd = tmp_path / "SAMPLE_0"
+    bc = {1: ("noslip", (1.2, 0.0, 0.0, 1.0)), 3: ("noslip", (1.0, 0.1, 0.0, 1.0), True),
+          4: ("inlet", (1.0, 0.5, 0.0, 1.0)), 2: ("outlet",)}
+    om = T.Mesh(mesh.x, mesh.y, mesh.bface_tag, mesh.bface_v, mesh.cell_v, mesh.periodic_ptr,
+                mesh.periodic_v, bc)
+    prm = T.Params(model="ns", mu_ref=0.01, flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8,
+                   final_time=0.05, bc_scheme="strong")
+    o = O.Oracle(om, prm)
+    o.set_state(vortex(mesh.x, mesh.y, box=None))
+    n, log = o.run(1000, want_log=True)
+    res = np.loadtxt(sd / "residual.dat")
+    assert res.shape[0] == n > 2
+    np.testing.assert_allclose(res[:, 3], log[:, 3], rtol=1e-9)
+    t, W = T.read_raw(str(sd / "sol0001_0.raw"))
+    assert t == pytest.approx(0.05, rel=1e-14)
+    assert rel_l2(W, o.get_state()) <= 1e-10
