@@ -207,6 +207,14 @@ int tsm_time(tsm_ctx* ctx, int sample, double* t, long* iter);
    res_total; iter; counter = index of the next output time stamp
    (src/fv.cc:2324-2343); done = the sample has left its time loop. */
 int tsm_status(tsm_ctx* ctx, int sample, double* v, long* iter, int* counter, int* done);
+/* The loop clock a restart file carries after the state (src/writer.cc:404-407,
+   read back by src/fv.cc:334-338): the last iteration, the stamp counter, the
+   elapsed time and res0, the first step's residual norm that every later one
+   is normalised by.  tsm_set_clock continues a run from a restart: after
+   tsm_set_state it sets the clock, and the next step starts with the loop's
+   initial res_total = 1e20 (src/fv.cc:2234); iter > 0 keeps res0. */
+int tsm_get_clock(tsm_ctx* ctx, int sample, long* iter, int* counter, double* t, double* res0);
+int tsm_set_clock(tsm_ctx* ctx, int sample, long iter, int counter, double t, double res0);
 /* Residual norms of the sample's last step before normalisation, as the
    steady-mode residual log prints them (FiniteVolume::compute_residual_norm,
    src/fv.cc:1821-1851; log_messages, src/fv.cc:1944-1957): norms[4] =

@@ -1269,6 +1269,18 @@ long orc_run(orc_solver* s, long nsteps, double* log) {
 }
 double orc_time(const orc_solver* s) { return s->elapsed; }
 
+// The loop clock a restart file carries (src/fv.cc:334-338, written by
+// src/writer.cc:404-407): last iteration, stamp counter, elapsed time and the
+// first step's residual norm that later ones are normalised by.  The loop
+// then starts with residual_norm_total = 1e20 (src/fv.cc:2234).
+void orc_get_clock(const orc_solver* s, long* iter, int* counter, double* t, double* res0) {
+  *iter = s->iter; *counter = (int)s->counter; *t = s->elapsed; *res0 = s->res0;
+}
+void orc_set_clock(orc_solver* s, long iter, int counter, double t, double res0) {
+  s->iter = iter; s->counter = (unsigned)counter; s->elapsed = t; s->res0 = res0;
+  s->res_total = 1e20; s->reached = false;
+}
+
 namespace {
 // Mean over a boundary face's two end points of the velocity and temperature
 // gradients formed from the entropy-variable gradients dE and the end points'

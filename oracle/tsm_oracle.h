@@ -99,6 +99,10 @@ void orc_forces(orc_solver* s, int force_index, const int* tags, int ntags, doub
 /* The entropy-variable gradients the reference's dE holds: after a step those of
    its last stage's input state; before any step those of the current state. */
 void orc_stage_gradients(orc_solver* s, double* out);
+/* Loop clock of a restart file: iteration, stamp counter, time, first-step
+   residual norm (src/fv.cc:334-338). */
+void orc_get_clock(const orc_solver* s, long* iter, int* counter, double* t, double* res0);
+void orc_set_clock(orc_solver* s, long iter, int counter, double t, double res0);
 /* Skin friction and heat flux (2 per bface whose tag is listed, bface order)
    from the gradients dE holds (src/output_skin_friction.cc, src/output_heatflux.cc). */
 void orc_surface(orc_solver* s, const int* tags, int ntags, double* out);

@@ -197,6 +197,8 @@ def lib():
         L.tsm_set_bc_state.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_forces.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_keep_stage_state.argtypes = [C.c_void_p, C.c_int]
+        L.tsm_get_clock.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
+        L.tsm_set_clock.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_int, C.c_double, C.c_double]
         L.tsm_stage_gradients.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_surface.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_error_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
@@ -414,6 +416,17 @@ class Solver:
         _check(lib().tsm_forces(self._h, sample, int(force_index), t.ctypes.data_as(C.c_void_p),
                                 t.size, _dptr(out)))
         return out
+
+    def clock(self, sample=0):
+        """(iter, counter, t, res0): the loop clock a restart file carries."""
+        it, cn, t, r0 = C.c_long(), C.c_int(), C.c_double(), C.c_double()
+        _check(lib().tsm_get_clock(self._h, sample, C.byref(it), C.byref(cn), C.byref(t),
+                                   C.byref(r0)))
+        return it.value, cn.value, t.value, r0.value
+
+    def set_clock(self, it, counter, t, res0, sample=0):
+        """Continue from a restart: set after set_state."""
+        _check(lib().tsm_set_clock(self._h, sample, int(it), int(counter), float(t), float(res0)))
 
     def keep_stage_state(self, on=True):
         """Keep each step's last-stage input state, whose gradients the

@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -108,6 +109,39 @@ bool read_part_file(const std::string& path, PartFile& P) {
 bool mkdirs(const std::string& d) {
   if (mkdir(d.c_str(), 0755) == 0 || errno == EEXIST) return true;
   return die("cannot create " + d);
+}
+
+// The loop clock a restart file carries after the state (src/writer.cc:404-407).
+struct Clock {
+  bool valid = false;
+  long iter = 0;
+  int counter = 0;
+  double t = 0.0, res0 = 1.0;
+};
+
+// Keeps the lines of a log whose first field is <= n, as the reference's
+// awk '$1 <= n' does on a restart (src/fv.cc:350-380): a numeric first field
+// compares as a number, any other as a string against n's decimal form; an
+// empty line stays.
+bool truncate_log(const std::string& path, long n) {
+  std::ifstream in(path);
+  if (!in) return true;
+  std::vector<std::string> keep;
+  std::string line;
+  const std::string ns = std::to_string(n);
+  while (std::getline(in, line)) {
+    std::istringstream ls(line);
+    std::string f;
+    if (!(ls >> f)) { keep.push_back(line); continue; }
+    char* end = nullptr;
+    const double v = std::strtod(f.c_str(), &end);
+    const bool numeric = end && *end == '\0' && end != f.c_str() && std::isfinite(v);
+    if (numeric ? v <= (double)n : f <= ns) keep.push_back(line);
+  }
+  in.close();
+  std::ofstream out(path, std::ios::trunc);
+  for (const auto& l : keep) out << l << '\n';
+  return (bool)out;
 }
 
 std::string stamp_name(const std::string& base, int counter) {
@@ -456,8 +490,15 @@ int main(int argc, char** argv) {
       }
   std::vector<int> head(ns, 0);
   std::vector<double> W(4 * (size_t)nv), dE(8 * (size_t)nv);
+  const bool steady = S.time_mode == "steady";
+  std::vector<Clock> rc_clock(ns);
   for (int k = 0; k < ns; ++k) {
     dirs[k] = "SAMPLE_" + std::to_string(S.sample_list[k]);
+    // a run without -r starts from an empty sample directory (src/fv.cc:226-231)
+    if (!restart) {
+      std::error_code ec;
+      std::filesystem::remove_all(dirs[k], ec);
+    }
     if (!mkdirs(dirs[k])) return 1;
     // initial state: restart file or expressions (src/fv.cc:300-402)
     bool from_restart = false;
@@ -468,9 +509,22 @@ int main(int argc, char** argv) {
         std::ifstream rf(dirs[k] + "/restart_" + std::to_string(h) + "_0.dat");
         double wz;
         for (int i = 0; i < nv; ++i) rf >> W[4 * i] >> W[4 * i + 1] >> W[4 * i + 2] >> wz >> W[4 * i + 3];
-        if (!rf) { std::cerr << "ERROR --> bad restart file in " << dirs[k] << std::endl; return 1; }
+        rc_clock[k].valid = (bool)(rf >> rc_clock[k].iter >> rc_clock[k].counter >> rc_clock[k].t >> rc_clock[k].res0);
+        if (!rc_clock[k].valid) { std::cerr << "ERROR --> bad restart file in " << dirs[k] << std::endl; return 1; }
         from_restart = true;
-        std::cout << "    Restart state of " << dirs[k] << " read (the run restarts its clock at t = 0)" << std::endl;
+        const Clock& q = rc_clock[k];
+        if (q.t >= S.p.final_time)
+          std::cout << "    SAMPLE " << S.sample_list[k] << " has already reached final time!!" << std::endl;
+        else if (q.iter >= S.p.max_iter && steady)
+          std::cout << "    SAMPLE " << S.sample_list[k] << " has already max iteration count!!" << std::endl;
+        else if (q.t > 0.0) {
+          // the logs keep the lines of the steps up to the restart (src/fv.cc:350-380)
+          if (S.write_log) truncate_log(dirs[k] + "/residual.dat", q.iter);
+          if (!S.forces.empty()) truncate_log(dirs[k] + "/force.dat", q.iter);
+          if (S.has_global) truncate_log(dirs[k] + "/global.dat", q.iter);
+        }
+      } else {
+        std::cout << "    Restart file(s) for " << dirs[k] << " does not exist. Staring from T=0 ..." << std::endl;
       }
     }
     if (!from_restart) {
@@ -495,12 +549,16 @@ int main(int argc, char** argv) {
         }
     }
     if (tsm_set_state(ctx, k, W.data()) != TSM_OK) { std::cerr << "ERROR --> " << tsm_last_error() << std::endl; return 1; }
+    if (rc_clock[k].valid &&
+        tsm_set_clock(ctx, k, rc_clock[k].iter, rc_clock[k].counter, rc_clock[k].t, rc_clock[k].res0) != TSM_OK) {
+      std::cerr << "ERROR --> restart clock of " << dirs[k] << ": " << tsm_last_error() << std::endl;
+      return 1;
+    }
     if (!restart) std::ofstream(dirs[k] + "/master_file.visit") << "!NBLOCKS 1" << std::endl;
     if (S.write_log) res_f[k].open(dirs[k] + "/residual.dat", std::ios::app);
     if (!S.forces.empty()) force_f[k].open(dirs[k] + "/force.dat", std::ios::app);
     if (S.has_global) glob_f[k].open(dirs[k] + "/global.dat", std::ios::app);
   }
-  const bool steady = S.time_mode == "steady";
   // a steady run lists only its final solution in the master file and keeps
   // overwriting sol0001 every write_frequency steps (src/fv.cc:2241-2245, :2318-2323, :2371-2377)
   WriterPool writer(std::min(ns, std::max(1, std::min(8, (int)std::thread::hardware_concurrency() - 1))));
@@ -610,13 +668,15 @@ int main(int argc, char** argv) {
     }
   };
   for (int k = 0; k < ns; ++k) {
-    if (S.write_soln) output(k, 0, 0.0, !(steady && restart));
+    if (S.write_soln && !restart) output(k, 0, 0.0);  // src/fv.cc:2240-2245
     keep(k, 0);
   }
   // time loop, all samples together; outputs as each reaches its stamps
   std::vector<double> logbuf(4 * (size_t)ns);
   std::vector<long> done(ns);
   std::vector<int> cnt_before(ns, 1), finished(ns, 0);
+  for (int k = 0; k < ns; ++k)
+    if (rc_clock[k].valid) cnt_before[k] = std::max(1, rc_clock[k].counter);
   int live = ns;
   while (live > 0) {
     if (tsm_run(ctx, 1, done.data(), logbuf.data()) != TSM_OK) {
@@ -670,12 +730,14 @@ int main(int argc, char** argv) {
         }
         if (S.write_restart && (iter % S.restart_frequency == 0 || v[0] == S.p.final_time)) {
           tsm_get_state(ctx, k, W.data());
-          writer.post(k, [nv, dir = dirs[k], h = head[k], iter, counter, t = v[0], w = W]() {
+          double res0 = 1.0;
+          tsm_get_clock(ctx, k, nullptr, nullptr, nullptr, &res0);
+          writer.post(k, [nv, dir = dirs[k], h = head[k], iter, counter, t = v[0], res0, w = W]() {
             std::ofstream fo(dir + "/restart_" + std::to_string(h) + "_0.dat");
             for (int i = 0; i < nv; ++i)
               fo << std::scientific << std::setprecision(15) << w[4 * i] << "  " << w[4 * i + 1] << "  "
                  << w[4 * i + 2] << "  " << 0.0 << "  " << w[4 * i + 3] << std::endl;
-            fo << iter << std::endl << counter << std::endl << t << std::endl << 1.0 << std::endl;
+            fo << iter << std::endl << counter << std::endl << t << std::endl << res0 << std::endl;
             const bool good = (bool)fo;
             fo.close();
             std::ofstream(dir + "/LAST_RESTART_HEAD.txt") << h << std::endl;

@@ -931,6 +931,33 @@ extern "C" int tsm_time(tsm_ctx* c, int s, double* t, long* iter) {
   return TSM_OK;
 }
 
+extern "C" int tsm_get_clock(tsm_ctx* c, int s, long* iter, int* counter, double* t, double* res0) {
+  if (!c || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  tsm::Ctl q;
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(&q, c->A.ctl + s, sizeof(q), hipMemcpyDeviceToHost));
+  if (iter) *iter = (long)q.iter;
+  if (counter) *counter = q.counter;
+  if (t) *t = q.t;
+  if (res0) *res0 = q.res0;
+  return TSM_OK;
+}
+
+extern "C" int tsm_set_clock(tsm_ctx* c, int s, long iter, int counter, double t, double res0) {
+  if (!c || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  if (iter < 0 || counter < 0 || counter > c->P.n_time_stamps + 1 || !(t >= 0.0) || !std::isfinite(t) ||
+      !(res0 > 0.0) || !std::isfinite(res0))
+    return fail(TSM_E_ARG, "clock: iter and counter must be >= 0 (counter <= time stamps + 1), t finite "
+                           "and >= 0, res0 finite and > 0");
+  tsm::Ctl q;
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(&q, c->A.ctl + s, sizeof(q), hipMemcpyDeviceToHost));
+  q.iter = iter; q.counter = counter; q.t = t; q.res0 = res0;
+  q.res_total = 1e20; q.reached = 0; q.done = 0;
+  HIP_TRY(hipMemcpy(c->A.ctl + s, &q, sizeof(q), hipMemcpyHostToDevice));
+  return TSM_OK;
+}
+
 extern "C" int tsm_status(tsm_ctx* c, int s, double* v, long* iter, int* counter, int* done) {
   if (!c || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
   tsm::Ctl q;

@@ -57,6 +57,8 @@ def lib():
         L.orc_set_bc_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_forces.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_stage_gradients.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_get_clock.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+        L.orc_set_clock.argtypes = [C.c_void_p, C.c_long, C.c_int, C.c_double, C.c_double]
         L.orc_surface.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_run.restype = C.c_long
         L.orc_run.argtypes = [C.c_void_p, C.c_long, C.c_void_p]
@@ -155,6 +157,15 @@ class Oracle:
         out = np.empty(4)
         lib().orc_forces(self._h, int(force_index), _p(t), t.size, _p(out))
         return out
+
+    def clock(self):
+        """(iter, counter, t, res0): the loop clock a restart file carries."""
+        it, cn, t, r0 = C.c_long(), C.c_int(), C.c_double(), C.c_double()
+        lib().orc_get_clock(self._h, C.byref(it), C.byref(cn), C.byref(t), C.byref(r0))
+        return it.value, cn.value, t.value, r0.value
+
+    def set_clock(self, it, counter, t, res0):
+        lib().orc_set_clock(self._h, int(it), int(counter), float(t), float(res0))
 
     def stage_gradients(self):
         a = np.empty((self.nv, 8))

@@ -530,3 +530,58 @@ def test_driver_strong_boundary_conditions_match_oracle(tmp_path):
     t, W = T.read_raw(str(sd / "sol0001_0.raw"))
     assert t == pytest.approx(0.05, rel=1e-14)
     assert rel_l2(W, o.get_state()) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_driver_restart_continues_the_run(tmp_path):
+    """-r (src/fv.cc:300-402): a finished run pointed back at its previous
+    restart file continues from that file's state and clock; the logs keep the
+    lines up to its iteration (awk '$1 <= iter', src/fv.cc:350-380), the
+    continued lines, the later outputs and the final state equal the
+    uninterrupted run's, no initial output is written, and the master file
+    gets the continued outputs appended."""
+    mesh = vortex_mesh(17)
+    write_part_file(mesh, str(tmp_path), periodic_tags=(1, 2, 3, 4))
+    text = VORTEX_PARAM.format(final_time=1.0, stamps=2).replace("restart_frequency     10",
+                                                                 "restart_frequency     7")
+    (tmp_path / "param.in").write_text(text)
+    run = lambda *a: subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", ".", *a],
+                                    cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    r = run()
+    assert r.returncode == 0, r.stderr
+    sd = tmp_path / "SAMPLE_0"
+    full = {f: (sd / f).read_text() for f in ("residual.dat", "global.dat", "sol0002_0.vtk",
+                                             "master_file.visit")}
+    nres = len(full["residual.dat"].splitlines())
+    head = int((sd / "LAST_RESTART_HEAD.txt").read_text())
+    other = 1 - head
+    tail = (sd / f"restart_{other}_0.dat").read_text().split()[5 * mesh.n_vertex:]
+    it0, cnt0, t0, res0 = int(tail[0]), int(tail[1]), float(tail[2]), float(tail[3])
+    assert 0 < it0 < nres and it0 % 7 == 0 and 0 < t0 < 1.0 and res0 > 0 and res0 != 1.0
+    (sd / "LAST_RESTART_HEAD.txt").write_text(f"{other}\n")
+    (sd / "sol0000_0.vtk").unlink()
+    r = run("-r")
+    assert r.returncode == 0, r.stderr
+    assert not (sd / "sol0000_0.vtk").exists()
+    a = np.loadtxt(sd / "residual.dat")
+    b = np.array([[float(v) for v in l.split()] for l in full["residual.dat"].splitlines()])
+    assert a.shape == b.shape and np.array_equal(a[:it0], b[:it0])
+    np.testing.assert_allclose(a[it0:], b[it0:], rtol=1e-9)
+    assert len((sd / "global.dat").read_text().splitlines()) == nres
+    va = _vtk_scalars(sd / "sol0002_0.vtk", "pressure", mesh.n_vertex)
+    vb = _vtk_scalars_text(full["sol0002_0.vtk"], "pressure", mesh.n_vertex)
+    np.testing.assert_allclose(va, vb, rtol=1e-12)
+    master = (sd / "master_file.visit").read_text().split()
+    assert master == full["master_file.visit"].split() + [f"sol000{k}_0.vtk" for k in range(cnt0, 3)]
+    # a run without -r starts from an empty sample directory
+    (sd / "stray.txt").write_text("x")
+    r = run()
+    assert r.returncode == 0, r.stderr
+    assert not (sd / "stray.txt").exists()
+    assert (sd / "residual.dat").read_text() == full["residual.dat"]
+
+
+def _vtk_scalars_text(text, name, n):
+    lines = text.splitlines()
+    i = lines.index(f"SCALARS {name} double 1")
+    return np.array([float(v) for v in lines[i + 2:i + 2 + n]])
