@@ -213,6 +213,11 @@ int tsm_status(tsm_ctx* ctx, int sample, double* v, long* iter, int* counter, in
    is normalised by.  tsm_set_clock continues a run from a restart: after
    tsm_set_state it sets the clock, and the next step starts with the loop's
    initial res_total = 1e20 (src/fv.cc:2234); iter > 0 keeps res0. */
+/* Solution bounds of the current state (FiniteVolume::compute_bounds,
+   src/fv.cc:2099-2171, the -b option): out[10] = min and max of T, p, u, v and
+   of the z velocity (zero in two dimensions), in the reference's column order
+   TEMP_MIN, TEMP_MAX, P_MIN, P_MAX, U_MIN, U_MAX, V_MIN, V_MAX, W_MIN, W_MAX. */
+int tsm_bounds(tsm_ctx* ctx, int sample, double* out);
 int tsm_get_clock(tsm_ctx* ctx, int sample, long* iter, int* counter, double* t, double* res0);
 int tsm_set_clock(tsm_ctx* ctx, int sample, long iter, int counter, double t, double res0);
 /* Residual norms of the sample's last step before normalisation, as the

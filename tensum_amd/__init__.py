@@ -198,6 +198,7 @@ def lib():
         L.tsm_forces.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_keep_stage_state.argtypes = [C.c_void_p, C.c_int]
         L.tsm_get_clock.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
+        L.tsm_bounds.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_set_clock.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_int, C.c_double, C.c_double]
         L.tsm_stage_gradients.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_surface.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
@@ -415,6 +416,12 @@ class Solver:
         out = np.zeros(4)
         _check(lib().tsm_forces(self._h, sample, int(force_index), t.ctypes.data_as(C.c_void_p),
                                 t.size, _dptr(out)))
+        return out
+
+    def bounds(self, sample=0):
+        """(10,) minima and maxima of T, p, u, v and w (zero) of the current state."""
+        out = np.zeros(10)
+        _check(lib().tsm_bounds(self._h, sample, _dptr(out)))
         return out
 
     def clock(self, sample=0):

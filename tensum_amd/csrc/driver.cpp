@@ -310,7 +310,7 @@ struct Moments {  // src/fv.cc:595-694 (primitive fields + density/mach/entropy)
 int main(int argc, char** argv) {
   std::string ifile, loc;
   int npart = -1;
-  bool restart = false, check_only = false;
+  bool restart = false, check_only = false, bounds = false;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     if (a == "-i" && i + 1 < argc) ifile = argv[++i];
@@ -318,7 +318,8 @@ int main(int argc, char** argv) {
     else if (a == "-L" && i + 1 < argc) loc = argv[++i];
     else if (a == "-r") restart = true;
     else if (a == "--check") { check_only = true; npart = npart < 1 ? 1 : npart; loc = loc.empty() ? "." : loc; }
-    else if (a == "-p" || a == "-b" || a == "-v" || a == "-d") {}
+    else if (a == "-b") bounds = true;
+    else if (a == "-p" || a == "-v" || a == "-d") {}
     else if (a == "-axi") { std::cerr << "ERROR --> -axi is not supported (2-D only)" << std::endl; return 1; }
     else { std::cerr << "Unknown command line flag: " << a << std::endl; ifile.clear(); break; }
   }
@@ -326,7 +327,8 @@ int main(int argc, char** argv) {
     std::cout << "Valid flags are:\n   -i filename   Specify input file name (required)\n"
                  "   -npart N      Specify number of mesh partitions N (required, 1)\n"
                  "   -L path       Location of mesh partition folder (required)\n"
-                 "   -r            Restart from SAMPLE_<id>/restart files\n";
+                 "   -r            Restart from SAMPLE_<id>/restart files\n"
+                 "   -b            Write the solution bounds to SAMPLE_<id>/bounds.dat\n";
     return 1;
   }
   if (npart != 1) { std::cerr << "ERROR --> one GPU holds the whole mesh: use -npart 1" << std::endl; return 1; }
@@ -480,7 +482,7 @@ int main(int argc, char** argv) {
   std::vector<double> dcarea(nv), mcarea(nv);
   tsm_geometry(ctx, nullptr, nullptr, dcarea.data(), mcarea.data());
   std::vector<std::string> dirs(ns);
-  std::vector<std::ofstream> res_f(ns), glob_f(ns), force_f(ns);
+  std::vector<std::ofstream> res_f(ns), glob_f(ns), force_f(ns), bnd_f(ns);
   // every force's tags must name boundary faces of the mesh (src/force.cc:25-106)
   for (auto& fo : S.forces)
     for (int t : fo.second)
@@ -556,6 +558,16 @@ int main(int argc, char** argv) {
     }
     if (!restart) std::ofstream(dirs[k] + "/master_file.visit") << "!NBLOCKS 1" << std::endl;
     if (S.write_log) res_f[k].open(dirs[k] + "/residual.dat", std::ios::app);
+    if (bounds) {  // -b: solution bounds after every logged step (src/fv.cc:258-278)
+      bnd_f[k].open(dirs[k] + "/bounds.dat", std::ios::app);
+      if (!restart) {
+        bnd_f[k] << std::left << std::setw(16 * 11) << std::setfill('-') << "" << std::endl;
+        for (const char* h : {"TIME", "TEMP_MIN", "TEMP_MAX", "P_MIN", "P_MAX", "U_MIN", "U_MAX", "V_MIN",
+                              "V_MAX", "W_MIN", "W_MAX"})
+          bnd_f[k] << std::left << std::setw(17) << std::setfill(' ') << h;
+        bnd_f[k] << std::endl << std::left << std::setw(16 * 11) << std::setfill('-') << "" << std::endl;
+      }
+    }
     if (!S.forces.empty()) force_f[k].open(dirs[k] + "/force.dat", std::ios::app);
     if (S.has_global) glob_f[k].open(dirs[k] + "/global.dat", std::ios::app);
   }
@@ -700,6 +712,23 @@ int main(int argc, char** argv) {
         } else if (S.write_log) {
           res_f[k] << std::setw(10) << (long)L[0] << "  " << std::scientific << std::setprecision(12)
                    << L[1] << "  " << L[2] << "  " << L[3] << std::endl;
+        }
+        if (bounds && S.write_log) {  // src/fv.cc:1971-1972, :2099-2171
+          double b[10];
+          if (tsm_bounds(ctx, k, b) != TSM_OK) {
+            std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
+            return 1;
+          }
+          bnd_f[k] << std::left << std::setw(17) << std::setfill(' ') << std::setprecision(10) << L[2];
+          for (double x : b) bnd_f[k] << std::left << std::setw(17) << std::setfill(' ') << std::setprecision(10) << x;
+          bnd_f[k] << std::endl;
+          if (b[0] < 0.0 || b[2] < 0.0) {
+            if (S.write_soln) output(k, counter, L[2]);
+            writer.finish();
+            std::cerr << "ERROR --> Error:: Observed negative temperature/pressure in solution (" << dirs[k]
+                      << ")" << std::endl;
+            return 1;
+          }
         }
         if (!S.forces.empty()) {  // src/force.cc:118-217, after each step
           std::ofstream& ff = force_f[k];

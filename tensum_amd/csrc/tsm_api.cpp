@@ -56,6 +56,7 @@ struct tsm_ctx {
   int* force_sel = nullptr;
   size_t force_cap = 0;
   double* W_keep = nullptr;     // last stage's input states (tsm_keep_stage_state)
+  double* bounds_buf = nullptr; // per-block minima and maxima (tsm_bounds)
   std::vector<int> h_btype;
   double* red = nullptr;   // reduction partials for global / error norms
   size_t red_cap = 0;
@@ -928,6 +929,32 @@ extern "C" int tsm_time(tsm_ctx* c, int s, double* t, long* iter) {
   HIP_TRY(hipMemcpy(&q, c->A.ctl + s, sizeof(q), hipMemcpyDeviceToHost));
   if (t) *t = q.t;
   if (iter) *iter = (long)q.iter;
+  return TSM_OK;
+}
+
+extern "C" int tsm_bounds(tsm_ctx* c, int s, double* out) {
+  if (!c || !out || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  const int nblk = std::max(1, std::min(1024, (c->G.nv + 255) / 256));
+  if (!c->bounds_buf) {
+    int rc = c->alloc(&c->bounds_buf, 8 * 1024);
+    if (rc) return rc;
+  }
+  c->csr_ready();
+  tsm::launch_bounds(c->G, c->A, s, c->bounds_buf, nblk, c->st);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> part(8 * (size_t)nblk);
+  HIP_TRY(hipMemcpyAsync(part.data(), c->bounds_buf, part.size() * sizeof(double),
+                         hipMemcpyDeviceToHost, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  // T, p, u, v as (min, max); w is zero in two dimensions
+  double r[8] = {1.0e20, -1.0e20, 1.0e20, -1.0e20, 1.0e20, -1.0e20, 1.0e20, -1.0e20};
+  for (int b = 0; b < nblk; ++b)
+    for (int k = 0; k < 4; ++k) {
+      r[2 * k] = std::min(r[2 * k], part[8 * (size_t)b + 2 * k]);
+      r[2 * k + 1] = std::max(r[2 * k + 1], part[8 * (size_t)b + 2 * k + 1]);
+    }
+  for (int k = 0; k < 8; ++k) out[k] = r[k];
+  out[8] = out[9] = 0.0;
   return TSM_OK;
 }
 

@@ -299,6 +299,9 @@ void launch_gradient(const DevGeo& G, const StageArgs& A, int S, hipStream_t st)
 void launch_gradient_one(const DevGeo& G, const StageArgs& A, int s, hipStream_t st);
 void launch_surface(const DevGeo& G, const StageArgs& A, int s, const int* sel, int nsel,
                     double* out, hipStream_t st);
+// per-block minima and maxima of T, p, u, v of sample s (8 doubles per block)
+void launch_bounds(const DevGeo& G, const StageArgs& A, int s, double* out, int nblocks,
+                   hipStream_t st);
 // strong boundary conditions on the stage output A.W_out of every live sample
 void launch_strong_bc(const DevGeo& G, const StageArgs& A, int S, hipStream_t st);
 void launch_dt(const DevGeo& G, const StageArgs& A, const StepParams& P, int S, hipStream_t st);

@@ -738,6 +738,43 @@ void launch_strong_bc(const DevGeo& G, const StageArgs& A, int S, hipStream_t st
     hipLaunchKernelGGL(k_strong_bc, dim3((G.nsbv + 255) / 256, S), dim3(256), 0, st, G, A);
 }
 
+// Solution bounds (src/fv.cc:2099-2171): per block the minimum and maximum of
+// T, p, u, v over its vertices of sample s, out[8 * block + 2q] = min,
+// out[8 * block + 2q + 1] = max for q = T, p, u, v; the host folds the blocks.
+__global__ __launch_bounds__(256) void k_bounds(DevGeo G, StageArgs A, int s, double* out) {
+  __shared__ double red[8][256];
+  const size_t off = (size_t)s * G.nv;
+  double lo[4] = {1.0e20, 1.0e20, 1.0e20, 1.0e20}, hi[4] = {-1.0e20, -1.0e20, -1.0e20, -1.0e20};
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < G.nv; v += gridDim.x * blockDim.x) {
+    const Prim w = load_prim(A.W_in, off + v);
+    const double q[4] = {w.T, w.p, w.u, w.v};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {  // std::min / std::max: a NaN never replaces the running value
+      lo[k] = q[k] < lo[k] ? q[k] : lo[k];
+      hi[k] = hi[k] < q[k] ? q[k] : hi[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) { red[2 * k][threadIdx.x] = lo[k]; red[2 * k + 1][threadIdx.x] = hi[k]; }
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double a = red[2 * k][threadIdx.x + h], b = red[2 * k + 1][threadIdx.x + h];
+        if (a < red[2 * k][threadIdx.x]) red[2 * k][threadIdx.x] = a;
+        if (red[2 * k + 1][threadIdx.x] < b) red[2 * k + 1][threadIdx.x] = b;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) out[8 * (size_t)blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+void launch_bounds(const DevGeo& G, const StageArgs& A, int s, double* out, int nblocks,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_bounds, dim3(nblocks), dim3(256), 0, st, G, A, s, out);
+}
+
 // Liou carbuncle sensor (src/compute_ssw.cc:7-39)
 __global__ void k_ssw_zero(DevGeo G, StageArgs A) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

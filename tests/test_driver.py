@@ -585,3 +585,32 @@ def _vtk_scalars_text(text, name, n):
     lines = text.splitlines()
     i = lines.index(f"SCALARS {name} double 1")
     return np.array([float(v) for v in lines[i + 2:i + 2 + n]])
+
+
+@pytest.mark.gpu
+def test_driver_bounds_file(tmp_path):
+    """-b (src/fv.cc:258-278, :2152-2163): bounds.dat has the reference's
+    header and one line per step, the time and the extremes of T, p, u, v, w
+    at 10 digits; the last line is the final state's."""
+    mesh = vortex_mesh(17)
+    write_part_file(mesh, str(tmp_path), periodic_tags=(1, 2, 3, 4))
+    text = VORTEX_PARAM.format(final_time=0.3, stamps=1).replace("format                vtk",
+                                                                 "format                raw")
+    (tmp_path / "param.in").write_text(text)
+    r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", ".", "-b"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    sd = tmp_path / "SAMPLE_0"
+    lines = (sd / "bounds.dat").read_text().splitlines()
+    assert lines[0] == "-" * 176 and lines[2] == "-" * 176
+    assert lines[1].split() == ["TIME", "TEMP_MIN", "TEMP_MAX", "P_MIN", "P_MAX", "U_MIN", "U_MAX",
+                                "V_MIN", "V_MAX", "W_MIN", "W_MAX"]
+    assert all(len(l) == 17 * 11 for l in lines[1:2])
+    res = np.loadtxt(sd / "residual.dat")
+    rows = np.array([[float(v) for v in l.split()] for l in lines[3:]])
+    assert rows.shape == (res.shape[0], 11)
+    np.testing.assert_allclose(rows[:, 0], res[:, 2], rtol=1e-9)
+    t, W = T.read_raw(str(sd / "sol0001_0.raw"))
+    want = [t, W[:, 0].min(), W[:, 0].max(), W[:, 3].min(), W[:, 3].max(), W[:, 1].min(),
+            W[:, 1].max(), W[:, 2].min(), W[:, 2].max(), 0.0, 0.0]
+    np.testing.assert_allclose(rows[-1], want, rtol=1e-9, atol=1e-12)

@@ -78,3 +78,19 @@ def test_bad_clocks_are_refused():
             s.set_clock(*bad)
     s.set_clock(3, 2, 0.6, 1.0)      # at the final time: the run is over
     assert s.run(5)[0] == 0
+
+
+def test_bounds_are_the_state_extremes():
+    """tsm_bounds (src/fv.cc:2099-2171) equals numpy's extremes of the state,
+    exactly, for each sample of a batch and a mesh of more vertices than one
+    pass of its reduction blocks covers."""
+    m = vortex_mesh(601)
+    s = T.Solver(m, CASES["tiled"], n_samples=2)
+    for k in range(2):
+        s.set_state(vortex(m.x, m.y) * (1.0 + 0.1 * k), k)
+    s.run(3)
+    for k in range(2):
+        W = s.get_state(k)
+        want = [W[:, 0].min(), W[:, 0].max(), W[:, 3].min(), W[:, 3].max(), W[:, 1].min(),
+                W[:, 1].max(), W[:, 2].min(), W[:, 2].max(), 0.0, 0.0]
+        assert np.array_equal(s.bounds(k), np.array(want))
