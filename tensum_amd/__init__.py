@@ -238,6 +238,79 @@ def _dptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def read_part_file(path, bc):
+    """Mesh from an NPART = 1 part file (PARTITION/part_file_0.dat, layout of
+    readme/README.md:155-201, read as src/read_part_mesh_file.cc:43-214 reads
+    it): nodes with their global ids, periodic sets, boundary faces (tag, two
+    gids, periodic flag), cells (three gids, ghost flag).  Global ids map to
+    0-based vertices in file order; bc is the boundary table {tag: (type, ...)}
+    as for Mesh.  Ghost cells (NPART > 1) are refused."""
+    with open(path) as f:
+        tok = f.read().split()
+    pos = 0
+
+    def take(n=1):
+        nonlocal pos
+        out = tok[pos:pos + n]
+        if len(out) < n:
+            raise TsmError(f"part file truncated: {path}")
+        pos += n
+        return out
+
+    def expect(word):
+        w = take()[0]
+        if w != word:
+            raise TsmError(f"part file {path}: {word} expected, found {w}")
+
+    expect("DIMENSION")
+    if int(take()[0]) != 2:
+        raise TsmError("part file: DIMENSION 2 expected")
+    expect("PARTITIONS")
+    if int(take()[0]) != 1:
+        raise TsmError("part file: one partition expected (NPART = 1)")
+    expect("NODES")
+    n = int(take()[0])
+    expect("OF")
+    if int(take()[0]) != n or n <= 0:
+        raise TsmError("part file: NODES")
+    gid = np.empty(n, dtype=np.int64)
+    x, y = np.empty(n), np.empty(n)
+    for i in range(n):
+        g, xi, yi, _z, ns = take(5)
+        gid[i], x[i], y[i] = int(g), float(xi), float(yi)
+        take(int(ns) + 1)  # owning partitions, boundary flag
+    g2l = {int(g): i for i, g in enumerate(gid)}
+    expect("PERIODIC")
+    ptr, mem = [0], []
+    for _ in range(int(take()[0])):
+        _id, cnt = take(2)
+        mem += [g2l[int(v)] for v in take(int(cnt))]
+        take(int(take()[0]))  # partitions of the set
+        ptr.append(len(mem))
+    expect("FACES")
+    nb = int(take()[0])
+    expect("OF")
+    take()
+    btag, bv = np.empty(nb, dtype=np.int32), np.empty(2 * nb, dtype=np.int32)
+    for k in range(nb):
+        t, a, b, _per = take(4)
+        btag[k], bv[2 * k], bv[2 * k + 1] = int(t), g2l[int(a)], g2l[int(b)]
+    expect("CELLS")
+    nc = int(take()[0])
+    expect("OF")
+    take()
+    cv = np.empty(3 * nc, dtype=np.int32)
+    for k in range(nc):
+        a, b, c, ghost = take(4)
+        if int(ghost):
+            raise TsmError("part file: ghost cells need NPART > 1")
+        cv[3 * k:3 * k + 3] = (g2l[int(a)], g2l[int(b)], g2l[int(c)])
+    if len(ptr) == 1:  # no periodic sets: empty arrays, as jittered_mesh makes them
+        ptr = []
+    return Mesh(x, y, btag, bv, cv, np.asarray(ptr, dtype=np.int32), np.asarray(mem, dtype=np.int32),
+                bc)
+
+
 def jittered_mesh(n, box=(0.0, 1.0, 0.0, 1.0), jitter=0.2, seed=0, periodic=False, bc=None):
     """Seeded jittered-triangle n x n lattice from libtsm's generator."""
     L = lib()
