@@ -97,16 +97,26 @@ struct tsm_ctx {
   // Makes W[cur] the state of the CSR kernels' double buffer (W[0]/W[1]) and
   // refreshes the entropy variables they read.
   void csr_ready() {
+    state_ready();
+    if (!e_valid) {
+      for (int s = 0; s < S; ++s) tsm::launch_prim2ent(G, A, s, st);
+      e_valid = true;
+    }
+  }
+  // W[cur] in the CSR kernels' double buffer, entropy variables as they are
+  void state_ready() {
     if (cur == 2) {
       (void)hipMemcpyAsync(W[0], W[2], 4 * A.cs * sizeof(double), hipMemcpyDeviceToDevice, st);
       cur = 0;
       e_valid = false;
     }
     bind_state();
-    if (!e_valid) {
-      for (int s = 0; s < S; ++s) tsm::launch_prim2ent(G, A, s, st);
-      e_valid = true;
-    }
+  }
+  // ... with sample s's entropy variables current (single-sample queries: the
+  // other samples' stay as they are, so a batch of queries costs one sample each)
+  void sample_ready(int s) {
+    state_ready();
+    if (!e_valid) tsm::launch_prim2ent(G, A, s, st);
   }
 };
 
@@ -567,9 +577,9 @@ extern "C" int tsm_get_state(tsm_ctx* c, int s, double* prim) {
 
 extern "C" int tsm_gradients(tsm_ctx* c, int s, double* dE) {
   if (!c || !dE || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
-  c->csr_ready();
+  c->sample_ready(s);
   c->A.ignore_done = 1;
-  tsm::launch_gradient(c->G, c->A, c->S, c->st);
+  tsm::launch_gradient_one(c->G, c->A, s, c->st);
   c->A.ignore_done = 0;
   HIP_TRY(hipGetLastError());
   return download_vec(c, c->A.dE, 8, s, dE);
@@ -593,11 +603,11 @@ extern "C" int tsm_forces(tsm_ctx* c, int s, int force_index, const int* tags, i
     if (rc) return rc;
     c->force_cap = sel.size();
   }
-  c->csr_ready();
+  c->sample_ready(s);
   HIP_TRY(hipMemcpyAsync(c->force_sel, sel.data(), sel.size() * sizeof(int), hipMemcpyHostToDevice,
                          c->st));
   c->A.ignore_done = 1;
-  if (c->P.model == 1) tsm::launch_gradient(c->G, c->A, c->S, c->st);
+  if (c->P.model == 1) tsm::launch_gradient_one(c->G, c->A, s, c->st);
   c->A.ignore_done = 0;
   tsm::launch_forces(c->G, c->A, s, force_index, c->force_sel, (int)sel.size(), c->P.model == 1,
                      c->force_buf + 4, c->force_buf, c->st);
@@ -614,7 +624,7 @@ extern "C" int tsm_keep_stage_state(tsm_ctx* c, int on) {
       int rc = c->alloc(&c->W_keep, 4 * c->A.cs);
       if (rc) return rc;
     }
-    c->csr_ready();
+    c->state_ready();
     HIP_TRY(hipMemcpyAsync(c->W_keep, c->W[c->cur], 4 * c->A.cs * sizeof(double),
                            hipMemcpyDeviceToDevice, c->st));
     c->A.W_keep = c->W_keep;
@@ -633,7 +643,8 @@ namespace {
 int sample_gradients(tsm_ctx* c, int s, bool stage) {
   if (stage && !c->A.W_keep)
     return fail(TSM_E_ARG, "the last stage's state is not kept (tsm_keep_stage_state)");
-  c->csr_ready();
+  if (stage) c->state_ready();
+  else c->sample_ready(s);
   tsm::StageArgs K = c->A;
   K.ignore_done = 1;
   if (stage) {  // entropy variables of the kept state in the scratch entropy buffer
@@ -939,7 +950,7 @@ extern "C" int tsm_bounds(tsm_ctx* c, int s, double* out) {
     int rc = c->alloc(&c->bounds_buf, 8 * 1024);
     if (rc) return rc;
   }
-  c->csr_ready();
+  c->state_ready();
   tsm::launch_bounds(c->G, c->A, s, c->bounds_buf, nblk, c->st);
   HIP_TRY(hipGetLastError());
   std::vector<double> part(8 * (size_t)nblk);

@@ -6,6 +6,7 @@ import pytest
 
 import tensum_amd as T
 import oracle_binding as O
+from cases import rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -52,3 +53,27 @@ def test_bad_force_requests_are_refused():
         s.forces([99])
     with pytest.raises(T.TsmError):
         s.forces([1], force_index=10 ** 6)
+
+
+@pytest.mark.parametrize("path", ["auto", "csr"])
+def test_single_sample_queries_of_a_batch(path):
+    """Per-sample queries after batched steps (each forms only its own
+    sample's entropy variables and gradients): gradients, Navier-Stokes
+    forces and current-state surfaces of every sample match its own oracle."""
+    kw, bc = CASES["ns_walls"]
+    m = T.jittered_mesh(25, seed=14, bc=bc)
+    prm = T.Params(cfl=0.5, final_time=10.0, kernel_path=path, **kw)
+    amps = (0.5, 1.0, 1.5)
+    s = T.Solver(m, prm, n_samples=len(amps))
+    for k, a in enumerate(amps):
+        s.set_state(_bump(m) * np.array([1.0, a, 1.0, 1.0]), k)
+    s.run(12)
+    wall = int(np.flatnonzero(m.bface_tag == 1)[2])
+    for k in (2, 0, 1):
+        o = O.Oracle(m, prm)
+        o.set_state(_bump(m) * np.array([1.0, amps[k], 1.0, 1.0]))
+        o.run(12)
+        assert rel_l2(s.gradients(k), o.gradients()) <= 1e-10
+        np.testing.assert_allclose(s.forces([1, 3], sample=k, force_index=wall),
+                                   o.forces([1, 3], force_index=wall), rtol=1e-10, atol=1e-14)
+        assert rel_l2(s.surface([1, 3], sample=k, stage=False), o.surface([1, 3])) <= 1e-10
