@@ -207,6 +207,10 @@ int tsm_time(tsm_ctx* ctx, int sample, double* t, long* iter);
    res_total; iter; counter = index of the next output time stamp
    (src/fv.cc:2324-2343); done = the sample has left its time loop. */
 int tsm_status(tsm_ctx* ctx, int sample, double* v, long* iter, int* counter, int* done);
+/* tsm_status and tsm_residual_norms of every sample in one transfer (a driver
+   stepping a batch reads them once per step): v[3*s..], iter[s], counter[s],
+   done[s], norms[4*s..]; any of them may be NULL. */
+int tsm_status_all(tsm_ctx* ctx, double* v, long* iter, int* counter, int* done, double* norms);
 /* The loop clock a restart file carries after the state (src/writer.cc:404-407,
    read back by src/fv.cc:334-338): the last iteration, the stamp counter, the
    elapsed time and res0, the first step's residual norm that every later one

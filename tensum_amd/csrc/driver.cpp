@@ -687,6 +687,9 @@ int main(int argc, char** argv) {
   std::vector<double> logbuf(4 * (size_t)ns);
   std::vector<long> done(ns);
   std::vector<int> cnt_before(ns, 1), finished(ns, 0);
+  std::vector<double> st_v(3 * (size_t)ns), st_rn(4 * (size_t)ns), g_ke(ns), g_ent(ns);
+  std::vector<long> st_iter(ns);
+  std::vector<int> st_cnt(ns), st_done(ns);
   for (int k = 0; k < ns; ++k)
     if (rc_clock[k].valid) cnt_before[k] = std::max(1, rc_clock[k].counter);
   int live = ns;
@@ -695,17 +698,21 @@ int main(int argc, char** argv) {
       std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
       return 1;
     }
+    // every sample's loop state, and its integrals, once per step
+    if (tsm_status_all(ctx, st_v.data(), st_iter.data(), st_cnt.data(), st_done.data(), st_rn.data()) != TSM_OK ||
+        (S.has_global && tsm_global_all(ctx, g_ke.data(), g_ent.data()) != TSM_OK)) {
+      std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
+      return 1;
+    }
     for (int k = 0; k < ns; ++k) {
       if (finished[k]) continue;
-      double v[3];
-      long iter;
-      int counter, dn;
-      tsm_status(ctx, k, v, &iter, &counter, &dn);
+      const double* v = &st_v[3 * (size_t)k];
+      const long iter = st_iter[k];
+      const int counter = st_cnt[k], dn = st_done[k];
       if (done[k] > 0) {
         const double* L = &logbuf[4 * (size_t)k];
         if (S.write_log && steady) {  // src/fv.cc:1944-1957
-          double rn[4];
-          tsm_residual_norms(ctx, k, rn);
+          const double* rn = &st_rn[4 * (size_t)k];
           res_f[k] << std::setw(10) << (long)L[0] << "  " << std::scientific << std::setprecision(4)
                    << L[1] << "  " << L[3] << "  " << rn[0] << "  " << rn[1] << "  " << rn[2] << "  "
                    << 0.0 << "  " << rn[3] << std::endl;
@@ -745,10 +752,8 @@ int main(int argc, char** argv) {
           }
         }
         if (S.has_global) {
-          double ke, ent;
-          tsm_global(ctx, k, &ke, &ent);
           glob_f[k] << std::defaultfloat << std::setprecision(6) << (long)L[0] << "  " << L[2]
-                    << "  " << ke << "  " << ent << std::endl;
+                    << "  " << g_ke[k] << "  " << g_ent[k] << std::endl;
         }
         if (steady) {
           if (S.write_soln && iter % S.write_frequency == 0) output(k, 1, v[0], false);

@@ -1008,6 +1008,23 @@ extern "C" int tsm_status(tsm_ctx* c, int s, double* v, long* iter, int* counter
   return TSM_OK;
 }
 
+extern "C" int tsm_status_all(tsm_ctx* c, double* v, long* iter, int* counter, int* done,
+                              double* norms) {
+  if (!c) return fail(TSM_E_ARG, "bad argument");
+  std::vector<tsm::Ctl> q(c->S);
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(q.data(), c->A.ctl, q.size() * sizeof(tsm::Ctl), hipMemcpyDeviceToHost));
+  for (int s = 0; s < c->S; ++s) {
+    if (v) { v[3 * s] = q[s].t; v[3 * s + 1] = q[s].dt; v[3 * s + 2] = q[s].res_total; }
+    if (iter) iter[s] = (long)q[s].iter;
+    if (counter) counter[s] = q[s].counter;
+    if (done) done[s] = q[s].done;
+    if (norms)
+      for (int k = 0; k < 4; ++k) norms[4 * s + k] = q[s].rn[k];
+  }
+  return TSM_OK;
+}
+
 extern "C" int tsm_residual_norms(tsm_ctx* c, int s, double* norms) {
   if (!c || s < 0 || s >= c->S || !norms) return fail(TSM_E_ARG, "bad argument");
   tsm::Ctl q;
@@ -1246,16 +1263,14 @@ extern "C" int tsm_probe_density(tsm_ctx* c, int np, const double* px, const dou
 extern "C" int tsm_global_all(tsm_ctx* c, double* ke, double* ent) {
   if (!c) return fail(TSM_E_ARG, "bad argument");
   const int nb = c->nblk_v, S = c->S;
-  double* d = nullptr;
-  HIP_TRY(hipMalloc(&d, (size_t)nb * S * 2 * sizeof(double)));
+  int rc = ensure_red(c, (size_t)nb * S * 2);  // the context's reduction buffer: no allocation per call
+  if (rc) return rc;
   c->bind_state();
-  tsm::launch_global_all(c->G, c->A, S, d, c->st);
-  hipError_t e = hipGetLastError();
+  tsm::launch_global_all(c->G, c->A, S, c->red, c->st);
+  HIP_TRY(hipGetLastError());
   std::vector<double> h((size_t)nb * S * 2);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-  if (e == hipSuccess) e = hipMemcpy(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost);
-  (void)hipFree(d);
-  if (e != hipSuccess) return fail(TSM_E_HIP, hipGetErrorString(e));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(h.data(), c->red, h.size() * sizeof(double), hipMemcpyDeviceToHost));
   for (int s = 0; s < S; ++s) {
     double K = 0, E = 0;
     for (int b = 0; b < nb; ++b) { K += h[((size_t)s * nb + b) * 2]; E += h[((size_t)s * nb + b) * 2 + 1]; }
