@@ -324,7 +324,7 @@ def cv_experiment(problem: Problem, n_fine, n_coarse, n_train, n_pilot, n_pair, 
 
 def mlmc_experiment(problem: Problem, levels, samples, n_ref, seed=0, log=None):
     """Multilevel MC against plain MC on the finest level at the same
-    GPU-seconds, both judged against a plain-MC reference with n_ref samples
+    GPU-seconds, both measured against a plain-MC reference with n_ref samples
     on the finest mesh.  Errors: RMS over the QoIs with non-zero reference
     variance of |estimate - reference| scaled by the reference's standard
     deviation (means) or variance (variances)."""

@@ -1,7 +1,7 @@
 """Pins of the oracle's mesh, geometry and solver functions (CPU only).
 
 The reference has no tests or golden vectors (its only numbers are README
-airfoil forces and the survey's probe runs of its binary).  These checks fix
+airfoil forces and probe runs of its binary built from its sources).  These checks fix
 the oracle by invariants and closed forms that a plausible mistake breaks:
  * geometry (src/grid_preproc.cc): dual areas tile the domain, every dual
    cell is closed, normals point from the first to the second face vertex,
