@@ -199,6 +199,7 @@ def lib():
         L.tsm_keep_stage_state.argtypes = [C.c_void_p, C.c_int]
         L.tsm_get_clock.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
         L.tsm_bounds.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.tsm_status_all.argtypes = [C.c_void_p] + [C.c_void_p] * 5
         L.tsm_set_clock.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_int, C.c_double, C.c_double]
         L.tsm_stage_gradients.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_surface.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
@@ -474,6 +475,18 @@ class Solver:
         it, cnt, dn = C.c_long(), C.c_int(), C.c_int()
         _check(lib().tsm_status(self._h, sample, _dptr(v), C.byref(it), C.byref(cnt), C.byref(dn)))
         return dict(t=v[0], dt=v[1], res=v[2], iter=it.value, counter=cnt.value, done=bool(dn.value))
+
+    def status_all(self):
+        """Every sample's loop state in one transfer: dict of arrays t, dt, res,
+        iter, counter, done and norms (n_samples, 4)."""
+        n = self.n_samples
+        v, rn = np.zeros((n, 3)), np.zeros((n, 4))
+        it, cnt, dn = np.zeros(n, np.int64), np.zeros(n, np.int32), np.zeros(n, np.int32)
+        _check(lib().tsm_status_all(self._h, _dptr(v), it.ctypes.data_as(C.c_void_p),
+                                    cnt.ctypes.data_as(C.c_void_p), dn.ctypes.data_as(C.c_void_p),
+                                    _dptr(rn)))
+        return dict(t=v[:, 0], dt=v[:, 1], res=v[:, 2], iter=it, counter=cnt, done=dn.astype(bool),
+                    norms=rn)
 
     def set_bc_state(self, values, sample=0):
         """Boundary values of one sample per bface end point: values (n_bface,

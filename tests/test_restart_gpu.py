@@ -94,3 +94,17 @@ def test_bounds_are_the_state_extremes():
         want = [W[:, 0].min(), W[:, 0].max(), W[:, 3].min(), W[:, 3].max(), W[:, 1].min(),
                 W[:, 1].max(), W[:, 2].min(), W[:, 2].max(), 0.0, 0.0]
         assert np.array_equal(s.bounds(k), np.array(want))
+
+
+def test_status_all_equals_per_sample_status():
+    m = vortex_mesh(17)
+    s = T.Solver(m, CASES["tiled"], n_samples=3)
+    for k in range(3):
+        s.set_state(vortex(m.x, m.y) * (1.0 + 0.2 * k), k)
+    s.run(7)
+    a = s.status_all()
+    for k in range(3):
+        one = s.status(k)
+        for key in ("t", "dt", "res", "iter", "counter", "done"):
+            assert a[key][k] == one[key]
+        assert np.array_equal(a["norms"][k], s.residual_norms(k))
