@@ -43,6 +43,7 @@ CASES = {
     "ns_rk4_sutherland": (BC, _prm(model="ns", mu_ref=0.02, viscosity="sutherland", T_0=0.4,
                                    time_scheme="rk4")),
     "ns_steady": (BC, _prm(model="ns", mu_ref=0.01, time_mode="steady", max_iter=10 ** 6)),
+    "euler_voronoi": (BC2, _prm(cell="voronoi")),
 }
 
 
