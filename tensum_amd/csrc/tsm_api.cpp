@@ -173,12 +173,6 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   bool wall_cells = false;  // Navier-Stokes cells with a non-periodic bface
   if (params->model == 1)
     for (int k = 0; k < nc; ++k) wall_cells = wall_cells || g.cell_has_nonper_bface[k];
-  if (wall_cells && g.removed_faces > 0) {
-    // the reference then reads the cells' face slots against the shortened
-    // face list (src/grid_preproc.cc:811-831), which the wall-cell tables do not follow
-    delete c;
-    return fail(TSM_E_ARG, "Navier-Stokes wall cells with Voronoi duals that lost empty faces are not supported");
-  }
   if ((size_t)nf >= (1u << 30) || (size_t)nb >= (1u << 30)) {
     delete c;
     return fail(TSM_E_MESH, "mesh too large for 30-bit face ids");
@@ -310,8 +304,12 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     for (int k = 0; k < nc; ++k) {
       int t = 0;
       for (int j = 0; j < 3; ++j) {
+        // slots keep their indices into the face list as it was before Voronoi
+        // duals lost their empty faces (src/grid_preproc.cc:633-647, :811-831);
+        // boundary faces come first and are never removed, so a slot past the
+        // shortened list, like any slot past the boundary faces, is interior
         int f = g.cell_slot_face[3 * k + j];
-        if (f < 0 || face_bface[f] < 0) continue;
+        if (f < 0 || f >= nf || face_bface[f] < 0) continue;
         if (t == 2) { delete c; return fail(TSM_E_MESH, "cell with three boundary edges"); }
         cwf[2 * k + t] = f;
         cwb[2 * k + t] = face_bface[f];

@@ -80,14 +80,3 @@ def test_finished_samples_stay_finished():
     dt, _ = s.step()
     assert dt[0] == 0.0
     assert np.array_equal(s.get_state(), w)
-
-
-def test_navier_stokes_walls_on_voronoi_with_removed_faces_are_refused():
-    # an unjittered lattice: hypotenuse faces between right triangles are empty
-    # Voronoi faces, which preprocessing removes
-    m = T.jittered_mesh(9, jitter=0.0, bc={t: ("noslip", (1.0, 0.0, 0.0, 1.0)) for t in (1, 2, 3, 4)})
-    prm = T.Params(model="ns", flux="kepec", reconstruct="first", mu_ref=0.01, cfl=0.5,
-                   final_time=1.0, cell="voronoi")
-    with pytest.raises(T.TsmError, match="Voronoi"):
-        T.Solver(m, prm)
-    T.Solver(m, T.Params(**{**prm.__dict__, "cell": "median"})).close()
