@@ -1,8 +1,9 @@
 """The `tensum` command-line driver: param.in grammar (CPU) and a full run (GPU).
 
-The param.in fixtures tests/golden/param_isentropic_vortex.in and
-param_uq_shocktube.in are the reference's own example inputs
-(examples/Euler/isentropic_vortex/param.in, examples/UQ/shocktube/param.in).
+The param.in fixtures tests/golden/param_isentropic_vortex.in,
+param_uq_shocktube.in and param_flat_plate.in are the reference's own example
+inputs (examples/Euler/isentropic_vortex/param.in, examples/UQ/shocktube/param.in,
+examples/NavierStokes/flat_plate/param.in).
 """
 import os
 import subprocess
@@ -614,3 +615,44 @@ def test_driver_bounds_file(tmp_path):
     want = [t, W[:, 0].min(), W[:, 0].max(), W[:, 3].min(), W[:, 3].max(), W[:, 1].min(),
             W[:, 1].max(), W[:, 2].min(), W[:, 2].max(), 0.0, 0.0]
     np.testing.assert_allclose(rows[-1], want, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_driver_runs_the_reference_flat_plate_example(tmp_path):
+    """The reference's Navier-Stokes example (examples/NavierStokes/flat_plate/
+    param.in, unchanged but for max_iter): steady, Voronoi duals (which lose
+    empty faces), an adiabatic noslip plate after a slip lead-in, farfield
+    inflow, pressure outflow and top, physical units (R = 287, p = 8610).
+    On a synthetic unit-square mesh with the example's boundary tags: the
+    steady residual log and the final state against the oracle."""
+    m0 = T.jittered_mesh(21, seed=3)
+    bv = np.asarray(m0.bface_v).reshape(-1, 2)
+    xm = 0.5 * (m0.x[bv[:, 0]] + m0.x[bv[:, 1]])
+    side = np.asarray(m0.bface_tag)
+    tags = np.where(side == 4, 100001, np.where(side == 2, 100004, np.where(side == 3, 100005,
+                    np.where(xm < 0.25, 100002, 100003)))).astype(np.int32)
+    Tin, uin, pin = 300.0, 34.7189, 8610.0
+    bc = {100001: ("farfield", (Tin, uin, 0.0, pin)), 100002: ("slip",),
+          100003: ("noslip", (Tin, 0.0, 0.0, pin), True), 100004: ("pressure", (Tin, uin, 0.0, pin)),
+          100005: ("pressure", (Tin, uin, 0.0, pin))}
+    mesh = T.Mesh(m0.x, m0.y, tags, m0.bface_v, m0.cell_v, m0.periodic_ptr, m0.periodic_v, bc)
+    write_part_file(mesh, str(tmp_path))
+    text = open(os.path.join(GOLD, "param_flat_plate.in")).read()
+    assert "max_iter     1000000" in text
+    (tmp_path / "param.in").write_text(text.replace("max_iter     1000000", "max_iter     50"))
+    r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    sd = tmp_path / "SAMPLE_0"
+    prm = T.Params(gas_const=287.0, model="ns", mu_ref=3.47189e-5, prandtl=0.72,
+                   flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.4, time_mode="steady",
+                   max_iter=50, min_residue=1e-8, cell="voronoi")
+    o = O.Oracle(mesh, prm)
+    o.set_state(np.tile([Tin, uin, 0.0, pin], (mesh.n_vertex, 1)))
+    n, log = o.run(10 ** 6, want_log=True)
+    res = np.loadtxt(sd / "residual.dat")
+    assert n == 50 and res.shape == (50, 8)
+    np.testing.assert_allclose(res[:, 2], log[:, 3], rtol=6e-5)
+    p = _vtk_scalars(sd / "sol0001_0.vtk", "pressure", mesh.n_vertex)
+    np.testing.assert_allclose(p, o.get_state()[:, 3], rtol=1e-10)
+    assert len((sd / "global.dat").read_text().splitlines()) == 50
