@@ -237,6 +237,14 @@ int tsm_error_norms(tsm_ctx* ctx, int sample, const double* exact_prim, double* 
 /* FiniteVolume::compute_global (src/fv.cc:2175-2225): kinetic energy and entropy. */
 int tsm_global(tsm_ctx* ctx, int sample, double* ke, double* entropy);
 
+/* Enstrophy of the current state, ens = 1/2 sum_i omega_i^2 dcarea_i w_i with
+   the weights of compute_global and the vorticity of Material::Vorticity
+   (src/material.h:622-627) from the entropy-variable gradients of the current
+   state (as tsm_gradients).  The reference writes vorticity fields
+   (src/writer.cc:241-255) but integrates no enstrophy; it sits beside
+   compute_global's integrals (src/fv.cc:2175-2225). */
+int tsm_enstrophy(tsm_ctx* ctx, int sample, double* ens);
+
 /* Monte Carlo reductions over all samples of the context (current states).
    tsm_moments: per vertex, in part-file order, 5 fields (T, u, v, p, rho):
    mean, M2 = sum of squared deviations and S2 = sum of squares, accumulated in
@@ -266,6 +274,13 @@ int tsm_sobol_max_dim(void);
    TSM_E_ARG for an unknown type or a PERT4 index outside [1, n_rnd], where the
    reference aborts. */
 int tsm_pert(int type, double x, double y, double p, const double* rnd, int n_rnd, double* out);
+
+/* Self-check of the device arithmetic (no counterpart in the reference): the
+   quotient and square root the stage kernels use (the refinement steps of the
+   IEEE sequences without their range scaling) beside the compiler's `/` and
+   sqrt() on the same operands: out[4i..4i+3] = fdiv(a_i, b_i), a_i / b_i,
+   fsqrt(a_i), sqrt(a_i). */
+int tsm_check_arith(long n, const double* a, const double* b, double* out);
 
 /* Geometry export in reference order, for parity checks. */
 int tsm_geometry_sizes(tsm_ctx* ctx, int* n_face, int* n_bface);

@@ -1464,6 +1464,24 @@ void orc_global(const orc_solver* s, double* ke, double* ent) {
   *ke = K; *ent = S;
 }
 
+// Enstrophy, 1/2 sum_i omega_i^2 dcarea_i w_i, beside compute_global's
+// integrals (src/fv.cc:2175-2225; the reference has no enstrophy): the
+// vorticity of Material::Vorticity (src/material.h:622-627) from the entropy-
+// variable gradients of the current state (compute_gradients),
+// omega = R T (dE[vy].x + v dE[l].x - dE[vx].y - u dE[l].y).
+double orc_enstrophy(orc_solver* s) {
+  compute_gradients(s);
+  const Mat& M = s->M;
+  double Z = 0;
+  for (int i = 0; i < s->nv; ++i) {
+    const Prim& w = s->W[i];
+    const V2* g = &s->dE[(size_t)i * 4];
+    const double om = M.p.gas_const * w.T * (g[2].x + w.v * g[3].x - g[1].y - w.u * g[3].y);
+    Z += (0.5 * om * om * s->dcarea[i]) * s->pweight[i];
+  }
+  return Z;
+}
+
 // src/fv.cc:595-694 (increment_moments): first sample sets mean = x, S2 = x*x,
 // M2 = 0; sample k >= 2 (1-based) updates primitives with PrimVar::equ
 // (src/primvar.h:213-221): M = x*(1/k) + M*((k-1)/k), and derived fields with

@@ -115,6 +115,9 @@ int orc_set_bc_state(orc_solver* s, const int* bface_v, const double* state);
 void orc_error_norms(const orc_solver* s, const double* exact_prim, double* out);
 /* global KE and entropy integrals */
 void orc_global(const orc_solver* s, double* ke, double* ent);
+/* 1/2 sum omega^2 dcarea w over the vertices, omega from the entropy-variable
+   gradients of the current state (Material::Vorticity, src/material.h:622-627). */
+double orc_enstrophy(orc_solver* s);
 
 /* Monte Carlo moments of S sample states (prim: S x nv x 4, sample-major),
    fields (T,u,v,p,rho), in sample order: mean, M2, S2 each nv x 5 */

@@ -215,6 +215,8 @@ def lib():
                                C.c_void_p]
         L.tsm_moments.argtypes = [C.c_void_p] + [C.c_void_p] * 3
         L.tsm_probe_density.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
+        L.tsm_check_arith.argtypes = [C.c_long] + [C.c_void_p] * 3
+        L.tsm_enstrophy.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_global_all.argtypes = [C.c_void_p] + [C.c_void_p] * 2
         L.tsm_mesh_jittered_sizes.argtypes = [C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 5
         L.tsm_mesh_jittered.argtypes = ([C.c_int] + [C.c_double] * 5 + [C.c_ulonglong, C.c_int]
@@ -328,6 +330,16 @@ def jittered_mesh(n, box=(0.0, 1.0, 0.0, 1.0), jitter=0.2, seed=0, periodic=Fals
     if bc is None:
         bc = {t: ("periodic",) if periodic else ("outlet",) for t in (1, 2, 3, 4)}
     return Mesh(x, y, bt, bv, cv, pp, pv, bc)
+
+
+def check_arith(a, b):
+    """The stage kernels' fdiv/fsqrt beside the device's IEEE / and sqrt():
+    (n, 4) array of fdiv(a, b), a / b, fsqrt(a), sqrt(a)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.empty((a.size, 4))
+    _check(lib().tsm_check_arith(a.size, _dptr(a), _dptr(b), _dptr(out)))
+    return out
 
 
 def host_geometry(mesh: Mesh, cell="median"):
@@ -565,6 +577,12 @@ class Solver:
         _check(lib().tsm_global(self._h, sample, C.byref(k), C.byref(e)))
         return k.value, e.value
 
+    def enstrophy(self, sample=0):
+        """1/2 sum omega^2 dcarea w of the current state."""
+        z = C.c_double()
+        _check(lib().tsm_enstrophy(self._h, sample, C.byref(z)))
+        return z.value
+
     def moments(self):
         """Per-vertex (mean, M2, S2) of (T, u, v, p, rho) over all samples, each (n_vertex, 5)."""
         m, q, z = (np.empty((self.nv, 5)) for _ in range(3))
@@ -578,6 +596,7 @@ class Solver:
         out = np.empty((self.n_samples, px.size))
         _check(lib().tsm_probe_density(self._h, px.size, _dptr(px), _dptr(py), _dptr(out)))
         return out
+
 
     def global_all(self):
         """Kinetic energy and entropy integrals of every sample: two (n_samples,) arrays."""

@@ -1059,6 +1059,26 @@ extern "C" int tsm_global(tsm_ctx* c, int s, double* ke, double* ent) {
   return TSM_OK;
 }
 
+extern "C" int tsm_enstrophy(tsm_ctx* c, int s, double* ens) {
+  if (!c || !ens || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
+  const int nb = c->nblk_v;
+  int rc = ensure_red(c, (size_t)nb * 24);
+  if (rc) return rc;
+  c->sample_ready(s);  // gradients of the current state of sample s
+  c->A.ignore_done = 1;
+  tsm::launch_gradient_one(c->G, c->A, s, c->st);
+  c->A.ignore_done = 0;
+  tsm::launch_enstrophy(c->G, c->A, s, c->red, c->st);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> h((size_t)nb);
+  HIP_TRY(hipStreamSynchronize(c->st));
+  HIP_TRY(hipMemcpy(h.data(), c->red, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double Z = 0;
+  for (int b = 0; b < nb; ++b) Z += h[b];
+  *ens = Z;
+  return TSM_OK;
+}
+
 extern "C" int tsm_error_norms(tsm_ctx* c, int s, const double* exact, double* out) {
   if (!c || !exact || !out || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
   const int nv = c->G.nv, nb = c->nblk_v;
@@ -1254,6 +1274,25 @@ extern "C" int tsm_probe_density(tsm_ctx* c, int np, const double* px, const dou
   if (e == hipSuccess) e = hipStreamSynchronize(c->st);
   if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)np * c->S * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(dpc); (void)hipFree(dpw); (void)hipFree(dout);
+  if (e != hipSuccess) return fail(TSM_E_HIP, hipGetErrorString(e));
+  return TSM_OK;
+}
+
+extern "C" int tsm_check_arith(long n, const double* a, const double* b, double* out) {
+  if (n < 1 || !a || !b || !out) return fail(TSM_E_ARG, "bad argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TSM_E_HIP, "no HIP device visible");
+  double* d = nullptr;
+  HIP_TRY(hipMalloc(&d, 6 * (size_t)n * sizeof(double)));
+  hipError_t e = hipMemcpy(d, a, 8 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d + n, b, 8 * (size_t)n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    tsm::launch_check_arith(n, d, d + n, d + 2 * n, nullptr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(out, d + 2 * n, 32 * (size_t)n, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
   if (e != hipSuccess) return fail(TSM_E_HIP, hipGetErrorString(e));
   return TSM_OK;
 }

@@ -310,6 +310,7 @@ void launch_step_final(const DevGeo& G, const StageArgs& A, const StepParams& P,
 void launch_ssw(const DevGeo& G, const StageArgs& A, int S, hipStream_t st);
 void launch_prim2ent(const DevGeo& G, const StageArgs& A, int s, hipStream_t st);
 void launch_global(const DevGeo& G, const StageArgs& A, int s, double* out, hipStream_t st);
+void launch_enstrophy(const DevGeo& G, const StageArgs& A, int s, double* out, hipStream_t st);
 void launch_errnorm(const DevGeo& G, const StageArgs& A, int s, const double* ex, double* out,
                     hipStream_t st);
 
@@ -319,5 +320,6 @@ void launch_moments(const DevGeo& G, const StageArgs& A, int S, double* mean, do
 void launch_probes(const StageArgs& A, int nv, int np, const int* pc, const double* pw, int S,
                    double* out, hipStream_t st);
 void launch_global_all(const DevGeo& G, const StageArgs& A, int S, double* out, hipStream_t st);
+void launch_check_arith(long n, const double* a, const double* b, double* out, hipStream_t st);
 
 }  // namespace tsm

@@ -961,6 +961,27 @@ void launch_ssw(const DevGeo& G, const StageArgs& A, int S, hipStream_t st) {
 void launch_prim2ent(const DevGeo& G, const StageArgs& A, int s, hipStream_t st) {
   hipLaunchKernelGGL(k_prim2ent, dim3((G.nv + 255) / 256), dim3(256), 0, st, G, A, s);
 }
+// Enstrophy partials, 1/2 omega^2 dcarea w per vertex summed per block, with
+// omega = R T (dE[vy].x + v dE[l].x - dE[vx].y - u dE[l].y) from the sample's
+// entropy-variable gradients (Material::Vorticity, src/material.h:622-627).
+__global__ __launch_bounds__(256) void k_enstrophy(DevGeo G, StageArgs A, int s, double* out) {
+  __shared__ double lds[4];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t off = (size_t)s * G.nv;
+  double q[1] = {0};
+  if (v < G.nv) {
+    const Prim w = load_prim(A.W_in, off + v);
+    const double* g = A.dE + 8 * (off + v);
+    const double om = A.gasR[s] * w.T * (g[4] + w.v * g[6] - g[3] - w.u * g[7]);
+    q[0] = (0.5 * om * om * G.dcarea[v]) * G.weight[v];
+  }
+  block_sum<1>(q, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = q[0];
+}
+void launch_enstrophy(const DevGeo& G, const StageArgs& A, int s, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_enstrophy, dim3((G.nv + 255) / 256), dim3(256), 0, st, G, A, s, out);
+}
+
 void launch_global(const DevGeo& G, const StageArgs& A, int s, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_global, dim3((G.nv + 255) / 256), dim3(256), 0, st, G, A, s, out);
 }
@@ -1055,6 +1076,22 @@ void launch_moments(const DevGeo& G, const StageArgs& A, int S, double* mean, do
                     double* s2, hipStream_t st) {
   hipLaunchKernelGGL(k_moments, dim3((G.nv + 255) / 256), dim3(256), 0, st, G, A, S, mean, m2, s2);
 }
+// The stage kernels' quotient and square root (fdiv, fsqrt) beside the
+// compiler's IEEE `/` and sqrt() on the same operands: out[4i..4i+3] =
+// fdiv(a, b), a / b, fsqrt(a), sqrt(a).
+__global__ void k_check_arith(long n, const double* a, const double* b, double* out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = a[i], y = b[i];
+  out[4 * i] = fdiv(x, y);
+  out[4 * i + 1] = x / y;
+  out[4 * i + 2] = fsqrt(x);
+  out[4 * i + 3] = sqrt(x);
+}
+void launch_check_arith(long n, const double* a, const double* b, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_check_arith, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b, out);
+}
+
 void launch_probes(const StageArgs& A, int nv, int np, const int* pc, const double* pw, int S,
                    double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_probes, dim3((np * S + 255) / 256), dim3(256), 0, st, A, nv, np, pc, pw, S, out);

@@ -66,6 +66,8 @@ def lib():
         L.orc_time.argtypes = [C.c_void_p]
         L.orc_error_norms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_global.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_enstrophy.restype = C.c_double
+        L.orc_enstrophy.argtypes = [C.c_void_p]
         L.orc_logavg.restype = C.c_double
         L.orc_logavg.argtypes = [C.c_double, C.c_double]
         L.orc_num_flux.argtypes = [C.POINTER(OrcParams)] + [C.c_void_p] * 7 + [C.c_double, C.c_void_p]
@@ -200,6 +202,9 @@ class Oracle:
         out = np.empty(24)
         lib().orc_error_norms(self._h, _p(a), _p(out))
         return out.reshape(8, 3)
+
+    def enstrophy(self):
+        return lib().orc_enstrophy(self._h)
 
     def global_integrals(self):
         k, e = C.c_double(), C.c_double()
