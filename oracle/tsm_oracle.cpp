@@ -1482,6 +1482,12 @@ double orc_enstrophy(orc_solver* s) {
   return Z;
 }
 
+// the Liou sensor of the current state, 0/1 per vertex (compute_ssw)
+void orc_ssw(orc_solver* s, int* out) {
+  compute_ssw(s);
+  for (int i = 0; i < s->nv; ++i) out[i] = s->ssw[i] > 0 ? 1 : 0;
+}
+
 // src/fv.cc:595-694 (increment_moments): first sample sets mean = x, S2 = x*x,
 // M2 = 0; sample k >= 2 (1-based) updates primitives with PrimVar::equ
 // (src/primvar.h:213-221): M = x*(1/k) + M*((k-1)/k), and derived fields with

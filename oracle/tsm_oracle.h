@@ -118,6 +118,9 @@ void orc_global(const orc_solver* s, double* ke, double* ent);
 /* 1/2 sum omega^2 dcarea w over the vertices, omega from the entropy-variable
    gradients of the current state (Material::Vorticity, src/material.h:622-627). */
 double orc_enstrophy(orc_solver* s);
+/* Liou carbuncle sensor of the current state (src/compute_ssw.cc:7-39; only
+   for kepes_tecno_roe): out[nv] = 1 where a face raises it, else 0. */
+void orc_ssw(orc_solver* s, int* out);
 
 /* Monte Carlo moments of S sample states (prim: S x nv x 4, sample-major),
    fields (T,u,v,p,rho), in sample order: mean, M2, S2 each nv x 5 */

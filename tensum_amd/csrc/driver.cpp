@@ -310,7 +310,7 @@ struct Moments {  // src/fv.cc:595-694 (primitive fields + density/mach/entropy)
 int main(int argc, char** argv) {
   std::string ifile, loc;
   int npart = -1;
-  bool restart = false, check_only = false, bounds = false;
+  bool restart = false, check_only = false, bounds = false, enstrophy = false;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     if (a == "-i" && i + 1 < argc) ifile = argv[++i];
@@ -319,6 +319,7 @@ int main(int argc, char** argv) {
     else if (a == "-r") restart = true;
     else if (a == "--check") { check_only = true; npart = npart < 1 ? 1 : npart; loc = loc.empty() ? "." : loc; }
     else if (a == "-b") bounds = true;
+    else if (a == "-e") enstrophy = true;
     else if (a == "-p" || a == "-v" || a == "-d") {}
     else if (a == "-axi") { std::cerr << "ERROR --> -axi is not supported (2-D only)" << std::endl; return 1; }
     else { std::cerr << "Unknown command line flag: " << a << std::endl; ifile.clear(); break; }
@@ -328,7 +329,8 @@ int main(int argc, char** argv) {
                  "   -npart N      Specify number of mesh partitions N (required, 1)\n"
                  "   -L path       Location of mesh partition folder (required)\n"
                  "   -r            Restart from SAMPLE_<id>/restart files\n"
-                 "   -b            Write the solution bounds to SAMPLE_<id>/bounds.dat\n";
+                 "   -b            Write the solution bounds to SAMPLE_<id>/bounds.dat\n"
+                 "   -e            With save_global: enstrophy as a fifth column of global.dat\n";
     return 1;
   }
   if (npart != 1) { std::cerr << "ERROR --> one GPU holds the whole mesh: use -npart 1" << std::endl; return 1; }
@@ -646,9 +648,13 @@ int main(int argc, char** argv) {
       return good;
     });
   };
+  // a failed download ends the run (no stale state is written): false after
+  // printing the library's message
   auto output = [&](int k, int counter, double t, bool master = true) {
-    tsm_get_state(ctx, k, W.data());
-    if (!raw) out_gradients(k, dE.data());
+    if (tsm_get_state(ctx, k, W.data()) != TSM_OK || (!raw && out_gradients(k, dE.data()) != TSM_OK)) {
+      std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
+      return false;
+    }
     const std::string name = stamp_name("sol", counter) + (raw ? ".raw" : ".vtk");
     const std::string dir = dirs[k];
     writer.post(k, [&P, &dcarea, &M, &S, raw, nv, dir, name, t, master, w = W,
@@ -659,6 +665,7 @@ int main(int argc, char** argv) {
       return good;
     });
     surfaces(k, counter);
+    return true;
   };
   // online statistics across samples at each stamp (Welford in sample order);
   // a steady run has two: the initial and the converged state
@@ -669,19 +676,23 @@ int main(int argc, char** argv) {
   std::vector<std::vector<std::vector<double>>> vsnap;  // per stamp and sample: vorticity
   if (S.online_stat && has_vort) vsnap.assign(nst, std::vector<std::vector<double>>(ns));
   auto keep = [&](int k, int counter) {
-    if (!S.online_stat) return;
-    tsm_get_state(ctx, k, W.data());
+    if (!S.online_stat) return true;
+    if (tsm_get_state(ctx, k, W.data()) != TSM_OK ||
+        (has_vort && out_gradients(k, dE.data()) != TSM_OK)) {
+      std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
+      return false;
+    }
     snap[counter][k] = W;
     if (has_vort) {
-      out_gradients(k, dE.data());
       std::vector<double> om(nv);
       for (int i = 0; i < nv; ++i) om[i] = vorticity(M, &W[4 * i], &dE[8 * i]);
       vsnap[counter][k] = std::move(om);
     }
+    return true;
   };
   for (int k = 0; k < ns; ++k) {
-    if (S.write_soln && !restart) output(k, 0, 0.0);  // src/fv.cc:2240-2245
-    keep(k, 0);
+    if (S.write_soln && !restart && !output(k, 0, 0.0)) return 1;  // src/fv.cc:2240-2245
+    if (!keep(k, 0)) return 1;
   }
   // time loop, all samples together; outputs as each reaches its stamps
   std::vector<double> logbuf(4 * (size_t)ns);
@@ -730,7 +741,7 @@ int main(int argc, char** argv) {
           for (double x : b) bnd_f[k] << std::left << std::setw(17) << std::setfill(' ') << std::setprecision(10) << x;
           bnd_f[k] << std::endl;
           if (b[0] < 0.0 || b[2] < 0.0) {
-            if (S.write_soln) output(k, counter, L[2]);
+            if (S.write_soln) (void)output(k, counter, L[2]);
             writer.finish();
             std::cerr << "ERROR --> Error:: Observed negative temperature/pressure in solution (" << dirs[k]
                       << ")" << std::endl;
@@ -753,19 +764,31 @@ int main(int argc, char** argv) {
         }
         if (S.has_global) {
           glob_f[k] << std::defaultfloat << std::setprecision(6) << (long)L[0] << "  " << L[2]
-                    << "  " << g_ke[k] << "  " << g_ent[k] << std::endl;
+                    << "  " << g_ke[k] << "  " << g_ent[k];
+          if (enstrophy) {  // no counterpart in the reference's global.dat
+            double z = 0.0;
+            if (tsm_enstrophy(ctx, k, &z) != TSM_OK) {
+              std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
+              return 1;
+            }
+            glob_f[k] << "  " << z;
+          }
+          glob_f[k] << std::endl;
         }
         if (steady) {
-          if (S.write_soln && iter % S.write_frequency == 0) output(k, 1, v[0], false);
+          if (S.write_soln && iter % S.write_frequency == 0 && !output(k, 1, v[0], false)) return 1;
         } else if (counter > cnt_before[k]) {
-          if (S.write_soln) output(k, cnt_before[k], v[0]);
-          keep(k, cnt_before[k]);
+          if (S.write_soln && !output(k, cnt_before[k], v[0])) return 1;
+          if (!keep(k, cnt_before[k])) return 1;
           cnt_before[k] = counter;
         }
         if (S.write_restart && (iter % S.restart_frequency == 0 || v[0] == S.p.final_time)) {
-          tsm_get_state(ctx, k, W.data());
           double res0 = 1.0;
-          tsm_get_clock(ctx, k, nullptr, nullptr, nullptr, &res0);
+          if (tsm_get_state(ctx, k, W.data()) != TSM_OK ||
+              tsm_get_clock(ctx, k, nullptr, nullptr, nullptr, &res0) != TSM_OK) {
+            std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
+            return 1;
+          }
           writer.post(k, [nv, dir = dirs[k], h = head[k], iter, counter, t = v[0], res0, w = W]() {
             std::ofstream fo(dir + "/restart_" + std::to_string(h) + "_0.dat");
             for (int i = 0; i < nv; ++i)
@@ -784,8 +807,8 @@ int main(int argc, char** argv) {
         finished[k] = 1;
         --live;
         if (steady) {  // final output of a steady run (src/fv.cc:2371-2382)
-          if (S.write_soln) output(k, 1, v[0]);
-          keep(k, 1);
+          if (S.write_soln && !output(k, 1, v[0])) return 1;
+          if (!keep(k, 1)) return 1;
         }
         if (!std::isfinite(v[2]))
           std::cerr << "    " << dirs[k] << ": non-finite residual at t = " << v[0] << ", time loop left" << std::endl;
@@ -803,7 +826,10 @@ int main(int argc, char** argv) {
       for (int i = 0; i < nv; ++i)
         for (int q = 0; q < 4; ++q) ex[4 * i + q] = e[q].eval(P.x[i], P.y[i], S.p.final_time);
       double o[24];
-      tsm_error_norms(ctx, k, ex.data(), o);
+      if (tsm_error_norms(ctx, k, ex.data(), o) != TSM_OK) {
+        std::cerr << "ERROR --> " << tsm_last_error() << std::endl;
+        return 1;
+      }
       std::ofstream ef(dirs[k] + "/error_norms.dat");
       const char* lab[8] = {"TEMPERATURE", "PRESSURE", "VEL_X", "VEL_Y", "VEL_Z", "DENSITY", "MACH", "ENTROPY"};
       ef << "N = " << nv << "\n" << std::scientific << std::setprecision(8);

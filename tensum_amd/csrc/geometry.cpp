@@ -244,8 +244,18 @@ bool build_geometry(const tsm_mesh& m, int cell_type, Geometry& g, std::string& 
   }
   g.removed_faces = 0;
   if (cell_type == 1) {  // src/grid_preproc.cc:640-655 remove_empty_faces
+    // Boundary entries lead face[] and must all survive: the cells' slot ids
+    // (and the reference's d_bnd_* lists, src/grid_preproc.cc:564-572) keep
+    // their pre-removal values, which name the right boundary faces only while
+    // nothing before them is removed.
     std::vector<Face> kept;
-    for (auto& f : face) if (f.measure > 0.0) kept.push_back(f);
+    for (auto& f : face) {
+      if (f.measure > 0.0) kept.push_back(f);
+      else if (f.type != -1) {
+        msg = "a boundary face of the Voronoi dual has zero length";
+        return false;
+      }
+    }
     g.removed_faces = (int)(face.size() - kept.size());
     face.swap(kept);
   }

@@ -69,8 +69,27 @@ def _mixed_mesh(n, kinds):
     st = MIXED_STATE
     bc = {}
     for tag, kind in zip((1, 2, 3, 4), kinds):
-        bc[tag] = (kind, st) if kind in ("inlet", "farfield", "pressure") else (kind,)
+        if kind in ("inlet", "farfield", "pressure"):
+            bc[tag] = (kind, st)
+        elif kind == "noslip":  # adiabatic wall at rest
+            bc[tag] = (kind, (st[0], 0.0, 0.0, st[3]), True)
+        elif kind == "noslip_T":  # isothermal wall at rest, T = 1.1
+            bc[tag] = ("noslip", (1.1, 0.0, 0.0, st[3]))
+        else:
+            bc[tag] = (kind,)
     return T.jittered_mesh(n, seed=5, bc=bc)
+
+
+def _vortex_case(flux, rec, n=17, steps=60, cfl=0.8, **kw):
+    return dict(mesh=lambda: vortex_mesh(n), ic=VORTEX_IC, consts={"gam": 1.4, "beta": 5},
+                state=lambda m: vortex(m.x, m.y, box=None), bc={(1, 2, 3, 4): ("periodic", None)},
+                params=dict(flux=flux, reconstruct=rec, cfl=cfl, **kw), steps=steps, final_time=None)
+
+
+def _riemann_case(flux, rec, n=25, steps=100, **kw):
+    return dict(mesh=lambda: riemann_mesh(n), ic=RIEMANN_IC, consts={},
+                state=lambda m: riemann(m.x, m.y), bc={(1, 2, 3, 4): ("outlet", None)},
+                params=dict(flux=flux, reconstruct=rec, cfl=0.5, **kw), steps=steps, final_time=None)
 
 
 # name -> settings.  `steps`: the restart after that step is kept; the run
@@ -126,6 +145,24 @@ CASES = {
         params=dict(model="ns", flux="kepec", reconstruct="first", cfl=0.5, mu_ref=1.0 / 1600,
                     prandtl=0.72),
         steps=200, final_time=None, save_global=True),
+    # the other two-point fluxes and every reconstruction of the TeCNO fluxes
+    "kep_vortex17": _vortex_case("kep", "first"),
+    "roe_ec_vortex17": _vortex_case("roe_ec", "first"),
+    "avg_vortex17": _vortex_case("simple_avg", "first", cfl=0.4),
+    "albada_vortex25": _vortex_case("kepes_tecno_roe", "van_albada", n=25, steps=100),
+    "second_vortex25": _vortex_case("kepes_tecno_rusanov", "second", n=25, steps=100),
+    "eno2_riemann25": _riemann_case("kepes_tecno_rusanov", "eno2"),
+    "first_riemann25": _riemann_case("kepes_tecno_roe", "first"),
+    # Navier-Stokes wall cells: adiabatic noslip floor, isothermal noslip lid,
+    # inlet and outlet; KEPES-TeCNO Roe with minmod and the viscous fluxes
+    "walls_ns23": dict(
+        mesh=lambda: _mixed_mesh(23, ("noslip", "outlet", "noslip_T", "inlet")), ic=MIXED_IC,
+        consts={}, state=lambda m: _mixed_state(m.x, m.y),
+        bc={(1,): ("noslip", None), (2,): ("outlet", None), (3,): ("noslip_T", 1.1),
+            (4,): ("inlet", MIXED_STATE)},
+        params=dict(model="ns", flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.4,
+                    mu_ref=0.01, prandtl=0.72),
+        steps=100, final_time=None, save_global=True),
     # the plain-method workload of the error-norm bar: 129^2 vortex to t = 10
     # (one period of the box): the exact state equals the initial one
     "vortex129_t10": dict(
@@ -196,6 +233,12 @@ def param_text(case, final_time, restart_frequency):
             lines.append(f"      {'pressure':12s} {st[3]!r}")
         elif kind == "pressure":
             lines.append(f"      pressure     {st[3]!r}")
+        elif kind == "noslip":
+            lines += ["      xvelocity    0.0", "      yvelocity    0.0", "      zvelocity    0.0"]
+        elif kind == "noslip_T":
+            lines[-1] = "      type         noslip"
+            lines += [f"      temperature  {st!r}", "      xvelocity    0.0",
+                      "      yvelocity    0.0", "      zvelocity    0.0"]
         lines.append("   }")
     lines.append("}")
     if c.get("exact"):

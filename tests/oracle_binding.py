@@ -68,6 +68,7 @@ def lib():
         L.orc_global.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_enstrophy.restype = C.c_double
         L.orc_enstrophy.argtypes = [C.c_void_p]
+        L.orc_ssw.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_logavg.restype = C.c_double
         L.orc_logavg.argtypes = [C.c_double, C.c_double]
         L.orc_num_flux.argtypes = [C.POINTER(OrcParams)] + [C.c_void_p] * 7 + [C.c_double, C.c_void_p]
@@ -202,6 +203,11 @@ class Oracle:
         out = np.empty(24)
         lib().orc_error_norms(self._h, _p(a), _p(out))
         return out.reshape(8, 3)
+
+    def ssw(self):
+        out = np.zeros(self.nv, dtype=np.int32)
+        lib().orc_ssw(self._h, out.ctypes.data_as(C.c_void_p))
+        return out
 
     def enstrophy(self):
         return lib().orc_enstrophy(self._h)

@@ -64,3 +64,31 @@ def test_gpu_enstrophy_matches_oracle():
     s = T.Solver(r, T.Params())
     s.set_state(_rotation(r))
     assert abs(s.enstrophy() - 2 * 0.7 ** 2) <= 1e-13
+
+
+@pytest.mark.gpu
+def test_driver_writes_enstrophy_column(tmp_path):
+    """tensum -e: global.dat gains a fifth column, the enstrophy after each
+    step (the reference's four columns unchanged)."""
+    import os
+    import subprocess
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from cases import write_part_file
+    from ref_cases import CASES, param_text
+    c = CASES["tgv33_ns"]
+    mesh = c["mesh"]()
+    write_part_file(mesh, str(tmp_path), periodic_tags=(1, 2, 3, 4))
+    (tmp_path / "param.in").write_text(param_text("tgv33_ns", 0.05, 10 ** 6))
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tensum_amd",
+                       "tensum")
+    r = subprocess.run([exe, "-i", "param.in", "-npart", "1", "-L", ".", "-e"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = np.loadtxt(tmp_path / "SAMPLE_0" / "global.dat", ndmin=2)
+    assert g.shape[1] == 5
+    o = O.Oracle(mesh, T.Params(final_time=0.05, **c["params"]))
+    o.set_state(c["state"](mesh))
+    n = o.run(10 ** 5)
+    assert g.shape[0] == n
+    np.testing.assert_allclose(g[-1, 2:], [*o.global_integrals(), o.enstrophy()], rtol=6e-6)

@@ -7,8 +7,11 @@ the oracle where no closed form does: the residual-norm convention of
 residual.dat (src/fv.cc:1801-1879), the boundary states and boundary fluxes
 of slip, outlet, inlet, farfield and pressure sides (src/bc.h:569-654,
 src/material.cc:110-127), the Liou fix with eigenvalue modification
-(src/compute_ssw.cc:7-39), Voronoi duals with RK4, the viscous residual, the
-error norms of error_norms.dat and the integrals of global.dat.
+(src/compute_ssw.cc:7-39), all six two-point fluxes (KEP's and simple_avg's
+included), the first, second, minmod, TVB, ENO2 and van Albada
+reconstructions, Voronoi duals with RK4, the viscous residual with adiabatic
+and isothermal noslip wall cells, the error norms of error_norms.dat and the
+integrals of global.dat.
 """
 import pytest
 
@@ -16,7 +19,8 @@ import oracle_binding as O
 from ref_fixture_check import check_case
 
 FAST = ["riemann33", "vortex33", "mixed23", "pressure23", "liou25", "vortex17_rk4_voronoi",
-        "tgv33_ns"]
+        "tgv33_ns", "kep_vortex17", "roe_ec_vortex17", "avg_vortex17", "albada_vortex25",
+        "second_vortex25", "eno2_riemann25", "first_riemann25", "walls_ns23"]
 
 
 @pytest.mark.parametrize("case", FAST)

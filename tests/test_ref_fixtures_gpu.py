@@ -17,7 +17,8 @@ from ref_fixture_check import check_case
 pytestmark = pytest.mark.gpu
 
 FAST = ["riemann33", "vortex33", "mixed23", "pressure23", "liou25", "vortex17_rk4_voronoi",
-        "tgv33_ns"]
+        "tgv33_ns", "kep_vortex17", "roe_ec_vortex17", "avg_vortex17", "albada_vortex25",
+        "second_vortex25", "eno2_riemann25", "first_riemann25", "walls_ns23"]
 
 
 @pytest.mark.parametrize("path", ["auto", "csr"])
