@@ -256,6 +256,18 @@ int tsm_moments(tsm_ctx* ctx, double* mean, double* m2, double* s2);
    the containing triangle: out[n_samples*n_probe] (sample-major).  Returns
    TSM_E_ARG if a point lies outside the mesh. */
 int tsm_probe_density(tsm_ctx* ctx, int n_probe, const double* px, const double* py, double* out);
+/* tsm_probe_density without device allocation.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops).  Probe points, weights and results
+   live in a fixed scratch array of 8 MiB that the library holds on each GPU it
+   runs on, until the process ends; the containing triangle is found on the
+   device (the same cell and weights as tsm_probe_density).  Limit: with
+   a = 256-byte rounding, a(8 np) x 2 + a(24 np) + a(8 np n_samples) + a(12 np)
+   + a(4 np) <= 8 MiB (e.g. 64 probes x 16,000 samples), else TSM_E_ARG and
+   nothing runs.  Calls on one GPU share the array: calls that overlap on one
+   GPU (several host threads) race.  Output as tsm_probe_density. */
+int tsm_probe_density_noalloc(tsm_ctx* ctx, int n_probe, const double* px, const double* py,
+                              double* out);
 /* Kinetic energy and entropy integrals (src/fv.cc:2175-2225) of every sample:
    ke[n_samples], entropy[n_samples] (either may be NULL). */
 int tsm_global_all(tsm_ctx* ctx, double* ke, double* entropy);

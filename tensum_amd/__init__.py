@@ -215,6 +215,7 @@ def lib():
                                C.c_void_p]
         L.tsm_moments.argtypes = [C.c_void_p] + [C.c_void_p] * 3
         L.tsm_probe_density.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
+        L.tsm_probe_density_noalloc.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3
         L.tsm_check_arith.argtypes = [C.c_long] + [C.c_void_p] * 3
         L.tsm_enstrophy.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_global_all.argtypes = [C.c_void_p] + [C.c_void_p] * 2
@@ -597,6 +598,14 @@ class Solver:
         _check(lib().tsm_probe_density(self._h, px.size, _dptr(px), _dptr(py), _dptr(out)))
         return out
 
+    def probe_density_noalloc(self, px, py):
+        """probe_density without device allocation (the library's fixed scratch
+        array; calls that overlap on one GPU race)."""
+        px = np.ascontiguousarray(px, dtype=np.float64)
+        py = np.ascontiguousarray(py, dtype=np.float64)
+        out = np.empty((self.n_samples, px.size))
+        _check(lib().tsm_probe_density_noalloc(self._h, px.size, _dptr(px), _dptr(py), _dptr(out)))
+        return out
 
     def global_all(self):
         """Kinetic energy and entropy integrals of every sample: two (n_samples,) arrays."""

@@ -36,6 +36,7 @@ struct tsm_ctx {
   int S = 1;
   tsm::Geometry geo;
   hipStream_t st = nullptr;
+  int dev = 0;              // device current at tsm_create (holds the buffers and the stream)
   tsm::DevGeo G{};
   tsm::StageArgs A{};
   tsm::StepParams SP{};
@@ -335,6 +336,7 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   }
   if (vbe.empty()) vbe.push_back(0);
   // ---- device ------------------------------------------------------------
+  if (hipGetDevice(&c->dev) != hipSuccess) { delete c; return fail(TSM_E_HIP, "hipGetDevice failed"); }
   hipError_t he = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
   if (he != hipSuccess) { delete c; return fail(TSM_E_HIP, "hipStreamCreate failed"); }
   tsm::DevGeo& G = c->G;
@@ -1275,6 +1277,60 @@ extern "C" int tsm_probe_density(tsm_ctx* c, int np, const double* px, const dou
   if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)np * c->S * sizeof(double), hipMemcpyDeviceToHost);
   (void)hipFree(dpc); (void)hipFree(dpw); (void)hipFree(dout);
   if (e != hipSuccess) return fail(TSM_E_HIP, hipGetErrorString(e));
+  return TSM_OK;
+}
+
+extern "C" int tsm_probe_density_noalloc(tsm_ctx* c, int np, const double* px, const double* py,
+                                         double* out) {
+  if (!c || np < 1 || !px || !py || !out) return fail(TSM_E_ARG, "bad argument");
+  // scratch layout: px, py, weights (3 per probe), densities (one per probe and
+  // sample), corner ids (3 int32 per probe), containing cell (one int32 per
+  // probe, -1 outside the mesh); each part 256-byte aligned
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t n = (size_t)np;
+  const size_t o_py = up(8 * n), o_pw = o_py + up(8 * n), o_out = o_pw + up(24 * n);
+  const size_t o_pc = o_out + up(8 * n * (size_t)c->S), o_cell = o_pc + up(12 * n);
+  const size_t need = o_cell + up(4 * n);
+  if (need > tsm::noalloc_scratch_bytes())
+    return fail(TSM_E_ARG, "tsm_probe_density_noalloc: " + std::to_string(np) + " probes x " +
+                               std::to_string(c->S) + " samples need " + std::to_string(need) +
+                               " bytes of scratch, more than its fixed " +
+                               std::to_string(tsm::noalloc_scratch_bytes()));
+  int prev = -1;
+  HIP_TRY(hipGetDevice(&prev));
+  if (prev != c->dev) HIP_TRY(hipSetDevice(c->dev));
+  struct Restore {  // the caller's device again on every return
+    int d, p;
+    ~Restore() { if (d != p) (void)hipSetDevice(p); }
+  } restore{c->dev, prev};
+  void* base = nullptr;
+  int he = tsm::noalloc_scratch(&base);
+  if (he) return fail(TSM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString((hipError_t)he));
+  char* b = static_cast<char*>(base);
+  double* dpx = reinterpret_cast<double*>(b);
+  double* dpy = reinterpret_cast<double*>(b + o_py);
+  double* dpw = reinterpret_cast<double*>(b + o_pw);
+  double* dout = reinterpret_cast<double*>(b + o_out);
+  int* dpc = reinterpret_cast<int*>(b + o_pc);
+  int* dcell = reinterpret_cast<int*>(b + o_cell);
+  HIP_TRY(hipMemcpyAsync(dpx, px, 8 * n, hipMemcpyHostToDevice, c->st));
+  HIP_TRY(hipMemcpyAsync(dpy, py, 8 * n, hipMemcpyHostToDevice, c->st));
+  tsm::launch_probe_locate(c->G, np, dpx, dpy, dpc, dpw, dcell, c->st);
+  HIP_TRY(hipGetLastError());
+  c->bind_state();
+  tsm::launch_probes(c->A, c->G.nv, np, dpc, dpw, c->S, dout, c->st);
+  HIP_TRY(hipGetLastError());
+  // the containing cells first (into the caller's output, which is large
+  // enough: 8 np S >= 4 np bytes), to report points outside the mesh
+  HIP_TRY(hipMemcpyAsync(out, dcell, 4 * n, hipMemcpyDeviceToHost, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
+  const int* cells = reinterpret_cast<const int*>(out);
+  for (int p = 0; p < np; ++p)
+    if (cells[p] < 0)
+      return fail(TSM_E_ARG, "probe point " + std::to_string(p) + " outside the mesh");
+  HIP_TRY(hipMemcpyAsync(out, dout, 8 * n * (size_t)c->S, hipMemcpyDeviceToHost, c->st));
+  HIP_TRY(hipStreamSynchronize(c->st));
   return TSM_OK;
 }
 

@@ -321,5 +321,11 @@ void launch_probes(const StageArgs& A, int nv, int np, const int* pc, const doub
                    double* out, hipStream_t st);
 void launch_global_all(const DevGeo& G, const StageArgs& A, int S, double* out, hipStream_t st);
 void launch_check_arith(long n, const double* a, const double* b, double* out, hipStream_t st);
+// tsm_noalloc.hip: the fixed scratch array (its device address on the current
+// device; 0 or a hipError_t) and the device point location of probe points
+size_t noalloc_scratch_bytes();
+int noalloc_scratch(void** out);
+void launch_probe_locate(const DevGeo& G, int np, const double* px, const double* py, int* pc,
+                         double* pw, int* cell, hipStream_t st);
 
 }  // namespace tsm
