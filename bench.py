@@ -94,18 +94,29 @@ def main():
     ap.add_argument("--mc", action="store_true",
                     help="second metric: surrogate control-variate MC against plain MC on the "
                          "uncertain isentropic vortex (prints its own JSON line)")
-    ap.add_argument("--mc-fine", type=int, default=129)
-    ap.add_argument("--mc-coarse", type=int, default=129, help="mesh of the surrogate's training samples")
-    ap.add_argument("--mc-time", type=float, default=10.0)
-    ap.add_argument("--mc-train", type=int, default=512)
-    ap.add_argument("--mc-pilot", type=int, default=64)
-    ap.add_argument("--mc-pair", type=int, default=1024)
-    ap.add_argument("--mc-ref", type=int, default=4096)
+    ap.add_argument("--mc-fine", type=int, default=None,
+                    help="fine mesh (vortex 129, riemann 1025)")
+    ap.add_argument("--mc-coarse", type=int, default=None,
+                    help="mesh of the surrogate's training samples (vortex 129, riemann 257)")
+    ap.add_argument("--mc-time", type=float, default=None, help="t_final (vortex 10, riemann 0.05)")
+    ap.add_argument("--mc-train", type=int, default=None, help="training samples (512 / 2048)")
+    ap.add_argument("--mc-pilot", type=int, default=None,
+                    help="fine samples fitting the control coefficients (64 / 0: fitted on the "
+                         "training samples)")
+    ap.add_argument("--mc-pair", type=int, default=None, help="paired fine samples (1024 / 64)")
+    ap.add_argument("--mc-ref", type=int, default=None,
+                    help="plain-MC reference samples on the fine mesh (4096 / 512)")
+    ap.add_argument("--mc-fine-batch", type=int, default=None,
+                    help="fine samples per solver context (riemann: 32)")
     ap.add_argument("--mlmc", action="store_true",
                     help="multilevel MC of the uncertain vortex against plain MC at equal "
                          "GPU-seconds (prints its own JSON line)")
     ap.add_argument("--mlmc-levels", default="129,257")
     ap.add_argument("--mlmc-samples", default="512,64")
+    ap.add_argument("--mlmc-throughput", action="store_true",
+                    help="vertex-stage updates/s of the multilevel batch (levels --mlmc-levels with "
+                         "--mlmc-samples samples, uncertain vortex) stepped by packed launches "
+                         "against per-level launches interleaved on streams (prints its own JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -127,6 +138,9 @@ def main():
         return
     if args.mlmc:
         mlmc_bench(args)
+        return
+    if args.mlmc_throughput:
+        mlmc_throughput(args)
         return
 
     import numpy as np
@@ -167,7 +181,10 @@ def main():
     s.set_state(w0)
     geo = s.geometry()
     nv, nf, nc = mesh.n_vertex, geo["face_v"].shape[0], mesh.cell_v.size // 3
+    # warm-up, per-kernel durations (HIP events on the library's stream, 20
+    # steps plus 20 more without events), then the timed steps
     s.run(args.warmup)
+    ku = s.bench(20)
     # timed region: K whole steps, synchronised on both sides (tsm_run
     # synchronises the stream before returning)
     t_before, it_before = s.time()
@@ -181,8 +198,6 @@ def main():
     print(f"bench: {args.steps} steps in {wall:.3f} s, {ms_per_step:.3f} ms/step", file=sys.stderr,
           flush=True)
 
-    # per-kernel durations (HIP events on the library's stream)
-    ku = s.bench(20)
     tiles = T.host_tiling(mesh, prm.tile_vertices or 128, two_rings=args.fused_gradient)
     kb = kernel_bytes(nv, nf, nc, tiles)
     if ku["dt_us"] == 0.0 and tiles["two_rings"]:  # fused stage kernel
@@ -240,17 +255,36 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+MC_DEFAULTS = {  # fine, coarse, t_final, train, pilot, pair, ref, fine batch
+    "vortex": dict(fine=129, coarse=129, time=10.0, train=512, pilot=64, pair=1024, ref=4096,
+                   fine_batch=None),
+    "riemann": dict(fine=1025, coarse=257, time=0.05, train=2048, pilot=0, pair=64, ref=512,
+                    fine_batch=32),
+}
+
+
 def mc_bench(args):
-    """Surrogate control variate vs plain MC (vortex with uncertain strength,
-    centre, gamma and Mach; QoIs: 64 probe densities, kinetic energy, entropy
-    integral).  value = GPU-seconds plain MC needs for the control variate's
-    standard error divided by the control variate's GPU-seconds (training
-    samples, fit, fine samples and surrogate evaluations all charged)."""
+    """Surrogate control variate vs plain MC.  vortex: uncertain strength,
+    centre, gamma and Mach; riemann: the four-quadrant Riemann problem with
+    uncertain split (U(0.75,0.85)^2) and quadrant pressures (x U(0.9,1.1)),
+    surrogate trained on 257^2 solves and paired with 1025^2 solves.  QoIs: 64
+    probe densities, kinetic energy, entropy integral.  value = GPU-seconds
+    plain MC needs for the control variate's standard error divided by the
+    control variate's GPU-seconds (training samples, fit, fine samples and
+    surrogate evaluations all charged)."""
     from tensum_amd import uq
-    p = uq.vortex_problem(final_time=args.mc_time)
+    d = dict(MC_DEFAULTS[args.workload if args.workload in MC_DEFAULTS else "vortex"])
+    for k, a in (("fine", args.mc_fine), ("coarse", args.mc_coarse), ("time", args.mc_time),
+                 ("train", args.mc_train), ("pilot", args.mc_pilot), ("pair", args.mc_pair),
+                 ("ref", args.mc_ref), ("fine_batch", args.mc_fine_batch)):
+        if a is not None:
+            d[k] = a
+    riemann = args.workload == "riemann"
+    p = uq.riemann_problem(final_time=d["time"]) if riemann else uq.vortex_problem(final_time=d["time"])
     say = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
-    r = uq.cv_experiment(p, args.mc_fine, args.mc_coarse, args.mc_train, args.mc_pilot,
-                         args.mc_pair, 65536, args.mc_ref, log=say)
+    r = uq.cv_experiment(p, d["fine"], d["coarse"], d["train"], d["pilot"], d["pair"], 65536,
+                         d["ref"], log=say, fine_batch=d["fine_batch"])
+    nin = len(p.bounds)
     out = {
         "metric": "MC moment error per GPU-second (control variate vs plain MC, equal error)",
         "value": r["speedup_at_equal_error"],
@@ -258,9 +292,13 @@ def mc_bench(args):
         "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 solver, bf16/fp32 surrogate",
         "data": "synthetic (seeded jittered meshes, scrambled Sobol inputs)",
-        "config": {"workload": f"isentropic vortex UQ, t_final {args.mc_time}",
-                   "fine_mesh": args.mc_fine, "coarse_mesh": args.mc_coarse,
-                   "surrogate": "MLP 5-256-256-256-66, GELU, BF16 MFMA, Adam"},
+        "config": {"workload": ("2D Riemann UQ (split ~U(0.75,0.85)^2, quadrant pressures "
+                                "x U(0.9,1.1)), KEPES-TeCNO Rusanov, minmod, CFL 0.5"
+                                if riemann else "isentropic vortex UQ") + f", t_final {d['time']}",
+                   "fine_mesh": d["fine"], "coarse_mesh": d["coarse"], "train_samples": d["train"],
+                   "paired_samples": d["pair"], "pilot_samples": d["pilot"],
+                   "surrogate_evaluations": 65536, "reference_samples": d["ref"],
+                   "surrogate": f"MLP {nin}-256-256-256-66, GELU, BF16 MFMA, FP32 accumulation, Adam"},
         "detail": r,
     }
     print(json.dumps(out), flush=True)
@@ -268,19 +306,65 @@ def mc_bench(args):
 
 def mlmc_bench(args):
     from tensum_amd import uq
-    p = uq.vortex_problem(final_time=args.mc_time)
+    p = uq.vortex_problem(final_time=args.mc_time if args.mc_time is not None else 10.0)
     levels = tuple(int(x) for x in args.mlmc_levels.split(","))
     samples = tuple(int(x) for x in args.mlmc_samples.split(","))
     say = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
-    r = uq.mlmc_experiment(p, levels, samples, args.mc_ref, log=say)
+    r = uq.mlmc_experiment(p, levels, samples, args.mc_ref if args.mc_ref is not None else 4096,
+                           log=say)
     out = {"metric": "MLMC moment error at the GPU-seconds of plain MC (uncertain vortex)",
            "value": r["plain_mc"]["mean_error"] / max(r["mlmc"]["mean_error"], 1e-300),
            "unit": "plain-MC mean error / MLMC mean error at equal GPU-seconds",
            "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic (seeded jittered meshes, scrambled Sobol inputs)",
-           "config": {"workload": f"isentropic vortex UQ, t_final {args.mc_time}",
+           "config": {"workload": f"isentropic vortex UQ, t_final {p.params.final_time}",
                       "levels": levels, "samples": samples},
            "detail": r}
+    print(json.dumps(out), flush=True)
+
+
+def mlmc_throughput(args):
+    """Throughput of one multilevel batch: every level's samples in one
+    context, advanced by packed launches spanning all levels (tsm_run_packed)
+    and, for comparison, by each level's own launches interleaved on per-level
+    streams (tsm_run_group).  Both runs start from the same states; the packed
+    states are checked equal to the interleaved ones bit for bit."""
+    import numpy as np
+
+    import tensum_amd as T
+    from tensum_amd import uq
+    p = uq.vortex_problem(final_time=1e9)
+    levels = tuple(int(x) for x in args.mlmc_levels.split(","))
+    samples = tuple(int(x) for x in args.mlmc_samples.split(","))
+    say = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
+    sets = {}
+    for mode in ("packed", "group"):
+        sets[mode] = []
+        for n, k in zip(levels, samples):
+            inp = uq.sample_inputs(k, p.bounds, seed=n)
+            sets[mode].append(uq.make_batch(p, p.mesh(n), inp))
+        say(f"{mode}: {len(levels)} levels built")
+    vs = sum(s.nv * s.n_samples for s in sets["packed"])
+    res = {}
+    for mode, fn in (("packed", T.run_packed), ("group", T.run_group)):
+        fn(sets[mode], args.warmup)
+        t0 = time.perf_counter()
+        done = fn(sets[mode], args.steps)
+        wall = time.perf_counter() - t0
+        assert all(int(d) == args.steps * s.n_samples for d, s in zip(done, sets[mode]))
+        res[mode] = {"vertex_stage_per_s": vs * 3 * args.steps / wall, "ms_per_step": 1e3 * wall / args.steps}
+        say(f"{mode}: {res[mode]['ms_per_step']:.2f} ms per step of all levels")
+    same = all(np.array_equal(a.get_state(k), b.get_state(k)) for a, b in zip(sets["packed"], sets["group"])
+               for k in range(a.n_samples))
+    out = {"metric": "FP64 vertex-stage updates/s of a multilevel batch (packed launches)",
+           "value": res["packed"]["vertex_stage_per_s"], "unit": "vertex-stage updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded jittered meshes, scrambled Sobol inputs)",
+           "config": {"workload": "uncertain isentropic vortex, KEPES-TeCNO Roe, minmod, SSP-RK3, CFL 0.8",
+                      "levels": levels, "samples": samples, "vertex_samples": vs},
+           "per_level_streams": res["group"], "packed": res["packed"],
+           "packed_over_streams": res["packed"]["vertex_stage_per_s"] / res["group"]["vertex_stage_per_s"],
+           "states_identical": bool(same)}
     print(json.dumps(out), flush=True)
 
 

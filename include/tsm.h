@@ -202,6 +202,16 @@ int tsm_step(tsm_ctx* ctx, double t_target, double* dt_out, double* res_norm_out
    No counterpart in the reference, which runs one sample at a time
    (src/fv.cc:2440-2459). */
 int tsm_run_group(tsm_ctx** ctxs, int n, long nsteps, long* done);
+/* tsm_run_group with every kernel of a step launched once over the tiles and
+   samples of all n contexts (one grid spanning every level's meshes, block
+   ranges per level; the levels' uneven sizes packed into the same launches),
+   on the first context's stream.  Each block computes what it computes in its
+   own context's launch, so the results are those of tsm_run bit for bit.
+   The contexts must run the Euler equations with SSP-RK3 on the tiled kernels
+   (gradients in a pass of their own) and share the flux and reconstruction;
+   TSM_E_ARG otherwise.  No counterpart in the reference, which runs its
+   samples one after another (src/fv.cc:2440-2459). */
+int tsm_run_packed(tsm_ctx** ctxs, int n, long nsteps, long* done);
 int tsm_time(tsm_ctx* ctx, int sample, double* t, long* iter);
 /* Loop state of one sample: v[0] t, v[1] dt of the last/next step, v[2]
    res_total; iter; counter = index of the next output time stamp

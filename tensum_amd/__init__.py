@@ -211,6 +211,7 @@ def lib():
         L.tsm_sobol.argtypes = [C.c_long, C.c_int, C.c_long, C.c_ulonglong, C.c_int, C.c_void_p]
         L.tsm_step.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
         L.tsm_run_group.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_void_p]
+        L.tsm_run_packed.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_void_p]
         L.tsm_pert.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int,
                                C.c_void_p]
         L.tsm_moments.argtypes = [C.c_void_p] + [C.c_void_p] * 3
@@ -406,6 +407,16 @@ def pert(ptype, x, y, p, rnd):
     out = C.c_double()
     _check(lib().tsm_pert(ptype, x, y, p, _dptr(r), r.size, C.byref(out)))
     return out.value
+
+
+def run_packed(solvers, nsteps):
+    """Advances several Solvers (the levels of a multilevel run) with one
+    launch per kernel spanning all of them; returns the steps each performed,
+    summed over its samples.  Euler, SSP-RK3, tiled kernels, one scheme."""
+    hs = (C.c_void_p * len(solvers))(*[s._h for s in solvers])
+    done = np.zeros(len(solvers), dtype=np.int64)
+    _check(lib().tsm_run_packed(hs, len(solvers), nsteps, _dptr(done)))
+    return done
 
 
 def run_group(solvers, nsteps):

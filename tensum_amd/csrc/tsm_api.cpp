@@ -66,6 +66,10 @@ struct tsm_ctx {
   // captured steps of the tiled path (graph_steps > 0)
   hipGraphExec_t gexec = nullptr;
   int gcur = -1;
+  // device copies of the level descriptors of packed multilevel steps
+  // (tsm_run_packed with this context first)
+  tsm::PackLevel* pack = nullptr;
+  size_t pack_cap = 0;
 
   ~tsm_ctx() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -307,8 +311,9 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
       for (int j = 0; j < 3; ++j) {
         // slots keep their indices into the face list as it was before Voronoi
         // duals lost their empty faces (src/grid_preproc.cc:633-647, :811-831);
-        // boundary faces come first and are never removed, so a slot past the
-        // shortened list, like any slot past the boundary faces, is interior
+        // boundary faces come first and are never removed (build_geometry
+        // refuses a dual that would lose one), so a slot past the shortened
+        // list, like any slot past the boundary faces, is interior
         int f = g.cell_slot_face[3 * k + j];
         if (f < 0 || f >= nf || face_bface[f] < 0) continue;
         if (t == 2) { delete c; return fail(TSM_E_MESH, "cell with three boundary edges"); }
@@ -925,6 +930,88 @@ extern "C" int tsm_run_group(tsm_ctx** cs, int n, long nsteps, long* done) {
     tsm_ctx* c = cs[i];
     std::vector<tsm::Ctl> after(c->S);
     HIP_TRY(hipStreamSynchronize(c->st));
+    HIP_TRY(hipMemcpy(after.data(), c->A.ctl, c->S * sizeof(tsm::Ctl), hipMemcpyDeviceToHost));
+    long tot = 0;
+    for (int s = 0; s < c->S; ++s) tot += (long)(after[s].iter - before[i][s].iter);
+    if (done) done[i] = tot;
+  }
+  return TSM_OK;
+}
+
+extern "C" int tsm_run_packed(tsm_ctx** cs, int n, long nsteps, long* done) {
+  if (!cs || n < 1 || nsteps < 0) return fail(TSM_E_ARG, "bad argument");
+  for (int i = 0; i < n; ++i) {
+    const tsm_ctx* c = cs[i];
+    if (!c) return fail(TSM_E_ARG, "null context in group");
+    if (!c->tiled || c->TG.fused || c->P.time_scheme != 0 || c->A.ns)
+      return fail(TSM_E_ARG, "packed steps need Euler SSP-RK3 contexts on the tiled kernels "
+                             "with a gradient pass of their own");
+    if (c->P.flux != cs[0]->P.flux || c->P.reconstruct != cs[0]->P.reconstruct)
+      return fail(TSM_E_ARG, "packed contexts must share the flux and the reconstruction");
+    for (int j = 0; j < i; ++j)
+      if (cs[j] == c) return fail(TSM_E_ARG, "a context appears twice in the group");
+  }
+  tsm_ctx* c0 = cs[0];
+  hipStream_t st = c0->st;
+  std::vector<std::vector<tsm::Ctl>> before(n);
+  for (int i = 0; i < n; ++i) {
+    tsm_ctx* c = cs[i];
+    before[i].resize(c->S);
+    HIP_TRY(hipStreamSynchronize(c->st));
+    HIP_TRY(hipMemcpy(before[i].data(), c->A.ctl, c->S * sizeof(tsm::Ctl), hipMemcpyDeviceToHost));
+  }
+  // level descriptors of the three stages: every stage reads W^n for U_old and
+  // the last writes W^{n+1} over it (the buffer rotation of enqueue_tiled_step)
+  std::vector<tsm::PackLevel> lv(3 * (size_t)n);
+  tsm::PackPlan plan{};
+  for (int r = 0; r < 3; ++r) {
+    for (int i = 0; i < n; ++i) {
+      tsm_ctx* c = cs[i];
+      double* Wn = c->W[c->cur];
+      double* Wa = c->W[(c->cur + 1) % 3];
+      double* Wb = c->W[(c->cur + 2) % 3];
+      const double* ins[3] = {Wn, Wa, Wb};
+      double* outs[3] = {Wa, Wb, Wn};
+      tsm::PackLevel& L = lv[(size_t)r * n + i];
+      L.G = c->G;
+      L.TG = c->TG;
+      L.A = c->A;
+      L.A.W_in = ins[r];
+      L.A.W_out = outs[r];
+      L.P = c->SP;
+      L.Wn = Wn;
+      L.dtpart = c->dtpart;
+      L.S = c->S;
+    }
+    tsm::pack_plan(&lv[(size_t)r * n], n, c0->P.flux, c0->P.reconstruct, plan);
+  }
+  if (c0->pack_cap < lv.size()) {
+    int rc = c0->alloc(&c0->pack, lv.size());
+    if (rc) return rc;
+    c0->pack_cap = lv.size();
+  }
+  HIP_TRY(hipMemcpyAsync(c0->pack, lv.data(), lv.size() * sizeof(tsm::PackLevel),
+                         hipMemcpyHostToDevice, st));
+  for (int i = 0; i < n; ++i) {  // the first step's dt and loop test, per context
+    tsm_ctx* c = cs[i];
+    c->bind_state();
+    if (nsteps > 0) tsm::launch_dt(c->G, c->A, c->SP, c->S, st);
+  }
+  for (long k = 0; k < nsteps; ++k) {
+    for (int r = 0; r < 3; ++r) {
+      const tsm::PackLevel* d = c0->pack + (size_t)r * n;
+      tsm::launch_packed_gradient(d, plan, st);
+      tsm::launch_packed_stage(c0->P.flux, c0->P.reconstruct, d, plan, r, r == 2, st);
+    }
+    tsm::launch_packed_final(c0->pack, plan, st);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int i = 0; i < n; ++i) {
+    tsm_ctx* c = cs[i];
+    c->bind_state();
+    c->e_valid = false;
+    std::vector<tsm::Ctl> after(c->S);
     HIP_TRY(hipMemcpy(after.data(), c->A.ctl, c->S * sizeof(tsm::Ctl), hipMemcpyDeviceToHost));
     long tot = 0;
     for (int s = 0; s < c->S; ++s) tot += (long)(after[s].iter - before[i][s].iter);

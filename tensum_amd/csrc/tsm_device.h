@@ -272,6 +272,34 @@ struct StageArgs {
   int scheme;      // 0 SSP-RK3, 1 RK4, 2 Heun (the tiled stage kernel's update)
 };
 
+// One context (a level of a multilevel run) in a packed launch: its geometry,
+// tiling, stage arguments (the stage's input and output buffers) and step
+// parameters, and where its blocks start in each packed grid.
+struct PackLevel {
+  DevGeo G;
+  TileGeo TG;
+  StageArgs A;
+  StepParams P;
+  const double* Wn;   // W^n (read by every stage for U_old)
+  double* dtpart;     // per-block dt partials
+  int S;              // samples
+  int base_fu, base_gper, base_grest, base_gbnd, base_final;
+};
+// Grid totals of a packed step over nl levels (computed by pack_plan).
+struct PackPlan {
+  int nl;
+  int fu_blocks, gper_blocks, grest_blocks, gbnd_blocks, final_blocks;
+  int fu_block, grad_block;   // threads per block: the largest any level needs
+  size_t fu_lds, grad_lds;    // dynamic LDS: the largest any level needs
+  bool grad_pass;
+};
+// Fills the block bases of lv[0..nl) and the plan (host).
+void pack_plan(PackLevel* lv, int nl, int flux, int rec, PackPlan& plan);
+void launch_packed_gradient(const PackLevel* d, const PackPlan& p, hipStream_t st);
+void launch_packed_stage(int flux, int rec, const PackLevel* d, const PackPlan& p, int r,
+                         bool last, hipStream_t st);
+void launch_packed_final(const PackLevel* d, const PackPlan& p, hipStream_t st);
+
 // LDS-staged tile kernels (Euler, SSP-RK3)
 size_t tile_grad_lds(const TileGeo& T);
 int tile_grad_block(const TileGeo& T);

@@ -73,6 +73,16 @@ __device__ __forceinline__ double entropy(const Gas& g, const Prim& w) {
 // xi = b/a and divides the series terms; forming f with one division and
 // multiplying by the reciprocals gives the same value to within an ulp with
 // two divisions instead of five on the series branch.
+// F(f) of the logarithmic mean of a and b for a given f = (b - a)/(b + a)
+__device__ __forceinline__ double logavg_F(double f, double a, double b) {
+  const double u = f * f;
+  if (u < 1.0e-2) {
+    const double u2 = u * u;
+    const double u3 = u2 * u;
+    return 1.0 + u * (1.0 / 3.0) + u2 * (1.0 / 5.0) + u3 * (1.0 / 7.0);
+  }
+  return fdiv(log(fdiv(b, a)) * 0.5, f);
+}
 __device__ __forceinline__ double logavg(double a, double b) {
   const double sum = a + b;
   const double f = fdiv(b - a, sum);
@@ -335,10 +345,14 @@ __device__ __forceinline__ Cons tecno_flux(const Gas& g, const FluxOpts& o, doub
   double rhol = L.rho, rhor = Rr.rho;
   double rho = logavg(rhol, rhor);
   double betal = L.beta, betar = Rr.beta;
-  double beta = logavg(betal, betar);
-  const double ib = fdiv(1.0, beta);
+  // beta's logarithmic mean enters only through 1/beta_hat: with
+  // r = 1/(beta_l + beta_r), f = (beta_r - beta_l) r and beta_hat = 1/(2 r F(f)),
+  // so 1/beta_hat = 2 F(f) r, and p_hat = (rho_l + rho_r) r / 2: one division
+  // where the reference's expressions take four (same values to rounding)
+  const double rb = fdiv(1.0, betal + betar);
+  const double ib = 2.0 * logavg_F((betar - betal) * rb, betal, betar) * rb;
   double a = fsqrt(0.5 * gam * ib);
-  double p = fdiv(0.5 * (rhol + rhor), betal + betar);
+  double p = (0.5 * (rhol + rhor)) * rb;
   double vx = l.u * 0.5 + r.u * 0.5, vy = l.v * 0.5 + r.v * 0.5;
   double vn = vx * unx + vy * uny;
   double vel2 = 0.5 * ((l.u * l.u + l.v * l.v) + (r.u * r.u + r.v * r.v));
@@ -355,16 +369,6 @@ __device__ __forceinline__ Cons tecno_flux(const Gas& g, const FluxOpts& o, doub
   double vsq = vx * vx + vy * vy;
   double H = a * a * ig1 + 0.5 * vsq;
   double v1 = vx * uny - vy * unx;
-  double const1 = fsqrt(rho * igam);
-  double psq = fsqrt(p);
-  double s0 = sqrt(0.5) * const1, s1 = fsqrt(gam - 1.0) * const1;
-  // scaled right eigenvectors, Rm[row][col]; rows: mass, x-mom, y-mom, energy
-  double Rm[4][4];
-  Rm[0][0] = 1 * s0;               Rm[0][1] = 1 * s1;     Rm[0][2] = 0 * psq;     Rm[0][3] = 1 * s0;
-  Rm[1][0] = (vx - a * unx) * s0;  Rm[1][1] = vx * s1;    Rm[1][2] = uny * psq;   Rm[1][3] = (vx + a * unx) * s0;
-  Rm[2][0] = (vy - a * uny) * s0;  Rm[2][1] = vy * s1;    Rm[2][2] = -unx * psq;  Rm[2][3] = (vy + a * uny) * s0;
-  Rm[3][0] = (H - a * vn) * s0;    Rm[3][1] = (0.5 * vsq) * s1; Rm[3][2] = v1 * psq; Rm[3][3] = (H + a * vn) * s0;
-
   double Lam[4];
   if (ROE) {
     double vnl = l.u * unx + l.v * uny, vnr = r.u * unx + r.v * uny;
@@ -381,35 +385,95 @@ __device__ __forceinline__ Cons tecno_flux(const Gas& g, const FluxOpts& o, doub
   }
   const double* Vl = L.V;
   const double* Vr = Rr.V;
-  double h = 0.0;
-  if (REC == R_TVB) h = fsqrt(rx * rx + ry * ry);
-  double dZ[4];
+  double D[4];
+  if (REC == R_ALBADA || REC == R_TVB) {
+    // Limiters that are not positively homogeneous (van Albada's epsilon, the
+    // TVB threshold M h^2) see the characteristic variables of the scaled
+    // eigenvectors, as the reference forms them (src/kepes_tecno_roe.cc:60-73).
+    double const1 = fsqrt(rho * igam);
+    double psq = fsqrt(p);
+    double s0 = sqrt(0.5) * const1, s1 = fsqrt(gam - 1.0) * const1;
+    // scaled right eigenvectors, Rm[row][col]; rows: mass, x-mom, y-mom, energy
+    double Rm[4][4];
+    Rm[0][0] = 1 * s0;               Rm[0][1] = 1 * s1;     Rm[0][2] = 0 * psq;     Rm[0][3] = 1 * s0;
+    Rm[1][0] = (vx - a * unx) * s0;  Rm[1][1] = vx * s1;    Rm[1][2] = uny * psq;   Rm[1][3] = (vx + a * unx) * s0;
+    Rm[2][0] = (vy - a * uny) * s0;  Rm[2][1] = vy * s1;    Rm[2][2] = -unx * psq;  Rm[2][3] = (vy + a * uny) * s0;
+    Rm[3][0] = (H - a * vn) * s0;    Rm[3][1] = (0.5 * vsq) * s1; Rm[3][2] = v1 * psq; Rm[3][3] = (H + a * vn) * s0;
+    double h = 0.0;
+    if (REC == R_TVB) h = fsqrt(rx * rx + ry * ry);
+    double dZ[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    double zl = 0.0, zr = 0.0, sgl = 0.0, sgr = 0.0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      zl += Rm[j][i] * Vl[j];
-      zr += Rm[j][i] * Vr[j];
-    }
-    if (REC != R_FIRST) {
+    for (int i = 0; i < 4; ++i) {
+      double zl = 0.0, zr = 0.0, sgl = 0.0, sgr = 0.0;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
+        zl += Rm[j][i] * Vl[j];
+        zr += Rm[j][i] * Vr[j];
         sgl += Rm[j][i] * gdl[j];
         sgr += Rm[j][i] * gdr[j];
       }
+      double zij, zji;
+      reconstruct1<REC>(zl, zr, sgl, sgr, h, o.tvbM, zij, zji);
+      dZ[i] = zji - zij;
     }
-    double zij, zji;
-    reconstruct1<REC>(zl, zr, sgl, sgr, h, o.tvbM, zij, zji);
-    dZ[i] = zji - zij;
-  }
-  double D[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    double d = 0.0;
+    for (int i = 0; i < 4; ++i) {
+      double d = 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) d += Rm[i][j] * Lam[j] * dZ[j];
-    D[i] = d;
+      for (int j = 0; j < 4; ++j) d += Rm[i][j] * Lam[j] * dZ[j];
+      D[i] = d;
+    }
+  } else {
+    // Positively homogeneous reconstructions (first, second, MUSCL, minmod,
+    // ENO2: phi(c a, c b) = c phi(a, b) for c > 0): with R = R0 diag(s),
+    // s = (sqrt(rho/(2 gam)), sqrt((gam-1) rho/gam), sqrt(p), sqrt(rho/(2 gam))),
+    // the reconstructed jump in the scaled characteristic variables is s times
+    // the one formed with the unscaled eigenvectors R0, so
+    // D = R Lam dZ = R0 (s^2 Lam dZ0): no square roots of the scalings, the
+    // projections exploit R0's zeros and ones, and only the jump
+    // dz0 = R0^T (V_r - V_l) enters (src/reconstruct.cc:13-96 written as
+    // z_ji - z_ij).  Same values to rounding.
+    const double hv = 0.5 * vsq;
+    auto proj = [&](double x0, double x1, double x2, double x3, double* z) {
+      const double b = x0 + vx * x1 + vy * x2;
+      const double t = b + H * x3;
+      const double u = a * (unx * x1 + uny * x2 + vn * x3);
+      z[0] = t - u;
+      z[1] = b + hv * x3;
+      z[2] = uny * x1 - unx * x2 + v1 * x3;
+      z[3] = t + u;
+    };
+    double dz[4], gl[4], gr[4];
+    proj(Vr[0] - Vl[0], Vr[1] - Vl[1], Vr[2] - Vl[2], Vr[3] - Vl[3], dz);
+    if (REC != R_FIRST) {
+      proj(gdl[0], gdl[1], gdl[2], gdl[3], gl);
+      proj(gdr[0], gdr[1], gdr[2], gdr[3], gr);
+    }
+    const double rg = rho * igam;
+    const double q[4] = {0.5 * rg, (gam - 1.0) * rg, p, 0.5 * rg};
+    double w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double dZi;
+      if (REC == R_FIRST) {
+        dZi = dz[i];
+      } else if (REC == R_SECOND) {
+        dZi = dz[i] - 0.5 * (gl[i] + gr[i]);
+      } else if (REC == R_MUSCL) {
+        const double K = 1.0 / 3.0;
+        const double dl = 2.0 * gl[i] - dz[i], dr = 2.0 * gr[i] - dz[i];
+        dZi = dz[i] - ((dl + dr) * (1.0 - K) + (2.0 * dz[i]) * (1.0 + K)) * 0.25;
+      } else {
+        const double dl = 2.0 * gl[i] - dz[i], dr = 2.0 * gr[i] - dz[i];
+        dZi = dz[i] - limited_slope<REC>(dl, dz[i], 0.0, 0.0) - limited_slope<REC>(dr, dz[i], 0.0, 0.0);
+      }
+      w[i] = (q[i] * Lam[i]) * dZi;
+    }
+    const double w13 = w[0] + w[1] + w[3], d30 = w[3] - w[0];
+    D[0] = w13;
+    D[1] = vx * w13 + a * unx * d30 + uny * w[2];
+    D[2] = vy * w13 + a * uny * d30 - unx * w[2];
+    D[3] = H * (w[0] + w[3]) + hv * w[1] + a * vn * d30 + v1 * w[2];
   }
   F.m -= 0.5 * D[0];
   F.mx -= 0.5 * D[1];

@@ -53,6 +53,22 @@ __device__ __forceinline__ void bsum(double* v, double* red) {
     }
   }
 }
+// XCD-aware tile order.  Workgroups go to the chip's 8 XCDs in turn (block b
+// to XCD b mod 8), each XCD with its own L2.  Block b of a row of n tiles
+// takes tile start(x) + b/8 of the contiguous range of XCD x = b mod 8, so
+// every XCD walks a compact run of neighbouring tiles and the halo records
+// it reads are mostly the owned records of tiles it has just staged (a
+// bijection of [0, n): XCD x gets n/8 tiles, one more for the first n mod 8).
+#ifndef TSM_XCDS
+#define TSM_XCDS 8
+#endif
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  if (TSM_XCDS <= 1) return b;
+  const int x = b % TSM_XCDS, j = b / TSM_XCDS;
+  const int q = n / TSM_XCDS, rem = n % TSM_XCDS;
+  return x * q + min(x, rem) + j;
+}
+
 // Face entries of a vertex's gather held in registers (see k_fu_tile).
 constexpr int GE = 12;
 
@@ -214,10 +230,9 @@ __device__ __forceinline__ void cell_grad(double xa, double ya, double xb, doubl
 // boundary-corrected cell are left to k_grad_bnd, which keeps this kernel's
 // register count low enough for eight waves per SIMD.
 template <bool PER>
-__global__ __launch_bounds__(512, PER ? 4 : 8) void k_grad_tile(DevGeo G, TileGeo TG, StageArgs A,
-                                                                int t0) {
+__device__ __forceinline__ void grad_tile_body(const DevGeo& G, const TileGeo& TG,
+                                               const StageArgs& A, int t, int s) {
   extern __shared__ double lds[];
-  const int t = blockIdx.x + t0, s = blockIdx.y;
   if (A.ctl[s].done) return;
   const int MS = TG.max_slots, TB = TG.tb;
   double* SE = lds;            // 4 x MS entropy variables
@@ -280,6 +295,25 @@ __global__ __launch_bounds__(512, PER ? 4 : 8) void k_grad_tile(DevGeo G, TileGe
     for (int q = 0; q < 8; ++q) dEv[q] = acc[q] * fact;
   }
 }
+template <bool PER>
+__global__ __launch_bounds__(512, PER ? 4 : 8) void k_grad_tile(DevGeo G, TileGeo TG, StageArgs A,
+                                                                int t0) {
+  grad_tile_body<PER>(G, TG, A, xcd_tile(blockIdx.x, gridDim.x) + t0, blockIdx.y);
+}
+// The same over the tiles of several contexts (one block per tile and sample
+// of every level): PER selects the periodic tiles [0, np) of each level, or
+// the others.
+template <bool PER>
+__global__ __launch_bounds__(512, PER ? 4 : 8) void k_grad_tile_packed(const PackLevel* __restrict__ pk,
+                                                                       int nl) {
+  const int b = blockIdx.x;
+  int l = 0;
+  while (l + 1 < nl && b >= (PER ? pk[l + 1].base_gper : pk[l + 1].base_grest)) ++l;
+  const PackLevel& L = pk[l];
+  const int np = L.TG.n_periodic_tiles, n = PER ? np : L.TG.ntiles - np;
+  const int local = b - (PER ? L.base_gper : L.base_grest);
+  grad_tile_body<PER>(L.G, L.TG, L.A, local % n + (PER ? 0 : np), local / n);
+}
 
 // Gradients of the vertices k_grad_tile<false> leaves out (a cell with a
 // boundary correction, outside periodic tiles): the same cell order and
@@ -287,9 +321,9 @@ __global__ __launch_bounds__(512, PER ? 4 : 8) void k_grad_tile(DevGeo G, TileGe
 // the lane group's first lane adds the cell gradients in cell order (the
 // order of the tile kernel), so the long per-cell load chains run side by side.
 constexpr int GB_LANES = 8;
-__global__ __launch_bounds__(256) void k_grad_bnd(DevGeo G, TileGeo TG, StageArgs A, int nbv) {
-  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / GB_LANES, j = threadIdx.x % GB_LANES;
-  const int s = blockIdx.y;
+__device__ __forceinline__ void grad_bnd_body(const DevGeo& G, const TileGeo& TG,
+                                              const StageArgs& A, int nbv, int bx, int s) {
+  const int i = (bx * blockDim.x + threadIdx.x) / GB_LANES, j = threadIdx.x % GB_LANES;
   if (A.ctl[s].done) return;
   const bool live = i < nbv;
   const int v = live ? TG.gbv[i] : 0;
@@ -330,6 +364,19 @@ __global__ __launch_bounds__(256) void k_grad_bnd(DevGeo G, TileGeo TG, StageArg
     for (int q = 0; q < 8; ++q) dEv[q] = acc[q] * fact;
   }
 }
+__global__ __launch_bounds__(256) void k_grad_bnd(DevGeo G, TileGeo TG, StageArgs A, int nbv) {
+  grad_bnd_body(G, TG, A, nbv, blockIdx.x, blockIdx.y);
+}
+constexpr int GB_BLOCK = 256;
+__host__ __device__ inline int grad_bnd_blocks(int nbv) { return (nbv * GB_LANES + GB_BLOCK - 1) / GB_BLOCK; }
+__global__ __launch_bounds__(256) void k_grad_bnd_packed(const PackLevel* __restrict__ pk, int nl) {
+  const int b = blockIdx.x;
+  int l = 0;
+  while (l + 1 < nl && b >= pk[l + 1].base_gbnd) ++l;
+  const PackLevel& L = pk[l];
+  const int nbx = grad_bnd_blocks(L.TG.n_gbv), local = b - L.base_gbnd;
+  grad_bnd_body(L.G, L.TG, L.A, L.TG.n_gbv, local % nbx, local / nbx);
+}
 
 // ---------------------------------------------------------------------------
 // flux + gather + update kernel
@@ -353,22 +400,25 @@ struct SlotLayout {
 // ceil(max_faces / FPT) threads per tile.
 constexpr int FPT = 2;
 
-// Waves per SIMD the flux kernel is compiled for: 3 keeps every value in
-// registers (at most 168 VGPRs); 4 caps it at 128 VGPRs with a few spills.
+// Waves per SIMD the Euler flux kernels are compiled for: 4 caps them at 128
+// VGPRs (the last stage's instance spills a few values); the Navier-Stokes
+// instances keep 3 (at most 168 VGPRs), which holds their viscous terms in
+// registers.
 #ifndef TSM_FU_WAVES
-#define TSM_FU_WAVES 3
+#define TSM_FU_WAVES 4
 #endif
 
 // NS: 0 Euler; 1 Navier-Stokes; 2 Navier-Stokes on a mesh with wall cells
 // (their terms need registers the other meshes should not pay for)
 template <int FLUX, int REC, int LAST, int NS>
-__global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo TG, StageArgs A, StepParams P,
-                                                 const double* Wn, double* dtpart, int r) {
+__device__ __forceinline__ void fu_tile_body(const DevGeo& G, const TileGeo& TG,
+                                             const StageArgs& A, const StepParams& P,
+                                             const double* Wn, double* dtpart, int r, int t,
+                                             int s, int nbx) {
   using L = SlotLayout<FLUX, REC, (NS != 0)>;
   extern __shared__ double lds[];
   __shared__ double wred[8 * 5];
   __shared__ int arrive;
-  const int t = blockIdx.x, s = blockIdx.y;
   if (A.ctl[s].done) return;
   const int MS = TG.max_slots, MF = TG.max_faces, TB = TG.tb;
   // LDS: slot rows Q (NQ x MS).  Once the face fluxes are in registers the
@@ -703,8 +753,33 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_fu_tile(DevGeo G, TileGeo
       dtv = (P.time_step > 0.0) ? P.time_step : fdiv(P.cfl * G.dcarea[v], lam);
       if (A.dtl) A.dtl[off + v] = dtv;  // steady mode: read by this vertex only, next step
     }
-    block_partials(part, dtv, wred, &arrive, A.partial, dtpart, (size_t)s * gridDim.x + t);
+    block_partials(part, dtv, wred, &arrive, A.partial, dtpart, (size_t)s * nbx + t);
   }
+}
+#ifndef TSM_FU_WAVES_LAST
+#define TSM_FU_WAVES_LAST TSM_FU_WAVES
+#endif
+template <int FLUX, int REC, int LAST, int NS>
+__global__ __launch_bounds__(512, NS ? 3 : (LAST ? TSM_FU_WAVES_LAST : TSM_FU_WAVES)) void k_fu_tile(DevGeo G, TileGeo TG,
+                                                                        StageArgs A, StepParams P,
+                                                                        const double* Wn,
+                                                                        double* dtpart, int r) {
+  fu_tile_body<FLUX, REC, LAST, NS>(G, TG, A, P, Wn, dtpart, r, xcd_tile(blockIdx.x, gridDim.x),
+                                    blockIdx.y, gridDim.x);
+}
+// The same over the tiles and samples of several contexts (the levels of a
+// multilevel run): block b belongs to the level whose range [base_fu, ...)
+// holds it, tile (b - base) mod ntiles, sample (b - base) / ntiles.
+template <int FLUX, int REC, int LAST, int NS>
+__global__ __launch_bounds__(512, NS ? 3 : (LAST ? TSM_FU_WAVES_LAST : TSM_FU_WAVES)) void k_fu_tile_packed(
+    const PackLevel* __restrict__ pk, int nl, int r) {
+  const int b = blockIdx.x;
+  int l = 0;
+  while (l + 1 < nl && b >= pk[l + 1].base_fu) ++l;
+  const PackLevel& L = pk[l];
+  const int nt = L.TG.ntiles, local = b - L.base_fu;
+  fu_tile_body<FLUX, REC, LAST, NS>(L.G, L.TG, L.A, L.P, L.Wn, L.dtpart, r, local % nt, local / nt,
+                                    nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -734,7 +809,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
   extern __shared__ double lds[];
   __shared__ double wred[8 * 5];
   __shared__ int arrive;
-  const int t = blockIdx.x, s = blockIdx.y;
+  const int t = xcd_tile(blockIdx.x, gridDim.x), s = blockIdx.y;
   if (A.ctl[s].done) return;
   const int MS1 = TG.max_slots1, MS2 = TG.max_slots, MF = TG.max_faces;
   double* Q = lds;                          // R1 x MS1
@@ -956,11 +1031,10 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
 
 // Residual norm, time advance, loop test and the next step's global dt
 // (src/fv.cc:1591-1618, :1801-1879, :2273-2316).
-__global__ __launch_bounds__(1024) void k_step_final_tiled(StageArgs A, StepParams P, int nblk,
-                                                          int nv, const double* dtpart,
-                                                          int log_slot) {
+__device__ __forceinline__ void step_final_body(const StageArgs& A, const StepParams& P, int nblk,
+                                                int nv, const double* dtpart, int log_slot,
+                                                int s) {
   __shared__ double red[64];
-  const int s = blockIdx.x;
   Ctl& c = A.ctl[s];
   if (c.done) return;
   double part[4] = {0, 0, 0, 0};
@@ -1019,6 +1093,18 @@ __global__ __launch_bounds__(1024) void k_step_final_tiled(StageArgs A, StepPara
       c.dt = dt;
     }
   }
+}
+__global__ __launch_bounds__(1024) void k_step_final_tiled(StageArgs A, StepParams P, int nblk,
+                                                          int nv, const double* dtpart,
+                                                          int log_slot) {
+  step_final_body(A, P, nblk, nv, dtpart, log_slot, blockIdx.x);
+}
+__global__ __launch_bounds__(1024) void k_step_final_packed(const PackLevel* __restrict__ pk, int nl) {
+  const int b = blockIdx.x;
+  int l = 0;
+  while (l + 1 < nl && b >= pk[l + 1].base_final) ++l;
+  const PackLevel& L = pk[l];
+  step_final_body(L.A, L.P, L.TG.ntiles, L.G.nv, L.dtpart, -1, b - L.base_final);
 }
 
 // ---------------------------------------------------------------------------
@@ -1171,6 +1257,91 @@ void launch_step_final_tiled(const DevGeo& G, const StageArgs& A, const StepPara
                              int nblk, const double* dtpart, int log_slot, hipStream_t st) {
   hipLaunchKernelGGL(k_step_final_tiled, dim3(S), dim3(1024), 0, st, A, P, nblk, G.nv, dtpart,
                      log_slot);
+}
+
+// ---------------------------------------------------------------------------
+// packed launches: every kernel of a step runs once over the tiles and
+// samples of all levels (contexts with the same scheme), each block doing
+// exactly what it does in its own context's launch
+// ---------------------------------------------------------------------------
+void pack_plan(PackLevel* lv, int nl, int flux, int rec, PackPlan& p) {
+  p = PackPlan{};
+  p.nl = nl;
+  for (int l = 0; l < nl; ++l) {
+    PackLevel& L = lv[l];
+    const TileGeo& T = L.TG;
+    const int np = T.n_periodic_tiles;
+    L.base_fu = p.fu_blocks;         p.fu_blocks += T.ntiles * L.S;
+    L.base_gper = p.gper_blocks;     p.gper_blocks += np * L.S;
+    L.base_grest = p.grest_blocks;   p.grest_blocks += (T.ntiles - np) * L.S;
+    L.base_gbnd = p.gbnd_blocks;     p.gbnd_blocks += (T.n_gbv > 0 ? grad_bnd_blocks(T.n_gbv) : 0) * L.S;
+    L.base_final = p.final_blocks;   p.final_blocks += L.S;
+    p.fu_block = std::max(p.fu_block, tile_fu_block(T));
+    p.grad_block = std::max(p.grad_block, tile_grad_block(T));
+    p.fu_lds = std::max(p.fu_lds, tile_fu_lds(T, flux, rec));
+    p.grad_lds = std::max(p.grad_lds, tile_grad_lds(T));
+    p.grad_pass = tile_needs_gradient_pass(T, flux, rec);
+  }
+}
+
+void launch_packed_gradient(const PackLevel* d, const PackPlan& p, hipStream_t st) {
+  if (!p.grad_pass) return;
+  if (p.gbnd_blocks > 0)
+    hipLaunchKernelGGL(k_grad_bnd_packed, dim3(p.gbnd_blocks), dim3(GB_BLOCK), 0, st, d, p.nl);
+  if (p.gper_blocks > 0) {
+    (void)hipFuncSetAttribute((const void*)k_grad_tile_packed<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.grad_lds);
+    hipLaunchKernelGGL(k_grad_tile_packed<true>, dim3(p.gper_blocks), dim3(p.grad_block), p.grad_lds,
+                       st, d, p.nl);
+  }
+  if (p.grest_blocks > 0) {
+    (void)hipFuncSetAttribute((const void*)k_grad_tile_packed<false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.grad_lds);
+    hipLaunchKernelGGL(k_grad_tile_packed<false>, dim3(p.grest_blocks), dim3(p.grad_block),
+                       p.grad_lds, st, d, p.nl);
+  }
+}
+
+namespace {
+template <int FLUX, int REC>
+void packed_fu(const PackLevel* d, const PackPlan& p, int r, bool last, hipStream_t st) {
+  auto go = [&](auto kern) {
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)p.fu_lds);
+    hipLaunchKernelGGL(kern, dim3(p.fu_blocks), dim3(p.fu_block), p.fu_lds, st, d, p.nl, r);
+  };
+  if (last) go(k_fu_tile_packed<FLUX, REC, 1, 0>);
+  else go(k_fu_tile_packed<FLUX, REC, 0, 0>);
+}
+template <int FLUX>
+void packed_fu_rec(int rec, const PackLevel* d, const PackPlan& p, int r, bool last,
+                   hipStream_t st) {
+  switch (rec) {
+    case R_FIRST: packed_fu<FLUX, R_FIRST>(d, p, r, last, st); break;
+    case R_SECOND: packed_fu<FLUX, R_SECOND>(d, p, r, last, st); break;
+    case R_MUSCL: packed_fu<FLUX, R_MUSCL>(d, p, r, last, st); break;
+    case R_ALBADA: packed_fu<FLUX, R_ALBADA>(d, p, r, last, st); break;
+    case R_MINMOD: packed_fu<FLUX, R_MINMOD>(d, p, r, last, st); break;
+    case R_TVB: packed_fu<FLUX, R_TVB>(d, p, r, last, st); break;
+    default: packed_fu<FLUX, R_ENO2>(d, p, r, last, st); break;
+  }
+}
+}  // namespace
+
+void launch_packed_stage(int flux, int rec, const PackLevel* d, const PackPlan& p, int r,
+                         bool last, hipStream_t st) {
+  switch (flux) {
+    case F_ROE: packed_fu_rec<F_ROE>(rec, d, p, r, last, st); break;
+    case F_RUS: packed_fu_rec<F_RUS>(rec, d, p, r, last, st); break;
+    case F_KEPEC: packed_fu<F_KEPEC, R_FIRST>(d, p, r, last, st); break;
+    case F_ROEEC: packed_fu<F_ROEEC, R_FIRST>(d, p, r, last, st); break;
+    case F_KEP: packed_fu<F_KEP, R_FIRST>(d, p, r, last, st); break;
+    default: packed_fu<F_AVG, R_FIRST>(d, p, r, last, st); break;
+  }
+}
+
+void launch_packed_final(const PackLevel* d, const PackPlan& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_final_packed, dim3(p.final_blocks), dim3(1024), 0, st, d, p.nl);
 }
 
 }  // namespace tsm

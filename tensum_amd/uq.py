@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import Params, Solver, jittered_mesh, run_group, sobol
+from . import Params, Solver, TsmError, jittered_mesh, run_group, run_packed, sobol
 
 # ---------------------------------------------------------------------------
 # uncertain inputs
@@ -153,15 +153,27 @@ def qoi(problem: Problem, solver: Solver, max_density=1e6):
 
 
 def advance(solvers, chunk=64, max_chunks=100000):
-    """Runs every solver until all its samples reached final_time."""
+    """Runs every solver until all its samples reached final_time: packed
+    launches spanning every solver where their scheme allows it (Euler,
+    SSP-RK3, tiled kernels, one flux and reconstruction), the solvers' own
+    launches interleaved on their streams otherwise."""
+    step = run_packed
     for _ in range(max_chunks):
-        if not np.any(run_group(solvers, chunk)):
+        try:
+            done = step(solvers, chunk)
+        except TsmError as e:
+            if step is run_group or "packed" not in str(e) and "share" not in str(e):
+                raise
+            step = run_group
+            continue
+        if not np.any(done):
             return
     raise RuntimeError("samples did not reach final_time")
 
 
-def plain_mc(problem: Problem, n_mesh, n_samples, seed=0, skip=0, batch=None):
-    """Plain Monte Carlo on one mesh: QoI per sample, in sample order."""
+def plain_mc(problem: Problem, n_mesh, n_samples, seed=0, skip=0, batch=None, log=None):
+    """Plain Monte Carlo on one mesh: QoI per sample, in sample order, solved
+    `batch` samples at a time (all at once by default)."""
     mesh = problem.mesh(n_mesh)
     inputs = sample_inputs(n_samples, problem.bounds, seed=seed, skip=skip)
     batch = batch or n_samples
@@ -172,6 +184,8 @@ def plain_mc(problem: Problem, n_mesh, n_samples, seed=0, skip=0, batch=None):
         advance([s])
         out.append(qoi(problem, s))
         s.close()
+        if log:
+            log(f"{min(b0 + batch, n_samples)}/{n_samples} samples at {n_mesh}^2")
     return inputs, np.vstack(out)
 
 
@@ -251,21 +265,24 @@ def _sync_time(t0):
 
 
 def cv_experiment(problem: Problem, n_fine, n_coarse, n_train, n_pilot, n_pair, n_surr, n_ref,
-                  epochs=300, batch=64, seed=0, log=None):
+                  epochs=300, batch=64, seed=0, log=None, fine_batch=None, coarse_batch=None):
     """Surrogate control variate against plain MC on the same fine mesh.
 
     Reference: plain MC with n_ref fine samples (its own cost is not charged).
     Control variate: an MLP trained on n_train coarse-mesh solver samples, its
-    coefficients fitted on n_pilot fine samples, then n_pair fine samples and
-    n_surr surrogate evaluations.  Returns the realised mean and variance
-    errors of both estimators against the reference (RMS over the QoIs with
-    non-zero reference variance, each scaled by the reference's standard
-    deviation), their GPU-seconds, and the GPU-seconds plain MC would need to
-    reach the control variate's standard error."""
+    coefficients fitted on n_pilot fine samples (n_pilot = 0: on the training
+    samples, coarse solver against surrogate; either way independent of the
+    paired samples, so the estimate stays unbiased), then n_pair fine samples
+    and n_surr surrogate evaluations.  fine_batch / coarse_batch: samples per
+    solver context on each mesh (all at once by default).  Returns the
+    realised mean and variance errors of both estimators against the reference
+    (RMS over the QoIs with non-zero reference variance, each scaled by the
+    reference's standard deviation), their GPU-seconds, and the GPU-seconds
+    plain MC would need to reach the control variate's standard error."""
     from .surrogate import Surrogate, _alpha, control_variate
     say = log or (lambda *_: None)
     t0 = time.perf_counter()
-    _, Q_ref = plain_mc(problem, n_fine, n_ref, seed=seed + 1000)
+    _, Q_ref = plain_mc(problem, n_fine, n_ref, seed=seed + 1000, batch=fine_batch, log=say)
     t_ref = _sync_time(t0)
     m_ref, v_ref = moments(Q_ref)
     keep = v_ref > 1e-24 * np.maximum(1.0, m_ref ** 2)
@@ -274,27 +291,32 @@ def cv_experiment(problem: Problem, n_fine, n_coarse, n_train, n_pilot, n_pair, 
     say(f"reference: {n_ref} samples at {n_fine}^2 in {t_ref:.1f} s")
 
     t0 = time.perf_counter()
-    u_tr, Q_tr = plain_mc(problem, n_coarse, n_train, seed=seed + 1)
+    u_tr, Q_tr = plain_mc(problem, n_coarse, n_train, seed=seed + 1, batch=coarse_batch, log=say)
     t_train_sims = _sync_time(t0)
     t0 = time.perf_counter()
     sur = Surrogate(len(problem.bounds), problem.n_qoi, max_batch=max(batch, 4096), seed=seed + 2)
     losses = sur.fit(_unit(problem, u_tr), Q_tr, epochs=epochs, batch=batch)
     t_fit = _sync_time(t0)
+    say(f"surrogate trained on {n_train} samples at {n_coarse}^2 in {t_fit:.1f} s, "
+        f"final loss {float(losses[-1]) if len(losses) else float('nan'):.3g}")
     t0 = time.perf_counter()
-    u_a, Q_a = plain_mc(problem, n_fine, n_pilot, seed=seed + 3)
-    u_p, Q_p = plain_mc(problem, n_fine, n_pair, seed=seed + 4)
+    if n_pilot:
+        u_a, Q_a = plain_mc(problem, n_fine, n_pilot, seed=seed + 3, batch=fine_batch, log=say)
+    u_p, Q_p = plain_mc(problem, n_fine, n_pair, seed=seed + 4, batch=fine_batch, log=say)
     t_fine = _sync_time(t0)
     t0 = time.perf_counter()
-    G_a = sur(_unit(problem, u_a))
+    G_a = sur(_unit(problem, u_a if n_pilot else u_tr))
     G_p = sur(_unit(problem, u_p))
     G_m = sur(_unit(problem, sample_inputs(n_surr, problem.bounds, seed=seed + 5)))
     t_eval = _sync_time(t0)
-    est = control_variate(Q_p, G_p, G_m, alpha=_alpha(Q_a, G_a), alpha2=_alpha(Q_a ** 2, G_a ** 2))
+    Q_fit = Q_a if n_pilot else Q_tr
+    est = control_variate(Q_p, G_p, G_m, alpha=_alpha(Q_fit, G_a),
+                          alpha2=_alpha(Q_fit ** 2, G_a ** 2))
     cv_seconds = t_train_sims + t_fit + t_fine + t_eval
 
     # plain MC with the same fine-sample budget as the control variate
     n_plain = n_pilot + n_pair
-    Q_plain = np.vstack([Q_a, Q_p])
+    Q_plain = np.vstack([Q_a, Q_p]) if n_pilot else Q_p
     m_pl, v_pl = moments(Q_plain)
 
     def err(m, v):
@@ -312,6 +334,8 @@ def cv_experiment(problem: Problem, n_fine, n_coarse, n_train, n_pilot, n_pair, 
                             "paired": n_pair, "surrogate_evals": n_surr,
                             "final_loss": float(losses[-1]) if len(losses) else None,
                             "median_variance_reduction": float(np.median(est.variance_reduction[keep])),
+                            "variance_reduction_integrals": [float(v) for v in est.variance_reduction[-2:]],
+                            "alpha_fit": "pilot fine samples" if n_pilot else "training samples",
                             "seconds": {"train_sims": t_train_sims, "fit": t_fit, "fine": t_fine,
                                         "surrogate": t_eval}},
         "plain_mc": {"mean_error": e_pl[0], "var_error": e_pl[1], "samples": n_plain,
