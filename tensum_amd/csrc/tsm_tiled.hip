@@ -242,6 +242,18 @@ __device__ __forceinline__ void grad_tile_body(const DevGeo& G, const TileGeo& T
   const int h0 = TG.hbeg[t], ns = nown + TG.hbeg[t + 1] - h0;
   const size_t off = (size_t)s * G.nv;
   const Gas gas{A.gamma[s], A.gasR[s]};
+  const int tid = threadIdx.x;
+  const bool own = tid < nown;
+  const int v = b0 + tid;
+  // the owned vertex's cell range and scale, requested before the slot phase
+  // so that their latency overlaps it
+  int kc0 = 0, kc1 = 0;
+  double fact = 0.0;
+  if (own) {
+    kc0 = TG.cptr[v];
+    kc1 = TG.cptr[v + 1];
+    fact = G.gfact[v];
+  }
   for (int i = threadIdx.x; i < ns; i += blockDim.x) {
     const int gid = (i < nown) ? b0 + i : TG.halo[h0 + i - nown];
     const Ent e = prim2ent(gas, ld_prim(A.W_in, off + gid));
@@ -249,13 +261,10 @@ __device__ __forceinline__ void grad_tile_body(const DevGeo& G, const TileGeo& T
     SX[i] = G.x[gid]; SX[MS + i] = G.y[gid];
   }
   __syncthreads();
-  const int tid = threadIdx.x;
-  const bool own = tid < nown;
-  const int v = b0 + tid;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool skip = false;
   if (own) {
-    for (int k = TG.cptr[v]; k < TG.cptr[v + 1]; ++k) {
+    for (int k = kc0; k < kc1; ++k) {
       const unsigned e = TG.cent[k];
       if (!PER && (e >> 30)) { skip = true; break; }
       const int a = e & 1023, b = (e >> 10) & 1023, d = (e >> 20) & 1023;
@@ -289,7 +298,6 @@ __device__ __forceinline__ void grad_tile_body(const DevGeo& G, const TileGeo& T
     }
   }
   if (own && !skip) {
-    const double fact = G.gfact[v];
     double* dEv = A.dE + 8 * (off + v);
 #pragma unroll
     for (int q = 0; q < 8; ++q) dEv[q] = acc[q] * fact;
@@ -437,6 +445,29 @@ __device__ __forceinline__ void fu_tile_body(const DevGeo& G, const TileGeo& TG,
   const Gas gas{A.gamma[s], A.gasR[s]};
 
   if (LAST && threadIdx.x == 0) arrive = 0;  // read after the phase barriers
+  // this thread's faces and its owned vertex's gather range, loaded before
+  // the slot phase so that their latency overlaps it
+  // (measured on the Rusanov TeCNO flux with the homogeneous limiters; the
+  // other instances load them where they are used, as those run short of
+  // registers or lose occupancy with them)
+  constexpr bool PREF = FLUX == F_RUS && !NS && REC != R_ALBADA && REC != R_TVB;
+  int gk0 = 0, gne = 0, gnf = 0;
+  if (PREF && threadIdx.x < nown) {
+    gk0 = TG.gptr[b0 + threadIdx.x];
+    gne = TG.gptr[b0 + threadIdx.x + 1] - gk0;
+  }
+  unsigned flr[FPT];
+  double fnx[FPT], fny[FPT];
+#pragma unroll
+  for (int q = 0; q < FPT; ++q) {
+    const int j = threadIdx.x + q * blockDim.x;
+    flr[q] = 0; fnx[q] = 1.0; fny[q] = 0.0;
+    if (PREF && j < nfl) {
+      flr[q] = TG.f_lr[f0 + j];
+      fnx[q] = TG.f_n[2 * (size_t)(f0 + j)];
+      fny[q] = TG.f_n[2 * (size_t)(f0 + j) + 1];
+    }
+  }
   // ---- phase 1: slot quantities ------------------------------------------
   for (int i = threadIdx.x; i < ns; i += blockDim.x) {
     const int gid = (i < nown) ? b0 + i : TG.halo[h0 + i - nown];
@@ -469,9 +500,10 @@ __device__ __forceinline__ void fu_tile_body(const DevGeo& G, const TileGeo& TG,
   for (int q = 0; q < FPT; ++q) {
     const int j = threadIdx.x + q * blockDim.x;
     if (j >= nfl) continue;
-    const unsigned lr = TG.f_lr[f0 + j];
+    const unsigned lr = PREF ? flr[q] : TG.f_lr[f0 + j];
     const int l = lr & 0xffff, rr = lr >> 16;
-    const double nx = TG.f_n[2 * (size_t)(f0 + j)], ny = TG.f_n[2 * (size_t)(f0 + j) + 1];
+    const double nx = PREF ? fnx[q] : TG.f_n[2 * (size_t)(f0 + j)];
+    const double ny = PREF ? fny[q] : TG.f_n[2 * (size_t)(f0 + j) + 1];
     const double area = fsqrt(nx * nx + ny * ny);
     const double ia = fdiv(1.0, area);
     Side Ls, Rs;
@@ -529,10 +561,20 @@ __device__ __forceinline__ void fu_tile_body(const DevGeo& G, const TileGeo& TG,
   // last stage's time-step loop); boundary faces and longer lists follow from
   // memory, so every sum keeps the reference order.
   unsigned ge[GE];
-  int gk0 = 0, gne = 0, gnf = 0;
+  // the update's operands, requested before the gather so that their latency
+  // overlaps it: W^n (U_old of every stage but the first SSP one) and the area
+  // (not in the last stage, whose time-step loop needs the registers)
+  double4 wn4 = make_double4(1.0, 0.0, 0.0, 1.0);
+  double area_v = 1.0;
   if (own) {
-    gk0 = TG.gptr[v];
-    gne = TG.gptr[v + 1] - gk0;
+    if (!PREF) {
+      gk0 = TG.gptr[v];
+      gne = TG.gptr[v + 1] - gk0;
+    }
+    if (!LAST && PREF) {
+      if (r != 0 || A.scheme == 1) wn4 = reinterpret_cast<const double4*>(Wn)[off + v];
+      area_v = G.dcarea[v];
+    }
 #pragma unroll
     for (int u = 0; u < GE; ++u) ge[u] = (u < gne) ? (unsigned)TG.gent[gk0 + u] : 0x4000u;
 #pragma unroll
@@ -649,7 +691,7 @@ __device__ __forceinline__ void fu_tile_body(const DevGeo& G, const TileGeo& TG,
   double lam = 0.0;
   Prim w1{1, 0, 0, 1};
   if (own) {
-    const double area = G.dcarea[v];
+    const double area = (LAST || !PREF) ? G.dcarea[v] : area_v;
     if (LAST) {
       const double wt = G.weight[v];
 #pragma unroll
@@ -673,7 +715,8 @@ __device__ __forceinline__ void fu_tile_body(const DevGeo& G, const TileGeo& TG,
         q[k] = (r < 3) ? acc[k] : r2;
       }
       const double c = (r < 3) ? -factor * rks[r < 3 ? r + 1 : 3] : -factor / 6.0;
-      const Cons Uo = prim2con(gas, ld_prim(Wn, off + v));
+      const Cons Uo = prim2con(gas, (PREF && !LAST) ? Prim{wn4.x, wn4.y, wn4.z, wn4.w}
+                                                    : ld_prim(Wn, off + v));
       U.m = Uo.m * 1.0 + q[0] * c; U.mx = Uo.mx * 1.0 + q[1] * c;
       U.my = Uo.my * 1.0 + q[2] * c; U.e = Uo.e * 1.0 + q[3] * c;
     } else {  // SSP-RK3, or Heun's two-stage SSP form (src/fv.cc:1640-1651, :1675-1686)
@@ -688,7 +731,8 @@ __device__ __forceinline__ void fu_tile_body(const DevGeo& G, const TileGeo& TG,
         U.m += 0.0 + acc[0] * c; U.mx += 0.0 + acc[1] * c;
         U.my += 0.0 + acc[2] * c; U.e += 0.0 + acc[3] * c;
       } else {
-        const Cons Uo = prim2con(gas, ld_prim(Wn, off + v));
+        const Cons Uo = prim2con(gas, (PREF && !LAST) ? Prim{wn4.x, wn4.y, wn4.z, wn4.w}
+                                                      : ld_prim(Wn, off + v));
         U.m += Uo.m * a + acc[0] * c; U.mx += Uo.mx * a + acc[1] * c;
         U.my += Uo.my * a + acc[2] * c; U.e += Uo.e * a + acc[3] * c;
       }

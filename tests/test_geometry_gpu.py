@@ -34,6 +34,8 @@ MESHES = {
     "shuffled": lambda: _shuffled(T.jittered_mesh(41, seed=3, bc={1: ("slip",), 2: ("outlet",),
                                                              3: ("slip",), 4: ("outlet",)})),
     "unjittered": lambda: T.jittered_mesh(17, jitter=0.0),
+    # the benchmark's mesh: 1,050,625 vertices, 2,097,152 cells
+    "riemann_1025": lambda: riemann_mesh(1025),
 }
 
 
