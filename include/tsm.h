@@ -129,6 +129,22 @@ int tsm_set_sample_material(tsm_ctx* ctx, int sample, double gamma, double gas_c
    with T > 0 and p > 0 where the type reads them. */
 int tsm_set_bc_state(tsm_ctx* ctx, int sample, const double* state);
 
+/* Boundary values that depend on time (replaces the time argument of
+   BoundaryCondition::apply, src/bc.h:411-562, which the reference sets to
+   int_step_time = elapsed_time + r*dt_global before stage r of Heun's scheme,
+   src/fv.cc:2296-2298; the other schemes never set it, so these values are
+   refused there).  bface_group[b] (mesh order) names bface b's group, -1 for
+   none; expr[4g+q] is the expression of field q (T, u, v, p) of group g in the
+   parameter file's syntax of x, y and t, NULL or "" to keep the value
+   tsm_set_bc_state or the boundary table gave; the constants name the
+   parameter file's user constants (pi is built in; pert() is not available).
+   Every sample's values are evaluated on the device at every end point before
+   each stage at that sample's stage time, and once now at its current time.
+   n_groups = 0 removes them (the values they wrote last stay in force). */
+int tsm_set_bc_time_expr(tsm_ctx* ctx, const int* bface_group, int n_groups,
+                         const char* const* expr, int n_const,
+                         const char* const* const_name, const double* const_value);
+
 /* Initial state at t = 0 (replaces the IC / restart branch of
    FiniteVolume::initialize_sample_run, src/fv.cc:320-402). prim: 4*n_vertex. */
 int tsm_set_state(tsm_ctx* ctx, int sample, const double* prim);

@@ -473,6 +473,12 @@ struct orc_solver {
   // end point's coordinates with the sample's random numbers, src/bc.h:411-562);
   // empty: every bface takes its tag's constant state
   std::vector<Prim> bst;
+  // boundary values of time (orc_set_bc_time_fn): callback, input bface end
+  // points and the values in the input layout
+  orc_bc_time_fn bt_fn = nullptr;
+  void* bt_user = nullptr;
+  std::vector<int> bt_bv;
+  std::vector<double> bt_in;
   std::vector<int> face_bface;  // the bface of each boundary edge's face, else -1
   Prim bstate_at(int bi, int vertex) const {
     if (bst.empty()) return bc.at(bface[bi].type).state;
@@ -1242,6 +1248,10 @@ long orc_run(orc_solver* s, long nsteps, double* log) {
     if (P.time_scheme == 1)
       for (auto& r : s->R2) r = Flx{0, 0, 0, 0};
     for (unsigned r = 0; r < n_rks; ++r) {
+      if (s->bt_fn && P.time_scheme == 2) {  // int_step_time of stage r (src/fv.cc:2296-2298)
+        s->bt_fn(s->bt_user, s->elapsed + r * s->dt_global, s->bt_in.data());
+        if (orc_set_bc_state(s, s->bt_bv.data(), s->bt_in.data())) return -1;
+      }
       if (!compute_residual(s)) return -1;
       if (r == n_rks - 1) {
         if (P.strong_bc) strong_zero_noslip(s);
@@ -1413,6 +1423,29 @@ int orc_set_bc_state(orc_solver* s, const int* bface_v, const double* state) {
       const int k = bface_v[2 * b + 1] == f.v[e] ? 1 : 0;
       const double* q = state + 8 * (size_t)b + 4 * k;
       s->bst[2 * (size_t)bi + e] = Prim{q[0], q[1], q[2], q[3]};
+    }
+  }
+  return 0;
+}
+
+int orc_set_bc_time_fn(orc_solver* s, const int* bface_v, orc_bc_time_fn fn, void* user) {
+  s->bt_fn = fn;
+  s->bt_user = user;
+  if (!fn) return 0;
+  const int nb = (int)s->bface.size();
+  s->bt_bv.assign(bface_v, bface_v + 2 * (size_t)nb);
+  // start from the values in force: per input bface end point
+  s->bt_in.assign(8 * (size_t)nb, 0.0);
+  std::map<std::pair<int, int>, int> at;
+  for (int bi = 0; bi < nb; ++bi)
+    at[{std::min(s->bface[bi].v[0], s->bface[bi].v[1]), std::max(s->bface[bi].v[0], s->bface[bi].v[1])}] = bi;
+  for (int b = 0; b < nb; ++b) {
+    auto it = at.find({std::min(bface_v[2 * b], bface_v[2 * b + 1]), std::max(bface_v[2 * b], bface_v[2 * b + 1])});
+    if (it == at.end()) { g_err = "bface not in the input list"; return -1; }
+    for (int k = 0; k < 2; ++k) {
+      const Prim q = s->bstate_at(it->second, bface_v[2 * b + k]);
+      double* o = &s->bt_in[8 * (size_t)b + 4 * k];
+      o[0] = q.T; o[1] = q.u; o[2] = q.v; o[3] = q.p;
     }
   }
   return 0;

@@ -110,6 +110,13 @@ void orc_surface(orc_solver* s, const int* tags, int ntags, double* out);
    bface_v[2b+k]): state[8b+4k..] = (T,u,v,p), the boundary expressions
    evaluated there; replaces the tags' constant states.  0 or -1. */
 int orc_set_bc_state(orc_solver* s, const int* bface_v, const double* state);
+/* Boundary values of time: in Heun runs, before the residual of stage r, the
+   callback receives the stage time t = elapsed + r*dt_global
+   (src/fv.cc:2296-2298) and the current values in orc_set_bc_state's layout
+   (8 per input bface), which it updates in place; they then replace the
+   boundary states.  NULL removes it. */
+typedef void (*orc_bc_time_fn)(void* user, double t, double* state);
+int orc_set_bc_time_fn(orc_solver* s, const int* bface_v, orc_bc_time_fn fn, void* user);
 /* error norms vs a given exact primitive field: out[24] =
    (L1,L2,Linf) x (T,p,u,v,w,rho,mach,entropy) */
 void orc_error_norms(const orc_solver* s, const double* exact_prim, double* out);

@@ -195,6 +195,8 @@ def lib():
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.tsm_residual_norms.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_set_bc_state.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.tsm_set_bc_time_expr.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                           C.c_void_p, C.c_void_p]
         L.tsm_forces.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.tsm_keep_stage_state.argtypes = [C.c_void_p, C.c_int]
         L.tsm_get_clock.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
@@ -519,6 +521,25 @@ class Solver:
         if a.size != 8 * self.n_bface:
             raise TsmError(f"expected {8 * self.n_bface} boundary values, got {a.size}")
         _check(lib().tsm_set_bc_state(self._h, sample, _dptr(a)))
+
+    def set_bc_time_expr(self, groups, exprs, constants=None):
+        """Boundary values that depend on t (Heun runs): groups (n_bface,)
+        names each bface's group (-1: none); exprs[g] = (T, u, v, p)
+        expressions of x, y and t for group g, None keeping a field's value;
+        constants = {name: value}.  An empty exprs removes them."""
+        g = np.ascontiguousarray(groups, dtype=np.int32).reshape(-1)
+        if g.size != self.n_bface:
+            raise TsmError(f"expected {self.n_bface} bface groups, got {g.size}")
+        flat = [e for four in exprs for e in four]
+        if len(flat) != 4 * len(exprs):
+            raise TsmError("each group needs four expressions (T, u, v, p)")
+        ex = (C.c_char_p * max(1, len(flat)))(*[None if e is None else e.encode() for e in flat])
+        constants = dict(constants or {})
+        names = (C.c_char_p * max(1, len(constants)))(*[k.encode() for k in constants])
+        vals = np.array(list(constants.values()) or [0.0], dtype=np.float64)
+        _check(lib().tsm_set_bc_time_expr(self._h, g.ctypes.data_as(C.c_void_p), len(exprs),
+                                          C.cast(ex, C.c_void_p), len(constants),
+                                          C.cast(names, C.c_void_p), _dptr(vals)))
 
     def forces(self, tags, sample=0, force_index=0):
         """(pressure x, y, viscous x, y) force on the boundary faces with these tags."""

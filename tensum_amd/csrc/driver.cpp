@@ -373,10 +373,12 @@ int main(int argc, char** argv) {
   // Each boundary's expressions (src/bc.h:411-562) are evaluated at every
   // bface end point with each sample's random numbers once the samples are
   // known (below); the table holds the values at the tag's first end point
-  // for sample 0.  Expressions of t are refused.
+  // for sample 0.  Expressions of t are handed to the library as text, which
+  // re-evaluates them on the device at every stage time (Heun only).
   const char* bnames[4] = {"temperature", "xvelocity", "yvelocity", "pressure"};
   std::map<int, std::vector<tsm::Expr>> bexpr;   // tag -> 4 expressions (empty: field not given)
   std::map<int, std::vector<bool>> bhas;
+  std::map<int, std::vector<std::string>> btime; // tag -> texts of its fields of t ("" none)
   for (auto& kv : S.bc) {
     bc_tag.push_back(kv.first);
     bc_type.push_back(bcode.at(kv.second.type));
@@ -389,8 +391,8 @@ int main(int argc, char** argv) {
       if (it == kv.second.fn.end()) continue;
       if (!ex[k].parse(it->second, S.constants, err)) { std::cerr << "ERROR --> " << err << std::endl; return 1; }
       if (ex[k].uses_time()) {
-        std::cerr << "ERROR --> boundary " << kv.first << ": time-dependent boundary values are not supported" << std::endl;
-        return 1;
+        btime[kv.first].resize(4);
+        btime[kv.first][k] = it->second;
       }
       has[k] = true;
     }
@@ -480,6 +482,27 @@ int main(int argc, char** argv) {
         std::cerr << "ERROR --> boundary values of SAMPLE_" << S.sample_list[k] << ": " << tsm_last_error() << std::endl;
         return 1;
       }
+  if (!btime.empty()) {  // boundary values of t: one group per tag that has them
+    std::vector<int> group(nbf, -1);
+    std::vector<const char*> texts;
+    std::vector<int> gtag;
+    for (auto& kv : btime) {
+      gtag.push_back(kv.first);
+      for (int q = 0; q < 4; ++q) texts.push_back(kv.second[q].empty() ? nullptr : kv.second[q].c_str());
+    }
+    for (int b = 0; b < nbf; ++b)
+      group[b] = (int)(std::find(gtag.begin(), gtag.end(), P.btag[b]) - gtag.begin());
+    for (int& g : group)
+      if (g == (int)gtag.size()) g = -1;
+    std::vector<const char*> cname;
+    std::vector<double> cval;
+    for (auto& kv : S.constants) { cname.push_back(kv.first.c_str()); cval.push_back(kv.second); }
+    if (tsm_set_bc_time_expr(ctx, group.data(), (int)gtag.size(), texts.data(), (int)cname.size(),
+                             cname.data(), cval.data()) != TSM_OK) {
+      std::cerr << "ERROR --> boundary values of t: " << tsm_last_error() << std::endl;
+      return 1;
+    }
+  }
   const Mat M{S.p.gamma, S.p.gas_const};
   std::vector<double> dcarea(nv), mcarea(nv);
   tsm_geometry(ctx, nullptr, nullptr, dcarea.data(), mcarea.data());

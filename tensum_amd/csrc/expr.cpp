@@ -5,6 +5,7 @@
 // unary minus).  Comparisons return 1 or 0 with fparser's absolute tolerance
 // of 1e-12 (x < y holds when x < y - 1e-12); & | ! treat |v| >= 0.5 as true.
 #include "expr.h"
+#include "tprog.h"
 
 #include <cctype>
 #include <cmath>
@@ -270,6 +271,90 @@ double Expr::eval_node(const Node* n, const double* v) const {
 double Expr::eval(double x, double y, double t) const {
   const double v[3] = {x, y, t};
   return root_ ? eval_node(root_.get(), v) : NAN;
+}
+
+bool Expr::node_uses_t(const Node* n) {
+  if (n->kind == Node::VAR) return n->var == 2;
+  for (const auto& k : n->kids)
+    if (node_uses_t(k.get())) return true;
+  return false;
+}
+
+bool Expr::node_has_pert(const Node* n) {
+  if (n->kind == Node::FUN && n->fname == "pert") return true;
+  for (const auto& k : n->kids)
+    if (node_has_pert(k.get())) return true;
+  return false;
+}
+
+bool Expr::emit_tp(const Node* n, std::vector<int>& code, int& sp, int& depth, std::string& err) {
+  auto push = [&](int op) { code.push_back(op); if (++sp > depth) depth = sp; };
+  if (!node_uses_t(n)) {
+    push(TP_SLOT | (int)tslots_.size() << 8);
+    tslots_.push_back(n);
+    return true;
+  }
+  switch (n->kind) {
+    case Node::NUM: return false;  // (uses no t)
+    case Node::VAR: push(TP_T); return true;
+    case Node::UN:
+      if (!emit_tp(n->kids[0].get(), code, sp, depth, err)) return false;
+      code.push_back(n->op == '-' ? TP_NEG : TP_NOT);
+      return true;
+    case Node::BIN: {
+      if (!emit_tp(n->kids[0].get(), code, sp, depth, err) ||
+          !emit_tp(n->kids[1].get(), code, sp, depth, err))
+        return false;
+      static const char ops[] = "+-*/%^<>lg=n&|";
+      static const int codes[] = {TP_ADD, TP_SUB, TP_MUL, TP_DIV, TP_MOD, TP_POW, TP_LT,
+                                  TP_GT,  TP_LE,  TP_GE,  TP_EQ,  TP_NE,  TP_AND, TP_OR};
+      const char* at = std::strchr(ops, n->op);
+      if (!at || !*at) { err = "unknown operator"; return false; }
+      code.push_back(codes[at - ops]);
+      --sp;
+      return true;
+    }
+    case Node::FUN: {
+      static const char* f1[] = {"exp",  "exp2", "log",  "log10", "log2",  "sqrt", "cbrt",
+                                 "sin",  "cos",  "tan",  "asin",  "acos",  "atan", "sinh",
+                                 "cosh", "tanh", "abs",  "floor", "ceil",  "trunc", "int"};
+      static const char* f2[] = {"pow", "atan2", "min", "max", "hypot"};
+      const std::string& f = n->fname;
+      const size_t na = n->kids.size();
+      int op = -1, arity = 0;
+      if (f == "if") { op = TP_IF; arity = 3; }
+      for (int k = 0; k < (int)(sizeof(f1) / sizeof(f1[0])); ++k)
+        if (f == f1[k]) { op = TP_F1 | (TF_EXP + k) << 8; arity = 1; }
+      for (int k = 0; k < (int)(sizeof(f2) / sizeof(f2[0])); ++k)
+        if (f == f2[k]) { op = TP_F2 | (TF_POW + k) << 8; arity = 2; }
+      if (op < 0) { err = "function " + f + " is not available in a boundary value of t"; return false; }
+      if ((int)na != arity) { err = "wrong number of arguments to " + f; return false; }
+      for (const auto& k : n->kids)
+        if (!emit_tp(k.get(), code, sp, depth, err)) return false;
+      code.push_back(op);
+      sp -= arity - 1;
+      return true;
+    }
+  }
+  return false;
+}
+
+bool Expr::time_program(std::vector<int>& code, int& depth, std::string& err) {
+  code.clear();
+  tslots_.clear();
+  depth = 0;
+  if (!root_) { err = "no expression"; return false; }
+  if (node_has_pert(root_.get())) { err = "pert() is not available in a boundary value of t"; return false; }
+  int sp = 0;
+  if (!emit_tp(root_.get(), code, sp, depth, err)) return false;
+  code.push_back(TP_END);
+  if (depth > TP_MAX_DEPTH) { err = "expression nests too deeply for the boundary-time evaluator"; return false; }
+  return true;
+}
+
+double Expr::time_slot(int k, double x, double y) const {
+  const double v[3] = {x, y, 0.0};
+  return eval_node(tslots_[k], v);
 }
 
 }  // namespace tsm

@@ -27,13 +27,24 @@ class Expr {
   void set_pert(std::function<double(int, double, double, double)> f) { pert_ = std::move(f); }
   double eval(double x, double y, double t) const;
   bool uses_time() const { return uses_t_; }
+  // Compiles the expression into a boundary-time program (tprog.h): every
+  // largest subexpression without t becomes a slot whose value time_slot()
+  // gives at a point.  Fails on pert() (the program has no random numbers)
+  // and on a wrong argument count.
+  bool time_program(std::vector<int>& code, int& depth, std::string& err);
+  int n_time_slots() const { return (int)tslots_.size(); }
+  double time_slot(int k, double x, double y) const;
 
  private:
   struct Node;
   std::shared_ptr<Node> root_;
   std::function<double(int, double, double, double)> pert_;
   bool uses_t_ = false;
+  std::vector<const Node*> tslots_;
   double eval_node(const Node* n, const double* v) const;
+  bool emit_tp(const Node* n, std::vector<int>& code, int& sp, int& depth, std::string& err);
+  static bool node_uses_t(const Node* n);
+  static bool node_has_pert(const Node* n);
   friend struct ExprParser;
 };
 

@@ -16,6 +16,8 @@
 
 #include "tsm_device.h"
 #include "tsm_host.h"
+#include "expr.h"
+#include "tprog.h"
 
 namespace {
 thread_local std::string g_last_error;
@@ -70,6 +72,9 @@ struct tsm_ctx {
   // (tsm_run_packed with this context first)
   tsm::PackLevel* pack = nullptr;
   size_t pack_cap = 0;
+  // time-dependent boundary values (tsm_set_bc_time_expr), re-evaluated
+  // before every stage
+  tsm::BcTime bt{};
 
   ~tsm_ctx() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -486,6 +491,26 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
 
 extern "C" void tsm_destroy(tsm_ctx* c) { delete c; }
 
+namespace {
+// Gives every sample a block of boundary values of its own (they share one
+// until a sample's values differ).
+int own_bstate(tsm_ctx* c) {
+  if (c->G.bs_sample != 0 || c->S == 1) return TSM_OK;
+  const int nb = c->G.nb;
+  double* all = nullptr;
+  int rc = c->alloc(&all, (size_t)c->S * 8 * nb);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->st));
+  for (int k = 0; k < c->S; ++k)
+    HIP_TRY(hipMemcpy(all + (size_t)k * 8 * nb, c->G.bstate, 8 * (size_t)nb * sizeof(double),
+                      hipMemcpyDeviceToDevice));
+  c->G.bstate = all;
+  c->G.bs_sample = 8LL * nb;
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }  // captured the old array
+  return TSM_OK;
+}
+}  // namespace
+
 extern "C" int tsm_set_bc_state(tsm_ctx* c, int s, const double* state) {
   if (!c || !state || s < 0 || s >= c->S) return fail(TSM_E_ARG, "bad argument");
   const int nb = c->G.nb;
@@ -503,20 +528,87 @@ extern "C" int tsm_set_bc_state(tsm_ctx* c, int s, const double* state) {
         return fail(TSM_E_STATE, "non-positive boundary T or p at bface " + std::to_string(bi));
       for (int k = 0; k < 4; ++k) h[o + k] = q[k];
     }
-  if (c->G.bs_sample == 0 && c->S > 1) {  // samples stop sharing one block
-    double* all = nullptr;
-    int rc = c->alloc(&all, (size_t)c->S * 8 * nb);
+  if (c->S > 1) {  // samples stop sharing one block
+    int rc = own_bstate(c);
     if (rc) return rc;
-    for (int k = 0; k < c->S; ++k)
-      HIP_TRY(hipMemcpy(all + (size_t)k * 8 * nb, c->G.bstate, 8 * (size_t)nb * sizeof(double),
-                        hipMemcpyDeviceToDevice));
-    c->G.bstate = all;
-    c->G.bs_sample = 8LL * nb;
-    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }  // captured the old array
   }
   HIP_TRY(hipStreamSynchronize(c->st));
   HIP_TRY(hipMemcpy(const_cast<double*>(c->G.bstate) + (size_t)s * c->G.bs_sample, h.data(),
                     h.size() * sizeof(double), hipMemcpyHostToDevice));
+  return TSM_OK;
+}
+
+extern "C" int tsm_set_bc_time_expr(tsm_ctx* c, const int* bface_group, int n_groups,
+                                    const char* const* expr, int n_const,
+                                    const char* const* const_name, const double* const_value) {
+  if (!c || n_groups < 0 || (n_groups > 0 && (!bface_group || !expr)) || n_const < 0 ||
+      (n_const > 0 && (!const_name || !const_value)))
+    return fail(TSM_E_ARG, "bad argument");
+  if (n_groups == 0) {
+    c->bt = tsm::BcTime{};
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    return TSM_OK;
+  }
+  // the reference advances the boundaries' time only in Heun's scheme; the
+  // other schemes leave int_step_time unset (src/fv.cc:2294-2299)
+  if (c->P.time_scheme != 2 || !c->P.unsteady)
+    return fail(TSM_E_ARG, "time-dependent boundary values need an unsteady run with time_scheme heuns");
+  std::map<std::string, double> consts;
+  for (int k = 0; k < n_const; ++k) {
+    if (!const_name[k]) return fail(TSM_E_ARG, "null constant name");
+    consts[const_name[k]] = const_value[k];
+  }
+  // one program per group and field (T, u, v, p), none where no text is given
+  std::vector<tsm::Expr> ex(4 * (size_t)n_groups);
+  std::vector<int> code, pbeg(4 * (size_t)n_groups, -1);
+  for (int g = 0; g < 4 * n_groups; ++g) {
+    if (!expr[g] || !expr[g][0]) continue;
+    std::string err;
+    std::vector<int> pc;
+    int depth = 0;
+    if (!ex[g].parse(expr[g], consts, err) || !ex[g].time_program(pc, depth, err))
+      return fail(TSM_E_ARG, "boundary group " + std::to_string(g / 4) + ": " + err);
+    pbeg[g] = (int)code.size();
+    code.insert(code.end(), pc.begin(), pc.end());
+  }
+  // entries: every end point of every bface in a group, each given field
+  const int nb = c->G.nb;
+  std::vector<int> target, prog, slot;
+  std::vector<double> slots;
+  for (int bi = 0; bi < nb; ++bi) {
+    const int g = bface_group[bi];
+    if (g < 0) continue;
+    if (g >= n_groups) return fail(TSM_E_ARG, "bface " + std::to_string(bi) + ": group out of range");
+    for (int e = 0; e < 2; ++e) {
+      const long long o = c->bmap[2 * bi + e];
+      const int v = c->geo.bface[o / 8].v[(o % 8) / 4];
+      for (int q = 0; q < 4; ++q) {
+        const int k = 4 * g + q;
+        if (pbeg[k] < 0) continue;
+        target.push_back((int)o + q);
+        prog.push_back(pbeg[k]);
+        slot.push_back((int)slots.size());
+        for (int j = 0; j < ex[k].n_time_slots(); ++j)
+          slots.push_back(ex[k].time_slot(j, c->geo.x[v], c->geo.y[v]));
+      }
+    }
+  }
+  int rc = own_bstate(c);
+  if (rc) return rc;
+  tsm::BcTime B{};
+  B.n = (int)target.size();
+  if ((rc = c->upload(&B.target, target)) || (rc = c->upload(&B.prog, prog)) ||
+      (rc = c->upload(&B.slot, slot)) || (rc = c->upload(&B.code, code)) ||
+      (rc = c->upload(&B.slots, slots)))
+    return rc;
+  c->bt = B;
+  if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }  // captured without them
+  // the values at the samples' current time
+  tsm::StageArgs A = c->A;
+  A.ignore_done = 1;
+  tsm::launch_bc_time(c->G, c->bt, A, c->S, 0, c->st);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->st));
   return TSM_OK;
 }
 
@@ -737,6 +829,7 @@ void enqueue_step(tsm_ctx* c, int log_slot, bool dt_ready = false) {
   if (c->P.liou_fix) tsm::launch_ssw(c->G, c->A, c->S, c->st);
   for (int r = 0; r < nrk; ++r) {
     c->bind_state();
+    tsm::launch_bc_time(c->G, c->bt, c->A, c->S, r, c->st);
     tsm::launch_gradient(c->G, c->A, c->S, c->st);
     tsm::launch_flux_update(c->P.flux, c->P.reconstruct, c->G, c->A, c->S, r, r == nrk - 1, mode,
                             c->st, true, true);
@@ -766,6 +859,7 @@ void enqueue_tiled_step(tsm_ctx* c, int log_slot, hipEvent_t* ev = nullptr) {
     c->A.W_in = ins[r];
     c->A.W_out = outs[r];
     if (ev) (void)hipEventRecord(ev[e++], c->st);
+    tsm::launch_bc_time(c->G, c->bt, c->A, c->S, r, c->st);
     if (tsm::tile_needs_gradient_pass(c->TG, c->P.flux, c->P.reconstruct)) {
       // boundary vertices, then the tiles (measured: on a second stream beside the
       // tile kernel the two compete for CU slots and the stage takes longer)

@@ -73,6 +73,19 @@ struct TileGeo {
   const unsigned short* gent;
 };
 
+// Boundary values that depend on time: entry j writes the value at offset
+// target[j] of each sample's boundary block, evaluating the program that
+// starts at code[prog[j]] over the slot values from slots[slot[j]] and the
+// stage time (tprog.h).
+struct BcTime {
+  int n;
+  const int* target;
+  const int* prog;
+  const int* slot;
+  const int* code;
+  const double* slots;
+};
+
 struct ViscParams {
   double mu_ref, T_ref, T_0, omega, prandtl;
   int mu_model, visc_active, heat_active;
@@ -332,6 +345,10 @@ void launch_bounds(const DevGeo& G, const StageArgs& A, int s, double* out, int 
                    hipStream_t st);
 // strong boundary conditions on the stage output A.W_out of every live sample
 void launch_strong_bc(const DevGeo& G, const StageArgs& A, int S, hipStream_t st);
+// Evaluates the time-dependent boundary values of every sample at its stage-r
+// time t + r dt (Heun).
+void launch_bc_time(const DevGeo& G, const BcTime& B, const StageArgs& A, int S, int r,
+                    hipStream_t st);
 void launch_dt(const DevGeo& G, const StageArgs& A, const StepParams& P, int S, hipStream_t st);
 void launch_step_final(const DevGeo& G, const StageArgs& A, const StepParams& P, int S,
                        int log_slot, hipStream_t st);

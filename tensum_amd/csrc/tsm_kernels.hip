@@ -12,6 +12,7 @@
 
 #include "tsm_device.h"
 #include "tsm_physics.h"
+#include "tprog.h"
 
 namespace tsm {
 
@@ -736,6 +737,105 @@ __global__ __launch_bounds__(256) void k_strong_bc(DevGeo G, StageArgs A) {
 void launch_strong_bc(const DevGeo& G, const StageArgs& A, int S, hipStream_t st) {
   if (G.nsbv > 0)
     hipLaunchKernelGGL(k_strong_bc, dim3((G.nsbv + 255) / 256, S), dim3(256), 0, st, G, A);
+}
+
+// Time-dependent boundary values of Heun's stage r, evaluated at the stage
+// time t^n + r dt (src/fv.cc:2296-2298) before the stage's first kernel: the
+// reference's boundary conditions evaluate their expressions at (x, y,
+// int_step_time) each time they are applied during the stage
+// (src/bc.h:411-562).  One lane per (entry, sample) interprets the entry's
+// postfix program; the comparisons keep the parser's 1e-12 tolerance and
+// & | ! take |v| >= 0.5 as true (expr.cpp).
+__device__ __forceinline__ double tp_f1(int f, double a) {
+  switch (f) {
+    case TF_EXP: return exp(a);
+    case TF_EXP2: return exp2(a);
+    case TF_LOG: return log(a);
+    case TF_LOG10: return log10(a);
+    case TF_LOG2: return log2(a);
+    case TF_SQRT: return sqrt(a);
+    case TF_CBRT: return cbrt(a);
+    case TF_SIN: return sin(a);
+    case TF_COS: return cos(a);
+    case TF_TAN: return tan(a);
+    case TF_ASIN: return asin(a);
+    case TF_ACOS: return acos(a);
+    case TF_ATAN: return atan(a);
+    case TF_SINH: return sinh(a);
+    case TF_COSH: return cosh(a);
+    case TF_TANH: return tanh(a);
+    case TF_ABS: return fabs(a);
+    case TF_FLOOR: return floor(a);
+    case TF_CEIL: return ceil(a);
+    case TF_TRUNC: return trunc(a);
+    case TF_INT: return floor(a + 0.5);
+  }
+  return __longlong_as_double(0x7ff8000000000000LL);
+}
+__device__ __forceinline__ double tp_f2(int f, double a, double b) {
+  switch (f) {
+    case TF_POW: return pow(a, b);
+    case TF_ATAN2: return atan2(a, b);
+    case TF_MIN: return a < b ? a : b;
+    case TF_MAX: return a > b ? a : b;
+    case TF_HYPOT: return hypot(a, b);
+  }
+  return __longlong_as_double(0x7ff8000000000000LL);
+}
+
+__global__ __launch_bounds__(64) void k_bc_time(DevGeo G, BcTime B, StageArgs A, int r) {
+  const int s = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= B.n) return;
+  const Ctl& c = A.ctl[s];
+  if (!A.ignore_done && c.done) return;
+  const double t = c.t + (double)r * c.dt;
+  const double* sv = B.slots + B.slot[j];
+  double st[TP_MAX_DEPTH];
+  int sp = 0;
+  const auto truth = [](double v) { return fabs(v) >= 0.5; };
+  for (int pc = B.prog[j];; ++pc) {
+    const int ins = B.code[pc], op = ins & 0xff, arg = ins >> 8;
+    if (op == TP_END) break;
+    if (op == TP_SLOT) { st[sp++] = sv[arg]; continue; }
+    if (op == TP_T) { st[sp++] = t; continue; }
+    if (op == TP_NEG) { st[sp - 1] = -st[sp - 1]; continue; }
+    if (op == TP_NOT) { st[sp - 1] = truth(st[sp - 1]) ? 0.0 : 1.0; continue; }
+    if (op == TP_F1) { st[sp - 1] = tp_f1(arg, st[sp - 1]); continue; }
+    if (op == TP_IF) {
+      sp -= 2;
+      st[sp - 1] = truth(st[sp - 1]) ? st[sp] : st[sp + 1];
+      continue;
+    }
+    const double a = st[sp - 2], b = st[sp - 1];
+    double v;
+    switch (op) {
+      case TP_ADD: v = a + b; break;
+      case TP_SUB: v = a - b; break;
+      case TP_MUL: v = a * b; break;
+      case TP_DIV: v = a / b; break;
+      case TP_MOD: v = fmod(a, b); break;
+      case TP_POW: v = pow(a, b); break;
+      case TP_LT: v = (a < b - 1e-12) ? 1.0 : 0.0; break;
+      case TP_GT: v = (b < a - 1e-12) ? 1.0 : 0.0; break;
+      case TP_LE: v = (a <= b + 1e-12) ? 1.0 : 0.0; break;
+      case TP_GE: v = (b <= a + 1e-12) ? 1.0 : 0.0; break;
+      case TP_EQ: v = (fabs(a - b) <= 1e-12) ? 1.0 : 0.0; break;
+      case TP_NE: v = (fabs(a - b) > 1e-12) ? 1.0 : 0.0; break;
+      case TP_AND: v = (truth(a) && truth(b)) ? 1.0 : 0.0; break;
+      case TP_OR: v = (truth(a) || truth(b)) ? 1.0 : 0.0; break;
+      default: v = tp_f2(arg, a, b); break;  // TP_F2
+    }
+    st[--sp - 1] = v;
+  }
+  double* bs = const_cast<double*>(G.bstate) + (size_t)s * G.bs_sample;
+  bs[B.target[j]] = st[0];
+}
+
+void launch_bc_time(const DevGeo& G, const BcTime& B, const StageArgs& A, int S, int r,
+                    hipStream_t st) {
+  if (B.n > 0)
+    hipLaunchKernelGGL(k_bc_time, dim3((B.n + 63) / 64, S), dim3(64), 0, st, G, B, A, r);
 }
 
 // Solution bounds (src/fv.cc:2099-2171): per block the minimum and maximum of

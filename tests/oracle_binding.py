@@ -55,6 +55,7 @@ def lib():
         L.orc_compute_dt.argtypes = [C.c_void_p]
         L.orc_local_dt.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_set_bc_state.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_set_bc_time_fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_forces.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
         L.orc_stage_gradients.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_get_clock.argtypes = [C.c_void_p] + [C.c_void_p] * 4
@@ -93,6 +94,9 @@ def params(p):
     for name, _ in OrcParams._fields_:
         setattr(o, name, getattr(c, name))
     return o
+
+
+_BT_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.POINTER(C.c_double))
 
 
 class Oracle:
@@ -153,6 +157,25 @@ class Oracle:
         a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
         bv = np.ascontiguousarray(self._bface_v, dtype=np.int32)
         if lib().orc_set_bc_state(self._h, _p(bv), _p(a)) != 0:
+            raise RuntimeError("oracle: " + lib().orc_error().decode())
+
+    def set_bc_time_fn(self, fn):
+        """fn(t, values) updates values (n_bface, 2, 4) = (T,u,v,p) per bface
+        end point in place at each Heun stage time t; None removes it."""
+        bv = np.ascontiguousarray(self._bface_v, dtype=np.int32)
+        self._bt_bv = bv
+        if fn is None:
+            self._bt_cb = None
+            cb = None
+        else:
+            nb = bv.size // 2
+
+            def call(_user, t, ptr):
+                a = np.ctypeslib.as_array(ptr, shape=(nb, 2, 4))
+                fn(t, a)
+            self._bt_cb = _BT_FN(call)
+            cb = C.cast(self._bt_cb, C.c_void_p)
+        if lib().orc_set_bc_time_fn(self._h, _p(bv), cb, None) != 0:
             raise RuntimeError("oracle: " + lib().orc_error().decode())
 
     def forces(self, tags, force_index=0):

@@ -656,3 +656,56 @@ def test_driver_runs_the_reference_flat_plate_example(tmp_path):
     p = _vtk_scalars(sd / "sol0001_0.vtk", "pressure", mesh.n_vertex)
     np.testing.assert_allclose(p, o.get_state()[:, 3], rtol=1e-10)
     assert len((sd / "global.dat").read_text().splitlines()) == 50
+
+
+@pytest.mark.gpu
+def test_driver_boundary_values_of_time_match_oracle(tmp_path):
+    """Boundary expressions of t under time_scheme heuns (the reference sets
+    the time they are evaluated at only there, src/fv.cc:2296-2298): a moving
+    wall and a pulsating inlet on Navier-Stokes walls, against the oracle fed
+    the same fields per stage; under ssprk3 the driver refuses them."""
+    import re
+    mesh = T.jittered_mesh(17, seed=16)
+    write_part_file(mesh, str(tmp_path))
+    text = VORTEX_PARAM.format(final_time=0.05, stamps=1)
+    bc_text = NS_WALL_BC.replace("xvelocity    0.1", "xvelocity    0.1*cos(5*t)").replace(
+        "xvelocity    0.5", "xvelocity    0.5 + 0.1*sin(8*t + y)")
+    text = re.sub(r"boundary\n\{.*?\n\}\nexact_soln", bc_text + "\nexact_soln", text, flags=re.S)
+    for a, b in (("model       euler", "model       ns"), ("mu_ref   0.0", "mu_ref   0.01"),
+                 ("find_error            yes", "find_error            no"),
+                 ("format                vtk", "format                raw")):
+        assert a in text
+        text = text.replace(a, b)
+    (tmp_path / "param.in").write_text(text)
+    run = lambda: subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=tmp_path,
+                                 capture_output=True, text=True, timeout=120)
+    r = run()
+    assert r.returncode != 0 and "heuns" in r.stderr, r.stderr
+    assert "time_scheme  ssprk3" in text
+    (tmp_path / "param.in").write_text(text.replace("time_scheme  ssprk3", "time_scheme  heuns"))
+    r = run()
+    assert r.returncode == 0, r.stderr
+    sd = tmp_path / "SAMPLE_0"
+    bc = {1: ("noslip", (1.2, 0.0, 0.0, 1.0)), 3: ("noslip", (1.0, 0.1, 0.0, 1.0), True),
+          4: ("inlet", (1.0, 0.5, 0.0, 1.0)), 2: ("outlet",)}
+    om = T.Mesh(mesh.x, mesh.y, mesh.bface_tag, mesh.bface_v, mesh.cell_v, mesh.periodic_ptr,
+                mesh.periodic_v, bc)
+    prm = T.Params(model="ns", mu_ref=0.01, flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.8,
+                   final_time=0.05, time_scheme="heuns")
+    o = O.Oracle(om, prm)
+    o.set_state(vortex(mesh.x, mesh.y, box=None))
+    bv = mesh.bface_v.reshape(-1, 2)
+    tag = mesh.bface_tag
+
+    def fields(t, a):
+        for e in range(2):
+            a[tag == 3, e, 1] = 0.1 * np.cos(5 * t)
+            a[tag == 4, e, 1] = 0.5 + 0.1 * np.sin(8 * t + mesh.y[bv[tag == 4, e]])
+    o.set_bc_time_fn(fields)
+    n, log = o.run(1000, want_log=True)
+    res = np.loadtxt(sd / "residual.dat")
+    assert res.shape[0] == n > 2
+    np.testing.assert_allclose(res[:, 3], log[:, 3], rtol=1e-9)
+    t, W = T.read_raw(str(sd / "sol0001_0.raw"))
+    assert t == pytest.approx(0.05, rel=1e-14)
+    assert rel_l2(W, o.get_state()) <= 1e-10
