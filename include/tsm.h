@@ -63,9 +63,9 @@ typedef struct tsm_params {
   int cell_type;      /* 0 median, 1 voronoi */
   int liou_fix;
   /* Kernel family (no counterpart in the reference): 0 = LDS-tiled stage
-     kernels where the configuration allows (Euler, SSP-RK3, no liou_fix),
-     CSR-gather kernels otherwise; 1 = always the CSR-gather kernels.  Both
-     give the reference's summation order. */
+     kernels where the configuration allows (every model, scheme and flux
+     option except bc_scheme strong), CSR-gather kernels otherwise; 1 = always
+     the CSR-gather kernels.  Both give the reference's summation order. */
   int kernel_path;
   int tile_vertices;  /* owned vertices per tile of the tiled kernels (0: 128) */
   int block_grad;     /* threads per tile of the tiled gradient kernel (0: auto) */

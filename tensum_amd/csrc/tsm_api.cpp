@@ -188,8 +188,9 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     return fail(TSM_E_MESH, "mesh too large for 30-bit face ids");
   }
   // ---- path and vertex numbering -----------------------------------------
-  // tiled stage kernels: SSP-RK3 without the Liou fix, Euler or Navier-Stokes
-  c->tiled = params->kernel_path == 0 && !params->liou_fix && !params->strong_bc;
+  // tiled stage kernels: every scheme, Euler or Navier-Stokes, with or without
+  // the Liou fix; strong boundary conditions keep the CSR kernels
+  c->tiled = params->kernel_path == 0 && !params->strong_bc;
   std::vector<int> perm(nv), inv(nv);
   if (c->tiled) {
     int tb = params->tile_vertices > 0 ? params->tile_vertices : 128;
@@ -855,6 +856,10 @@ void enqueue_tiled_step(tsm_ctx* c, int log_slot, hipEvent_t* ev = nullptr) {
   if (nrk == 3) outs[2] = Wn;
   if (nrk == 2) outs[1] = Wn;
   int e = 0;
+  if (c->P.liou_fix) {  // the step's Liou sensor from W^n (src/fv.cc:2281-2285)
+    c->A.W_in = Wn;
+    tsm::launch_ssw(c->G, c->A, c->S, c->st);
+  }
   for (int r = 0; r < nrk; ++r) {
     c->A.W_in = ins[r];
     c->A.W_out = outs[r];
@@ -1037,9 +1042,9 @@ extern "C" int tsm_run_packed(tsm_ctx** cs, int n, long nsteps, long* done) {
   for (int i = 0; i < n; ++i) {
     const tsm_ctx* c = cs[i];
     if (!c) return fail(TSM_E_ARG, "null context in group");
-    if (!c->tiled || c->TG.fused || c->P.time_scheme != 0 || c->A.ns)
-      return fail(TSM_E_ARG, "packed steps need Euler SSP-RK3 contexts on the tiled kernels "
-                             "with a gradient pass of their own");
+    if (!c->tiled || c->TG.fused || c->P.time_scheme != 0 || c->A.ns || c->P.liou_fix)
+      return fail(TSM_E_ARG, "packed steps need Euler SSP-RK3 contexts without the Liou fix on "
+                             "the tiled kernels with a gradient pass of their own");
     if (c->P.flux != cs[0]->P.flux || c->P.reconstruct != cs[0]->P.reconstruct)
       return fail(TSM_E_ARG, "packed contexts must share the flux and the reconstruction");
     for (int j = 0; j < i; ++j)
@@ -1419,45 +1424,34 @@ extern "C" int tsm_moments(tsm_ctx* c, double* mean, double* m2, double* s2) {
 extern "C" int tsm_probe_density(tsm_ctx* c, int np, const double* px, const double* py,
                                  double* out) {
   if (!c || np < 1 || !px || !py || !out) return fail(TSM_E_ARG, "bad argument");
-  const auto& g = c->geo;
-  std::vector<int> pc(3 * (size_t)np);
-  std::vector<double> pw(3 * (size_t)np);
-  for (int p = 0; p < np; ++p) {  // brute-force point location (once per call)
-    int found = -1;
-    double best = -1e300, wb[3] = {0, 0, 0};
-    for (int k = 0; k < g.nc && found < 0; ++k) {
-      const int* v = &g.cell[3 * (size_t)k];
-      double x0 = g.x[v[0]], y0 = g.y[v[0]], x1 = g.x[v[1]], y1 = g.y[v[1]], x2 = g.x[v[2]], y2 = g.y[v[2]];
-      double det = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0);
-      double l1 = ((px[p] - x0) * (y2 - y0) - (x2 - x0) * (py[p] - y0)) / det;
-      double l2 = ((x1 - x0) * (py[p] - y0) - (px[p] - x0) * (y1 - y0)) / det;
-      double l0 = 1.0 - l1 - l2;
-      double mn = std::fmin(l0, std::fmin(l1, l2));
-      if (mn >= -1e-12) { found = k; wb[0] = l0; wb[1] = l1; wb[2] = l2; }
-      else if (mn > best) best = mn;
-    }
-    if (found < 0) return fail(TSM_E_ARG, "probe point " + std::to_string(p) + " outside the mesh");
-    for (int j = 0; j < 3; ++j) {
-      pc[3 * (size_t)p + j] = c->perm[g.cell[3 * (size_t)found + j]];
-      pw[3 * (size_t)p + j] = wb[j];
-    }
-  }
-  int* dpc = nullptr;
-  double *dpw = nullptr, *dout = nullptr;
-  HIP_TRY(hipMalloc(&dpc, pc.size() * sizeof(int)));
-  hipError_t e = hipMalloc(&dpw, pw.size() * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&dout, (size_t)np * c->S * sizeof(double));
-  if (e == hipSuccess) e = hipMemcpy(dpc, pc.data(), pc.size() * sizeof(int), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(dpw, pw.data(), pw.size() * sizeof(double), hipMemcpyHostToDevice);
+  // device point location (k_probe_locate: the first cell in cell order that
+  // holds the point), then the interpolation of every sample
+  const size_t n = (size_t)np;
+  double *dpx = nullptr, *dpy = nullptr, *dpw = nullptr, *dout = nullptr;
+  int *dpc = nullptr, *dcell = nullptr;
+  hipError_t e = hipMalloc(&dpx, 8 * n);
+  if (e == hipSuccess) e = hipMalloc(&dpy, 8 * n);
+  if (e == hipSuccess) e = hipMalloc(&dpw, 24 * n);
+  if (e == hipSuccess) e = hipMalloc(&dout, 8 * n * (size_t)c->S);
+  if (e == hipSuccess) e = hipMalloc(&dpc, 12 * n);
+  if (e == hipSuccess) e = hipMalloc(&dcell, 4 * n);
+  if (e == hipSuccess) e = hipMemcpyAsync(dpx, px, 8 * n, hipMemcpyHostToDevice, c->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(dpy, py, 8 * n, hipMemcpyHostToDevice, c->st);
   if (e == hipSuccess) {
+    tsm::launch_probe_locate(c->G, np, dpx, dpy, dpc, dpw, dcell, c->st);
     c->bind_state();
     tsm::launch_probes(c->A, c->G.nv, np, dpc, dpw, c->S, dout, c->st);
     e = hipGetLastError();
   }
+  std::vector<int> cells(n);
+  if (e == hipSuccess) e = hipMemcpyAsync(cells.data(), dcell, 4 * n, hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, 8 * n * (size_t)c->S, hipMemcpyDeviceToHost, c->st);
   if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-  if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)np * c->S * sizeof(double), hipMemcpyDeviceToHost);
-  (void)hipFree(dpc); (void)hipFree(dpw); (void)hipFree(dout);
+  (void)hipFree(dpx); (void)hipFree(dpy); (void)hipFree(dpw); (void)hipFree(dout);
+  (void)hipFree(dpc); (void)hipFree(dcell);
   if (e != hipSuccess) return fail(TSM_E_HIP, hipGetErrorString(e));
+  for (int p = 0; p < np; ++p)
+    if (cells[p] < 0) return fail(TSM_E_ARG, "probe point " + std::to_string(p) + " outside the mesh");
   return TSM_OK;
 }
 

@@ -139,12 +139,14 @@ __device__ __forceinline__ double bmin(double x, double* red) {
   return x;
 }
 
+// ssw: the face's Liou sensor, ssw[l] + ssw[r] (src/fv.cc:1081; KEPES-TeCNO Roe only)
 template <int FLUX, int REC>
 __device__ __forceinline__ Cons pair_flux(const Gas& gas, const FluxOpts& fo, double rx, double ry,
                                           const Side& L, const Side& R, const double* gdl,
-                                          const double* gdr, double unx, double uny) {
+                                          const double* gdr, double unx, double uny,
+                                          double ssw = 0.0) {
   if (FLUX == F_ROE || FLUX == F_RUS)
-    return tecno_flux<FLUX == F_ROE, REC>(gas, fo, rx, ry, L, R, gdl, gdr, unx, uny, 0.0);
+    return tecno_flux<FLUX == F_ROE, REC>(gas, fo, rx, ry, L, R, gdl, gdr, unx, uny, ssw);
   if (FLUX == F_KEPEC) return kepec_flux_s(gas, L, R, unx, uny);
   if (FLUX == F_KEP) return kep_flux(gas, L.w, R.w, unx, uny);
   if (FLUX == F_ROEEC) return roe_ec_flux(gas, L.w, R.w, unx, uny);
@@ -478,7 +480,9 @@ __device__ __forceinline__ void fu_tile_body(const DevGeo& G, const TileGeo& TG,
     if (L::TEC)
 #pragma unroll
       for (int k = 0; k < 4; ++k) Q[(L::QS + k) * MS + i] = q.V[k];
-    if (FLUX == F_ROE) Q[L::QC * MS + i] = q.c;
+    // Roe: the sound speed, negated where the vertex's Liou sensor is set
+    // (liou_fix; c > 0, so the sign carries the flag without an LDS row)
+    if (FLUX == F_ROE) Q[L::QC * MS + i] = (A.ssw && A.ssw[off + gid] > 0.0) ? -q.c : q.c;
     if (L::GRAD)
 #pragma unroll
       for (int k = 0; k < 8; ++k) Q[(L::QD + k) * MS + i] = A.dE[8 * (off + gid) + k];
@@ -516,8 +520,11 @@ __device__ __forceinline__ void fu_tile_body(const DevGeo& G, const TileGeo& TG,
       Ls.V[k] = L::TEC ? Q[(L::QS + k) * MS + l] : 0.0;
       Rs.V[k] = L::TEC ? Q[(L::QS + k) * MS + rr] : 0.0;
     }
-    Ls.c = (FLUX == F_ROE) ? Q[L::QC * MS + l] : 0.0;
-    Rs.c = (FLUX == F_ROE) ? Q[L::QC * MS + rr] : 0.0;
+    const double cl = (FLUX == F_ROE) ? Q[L::QC * MS + l] : 0.0;
+    const double cr = (FLUX == F_ROE) ? Q[L::QC * MS + rr] : 0.0;
+    Ls.c = fabs(cl);
+    Rs.c = fabs(cr);
+    const double ssw = (cl < 0.0 || cr < 0.0) ? 1.0 : 0.0;
     double gdl[4] = {0, 0, 0, 0}, gdr[4] = {0, 0, 0, 0}, rx = 0.0, ry = 0.0;
     if (L::GRAD) {
       rx = Q[L::QX * MS + rr] - Q[L::QX * MS + l];
@@ -528,7 +535,7 @@ __device__ __forceinline__ void fu_tile_body(const DevGeo& G, const TileGeo& TG,
         gdr[k] = Q[(L::QD + 2 * k) * MS + rr] * rx + Q[(L::QD + 2 * k + 1) * MS + rr] * ry;
       }
     }
-    const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gdl, gdr, nx * ia, ny * ia);
+    const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gdl, gdr, nx * ia, ny * ia, ssw);
     fl[q][0] = F.m * area; fl[q][1] = F.mx * area; fl[q][2] = F.my * area; fl[q][3] = F.e * area;
     fn[q][0] = nx; fn[q][1] = ny; fn[q][2] = area;
   }
@@ -885,7 +892,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
       Q[4 * MS1 + i] = q.rho; Q[5 * MS1 + i] = q.beta;
 #pragma unroll
       for (int k = 0; k < 4; ++k) QV[k * MS1 + i] = q.V[k];
-      if (FL::ROE) QC[i] = q.c;
+      if (FL::ROE) QC[i] = (A.ssw && A.ssw[off + gid] > 0.0) ? -q.c : q.c;  // sign: Liou sensor
     }
   }
   __syncthreads();
@@ -965,8 +972,10 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
     Ls.beta = Q[5 * MS1 + l]; Rs.beta = Q[5 * MS1 + rr];
 #pragma unroll
     for (int k = 0; k < 4; ++k) { Ls.V[k] = QV[k * MS1 + l]; Rs.V[k] = QV[k * MS1 + rr]; }
-    Ls.c = FL::ROE ? QC[l] : 0.0;
-    Rs.c = FL::ROE ? QC[rr] : 0.0;
+    const double cl = FL::ROE ? QC[l] : 0.0, cr = FL::ROE ? QC[rr] : 0.0;
+    Ls.c = fabs(cl);
+    Rs.c = fabs(cr);
+    const double ssw = (cl < 0.0 || cr < 0.0) ? 1.0 : 0.0;
     const double rx = SX[rr] - SX[l], ry = SX[MS2 + rr] - SX[MS2 + l];
     double gdl[4], gdr[4];
 #pragma unroll
@@ -974,7 +983,7 @@ __global__ __launch_bounds__(512, TSM_FU_WAVES) void k_stage_fused(DevGeo G, Til
       gdl[k] = QD[2 * k * MS1 + l] * rx + QD[(2 * k + 1) * MS1 + l] * ry;
       gdr[k] = QD[2 * k * MS1 + rr] * rx + QD[(2 * k + 1) * MS1 + rr] * ry;
     }
-    const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gdl, gdr, nx * ia, ny * ia);
+    const Cons F = pair_flux<FLUX, REC>(gas, A.fo, rx, ry, Ls, Rs, gdl, gdr, nx * ia, ny * ia, ssw);
     fl[q][0] = F.m * area; fl[q][1] = F.mx * area; fl[q][2] = F.my * area; fl[q][3] = F.e * area;
     fn[q][0] = nx; fn[q][1] = ny; fn[q][2] = area;
   }

@@ -85,6 +85,30 @@ def test_residual_liou_fix_and_evmod():
     assert rel_l2(s.residual(), o.residual()) < 1e-12
 
 
+@pytest.mark.parametrize("path", ["auto", "fused", "graph", "csr"])
+def test_liou_fix_runs_on_every_kernel_family(path):
+    """The Liou carbuncle fix (src/compute_ssw.cc:7-39, used by KEPES-TeCNO Roe,
+    src/kepes_tecno_roe.cc:87) once per step from W^n: 120 steps of the
+    Riemann problem, whose shocks set the sensor, on the tiled kernels (the
+    sensor rides in the sign of the sound speed's LDS row), the fused stage
+    kernel, graph replay and the CSR kernels."""
+    m = riemann_mesh(33)
+    kw = {"auto": {}, "fused": dict(fused_gradient=True), "graph": dict(graph_steps=8),
+          "csr": dict(kernel_path="csr")}[path]
+    prm = T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.5, final_time=0.8,
+                   liou_fix=True, **kw)
+    s, o = _pair(m, prm, riemann(m.x, m.y))
+    s.run(120)
+    o.run(120)
+    assert o.ssw().sum() > 0
+    assert rel_l2(conserved(s.get_state()), conserved(o.get_state())) <= 1e-10
+    plain = T.Solver(m, T.Params(flux="kepes_tecno_roe", reconstruct="minmod", cfl=0.5,
+                                 final_time=0.8, **kw))
+    plain.set_state(riemann(m.x, m.y))
+    plain.run(120)
+    assert not np.array_equal(plain.get_state(), s.get_state())
+
+
 @pytest.mark.parametrize("bc", ["slip", "farfield", "inlet", "pressure"])
 def test_residual_boundary_types(bc):
     st = (1.0, 0.3, 0.1, 1.2)
