@@ -46,10 +46,9 @@ int noalloc_scratch(void** out) {
   return 0;
 }
 
-// Containing triangle of each probe point: the first cell, in cell order, whose
-// barycentric coordinates are all >= -1e-12, with the same expressions as the
-// host's search in tsm_probe_density (so the same cell and the same weights).
-// One workgroup per probe scans the cells; cell[p] = the cell (-1 if none holds
+// Containing triangle of each probe point (tsm_probe_density and its
+// no-allocation variant): the first cell, in cell order, whose barycentric
+// coordinates are all >= -1e-12.  One workgroup per probe scans the cells; cell[p] = the cell (-1 if none holds
 // the point), pc[3p..] = its corner vertex ids (0 if none), pw[3p..] = weights.
 __global__ __launch_bounds__(256) void k_probe_locate(DevGeo G, const double* px, const double* py,
                                                       int* pc, double* pw, int* cell) {

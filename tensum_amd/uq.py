@@ -136,10 +136,14 @@ def _one(inputs, k):
     return {name: float(v[k]) for name, v in inputs.items()}
 
 
-def make_batch(problem: Problem, mesh, inputs):
-    """One Solver holding every sample of `inputs` (dict of arrays)."""
+def make_batch(problem: Problem, mesh, inputs, solver=None):
+    """One Solver holding every sample of `inputs` (dict of arrays): `solver`
+    when given (a context of the same mesh and sample count, whose samples
+    then start again from t = 0: set_state resets each sample's clock), else a
+    new one."""
     n = len(next(iter(inputs.values())))
-    s = Solver(mesh, problem.params, n_samples=n)
+    s = solver if solver is not None else Solver(mesh, problem.params, n_samples=n)
+    assert s.n_samples == n
     gam = np.broadcast_to(problem.gamma(inputs), (n,))
     for k in range(n):
         s.set_material(k, float(gam[k]), 1.0)
@@ -161,7 +165,7 @@ def qoi(problem: Problem, solver: Solver, max_density=1e6):
     return Q
 
 
-def advance(solvers, chunk=64, max_chunks=100000, packed=False):
+def advance(solvers, chunk=64, max_chunks=100000, packed=False, log=None, log_every=30.0):
     """Runs every solver until all its samples reached final_time: the
     solvers' own launches interleaved on their streams, or (packed=True) one
     launch per kernel spanning every solver where their scheme allows it
@@ -169,9 +173,14 @@ def advance(solvers, chunk=64, max_chunks=100000, packed=False):
     results bit for bit).  Measured on the four-level vortex batch (65^2 to
     513^2 with 4096/1024/256/64 samples) the streams are faster: 6.2e9 against
     5.1e9 vertex-stage updates/s, as their gradient and flux passes of
-    different levels overlap."""
+    different levels overlap.  log: called with a progress line every
+    log_every seconds."""
     step = run_packed if packed else run_group
-    for _ in range(max_chunks):
+    t_last = time.perf_counter()
+    for k in range(max_chunks):
+        if log and time.perf_counter() - t_last > log_every:
+            t_last = time.perf_counter()
+            log(f"advanced {k * chunk} steps")
         try:
             done = step(solvers, chunk)
         except TsmError as e:
@@ -191,14 +200,20 @@ def plain_mc(problem: Problem, n_mesh, n_samples, seed=0, skip=0, batch=None, lo
     inputs = sample_inputs(n_samples, problem.bounds, seed=seed, skip=skip)
     batch = batch or n_samples
     out = []
+    s = None  # one context serves every batch of its size (set-up is paid once)
     for b0 in range(0, n_samples, batch):
         part = {k: v[b0:b0 + batch] for k, v in inputs.items()}
-        s = make_batch(problem, mesh, part)
-        advance([s])
+        nb = len(next(iter(part.values())))
+        if s is not None and s.n_samples != nb:
+            s.close()
+            s = None
+        s = make_batch(problem, mesh, part, s)
+        advance([s], log=log)
         out.append(qoi(problem, s))
-        s.close()
         if log:
             log(f"{min(b0 + batch, n_samples)}/{n_samples} samples at {n_mesh}^2")
+    if s is not None:
+        s.close()
     return inputs, np.vstack(out)
 
 
@@ -224,7 +239,7 @@ def moments(Q):
     return M, M2 / max(Q.shape[0] - 1, 1)
 
 
-def mlmc(problem: Problem, levels, samples, seed=0):
+def mlmc(problem: Problem, levels, samples, seed=0, log=None):
     """Multilevel Monte Carlo: E[Q_L] = E[Q_0] + sum_l E[Q_l - Q_{l-1}] with the
     same inputs on both meshes of a level pair, and the variance telescoped
     the same way from unbiased sample variances, Var[Q_L] = Var[Q_0] +
@@ -243,7 +258,7 @@ def mlmc(problem: Problem, levels, samples, seed=0):
         coarse = make_batch(problem, meshes[levels[li - 1]], inp) if li > 0 else None
         groups.append((fine, coarse))
         ins.append(inp)
-    advance([s for g in groups for s in g if s is not None])
+    advance([s for g in groups for s in g if s is not None], log=log)
     mean = 0.0
     sec = 0.0
     se2 = 0.0
@@ -359,23 +374,25 @@ def cv_experiment(problem: Problem, n_fine, n_coarse, n_train, n_pilot, n_pair, 
     }
 
 
-def mlmc_experiment(problem: Problem, levels, samples, n_ref, seed=0, log=None):
+def mlmc_experiment(problem: Problem, levels, samples, n_ref, seed=0, log=None, batch=None):
     """Multilevel MC against plain MC on the finest level at the same
     GPU-seconds, both measured against a plain-MC reference with n_ref samples
     on the finest mesh.  Errors: RMS over the QoIs with non-zero reference
     variance of |estimate - reference| scaled by the reference's standard
-    deviation (means) or variance (variances)."""
+    deviation (means) or variance (variances).  batch: samples per context of
+    the plain-MC runs on the finest mesh (all at once by default)."""
     say = log or (lambda *_: None)
     t0 = time.perf_counter()
-    _, Q_ref = plain_mc(problem, levels[-1], n_ref, seed=seed + 1000)
+    _, Q_ref = plain_mc(problem, levels[-1], n_ref, seed=seed + 1000, batch=batch, log=say)
     t_ref = time.perf_counter() - t0
     m_ref, v_ref = moments(Q_ref)
     keep = v_ref > 1e-24 * np.maximum(1.0, m_ref ** 2)
     say(f"reference: {n_ref} samples at {levels[-1]}^2 in {t_ref:.1f} s")
-    est = mlmc(problem, levels, samples, seed=seed)
+    est = mlmc(problem, levels, samples, seed=seed, log=say)
+    say(f"multilevel estimate in {est.seconds:.1f} s")
     n_plain = max(2, int(round(est.seconds / (t_ref / n_ref))))
     t0 = time.perf_counter()
-    _, Q_pl = plain_mc(problem, levels[-1], n_plain, seed=seed + 7)
+    _, Q_pl = plain_mc(problem, levels[-1], n_plain, seed=seed + 7, batch=batch, log=say)
     t_pl = time.perf_counter() - t0
     m_pl, v_pl = moments(Q_pl)
 

@@ -25,6 +25,21 @@ def test_run_group_equals_separate_runs():
             assert abs(a.time(k)[0] - 0.3) <= 1e-15
 
 
+def test_reused_context_gives_the_results_of_fresh_ones():
+    """plain_mc reuses one context for every batch of its size (set_state
+    restarts each sample's clock); the QoIs equal those of a fresh context per
+    batch bit for bit, including a smaller last batch."""
+    p = uq.vortex_problem(final_time=0.4)
+    u, Q = uq.plain_mc(p, 17, 10, seed=3, batch=4)
+    m = p.mesh(17)
+    for b0 in range(0, 10, 4):
+        part = {k: v[b0:b0 + 4] for k, v in u.items()}
+        s = uq.make_batch(p, m, part)
+        uq.advance([s])
+        assert np.array_equal(uq.qoi(p, s), Q[b0:b0 + 4])
+        s.close()
+
+
 def test_batched_samples_match_one_at_a_time_with_material():
     p = uq.vortex_problem(final_time=0.2)
     inp = uq.sample_inputs(4, p.bounds, seed=3)
