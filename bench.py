@@ -53,18 +53,29 @@ PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                        "riemann1025_tiled_kernels.md")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel
-    calibration): the line 'traffic <kernel>: <bytes> B'.  None if absent."""
+def profile_figure(what, kernel):
+    """A per-launch figure of `kernel` from the committed rocprofv3 summary:
+    the line '<what> <kernel>: <n> B' ('traffic': HBM bytes from FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by the copy-kernel calibration) or
+    '<what> <kernel>: <n> wave-instructions' ('valu': SQ_INSTS_VALU per
+    dispatch).  None if absent."""
     if not os.path.exists(PROFILE):
         return None
     with open(PROFILE) as f:
         for line in f:
-            m = re.match(r"^traffic (\w+): (\d+) B", line.strip())
-            if m and m.group(1) == kernel:
-                return int(m.group(2))
+            m = re.match(r"^(\w+) (\w+): (\d+) ", line.strip())
+            if m and m.group(1) == what and m.group(2) == kernel:
+                return int(m.group(3))
     return None
+
+
+def measured_traffic(kernel):
+    return profile_figure("traffic", kernel)
+
+
+# FP64 vector issue peak of MI355X in wave64 instructions per second: one per
+# CU per clock (256 CUs x 2.4 GHz; 78.6 TFLOP/s = that x 64 lanes x 2 flops).
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 2.4e9
 
 
 # Whole-step algorithmic bytes per vertex-stage of an SSP-RK3 step at the
@@ -90,7 +101,7 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="steps per captured HIP graph (0: none)")
     ap.add_argument("--block", type=int, default=0, help="threads per tile of the stage kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--mc", action="store_true",
                     help="second metric: surrogate control-variate MC against plain MC on the "
                          "uncertain isentropic vortex (prints its own JSON line)")
@@ -216,23 +227,26 @@ def main():
                 "traffic": (measured_traffic(dom) if args.workload == "riemann" and n == 1025
                             else None),
                 "algorithmic_bytes_per_launch": kb[dom],
+                "valu_issue_frac": valu_issue_frac(dom, dur_us, args),
                 "kernel_us": {k: round(v, 2) for k, v in ku.items()},
                 "step_frac_at_389B_per_vertex_stage":
                     round(value * STEP_BYTES_PER_VERTEX_STAGE / (HBM_PEAK_GBS * 1e9), 4)}
 
     cpu = None
     if not args.no_cpu_baseline:
-        import oracle_binding as O
-        print("bench: timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
-        o = O.Oracle(mesh, prm)
-        o.set_state(w0)
-        c0 = time.perf_counter()
-        k = o.run(args.cpu_steps)
-        c1 = time.perf_counter()
-        cpu = {"value": nv * 3 * k / (c1 - c0), "unit": "vertex-stage updates/s", "cores": 1,
-               "kind": "port",
-               "sample": f"{k} SSP-RK3 steps of the same {n}x{n} Riemann workload, oracle (C++ "
-                         f"restatement of the reference), single thread"}
+        cpu = reference_cpu_baseline(args, mesh, prm, w0) if args.workload == "riemann" else None
+        if cpu is None:
+            import oracle_binding as O
+            print("bench: timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
+            o = O.Oracle(mesh, prm)
+            o.set_state(w0)
+            c0 = time.perf_counter()
+            k = o.run(args.cpu_steps)
+            c1 = time.perf_counter()
+            cpu = {"value": nv * 3 * k / (c1 - c0), "unit": "vertex-stage updates/s", "cores": 1,
+                   "kind": "port",
+                   "sample": f"{k} SSP-RK3 steps of the same {n}x{n} {args.workload} workload, "
+                             f"oracle (C++ restatement of the reference), single thread"}
 
     out = {
         "metric": "FP64 vertex-stage updates/s at 1M verts",
@@ -253,6 +267,73 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def valu_issue_frac(kernel, dur_us, args):
+    """VALU wave-instructions per launch of `kernel` (the committed profile's
+    SQ_INSTS_VALU count, 1025^2 Riemann only) over its measured duration,
+    against the FP64 issue peak: the fraction of the vector issue rate the
+    kernel uses."""
+    if args.workload != "riemann" or args.n != 1025:
+        return None
+    n = profile_figure("valu", kernel)
+    return round(n / (dur_us * 1e-6) / VALU_PEAK_WAVE_INSTR_PER_S, 4) if n else None
+
+
+def reference_cpu_baseline(args, mesh, prm, w0):
+    """The reference itself (oracle/_ref/tensum, built from its sources with
+    one MPI rank) on the first steps of the same Riemann workload: the part
+    file of this mesh, the parameter file of the reference's own grammar with
+    final_time at the time of step cpu_steps (from this library's dt, which is
+    the reference's), rate = V x 3 x steps / SAMPLE_SOLVE (the reference's own
+    loop timer).  None when the binary is absent or fails."""
+    import subprocess
+    import tempfile
+
+    import tensum_amd as T
+    from cases import write_part_file
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from ref_cases import param_text
+    exe = os.path.join(ROOT, "oracle", "_ref", "tensum")
+    if not os.access(exe, os.X_OK):
+        return None
+    k = max(1, args.cpu_steps)
+    g = T.Solver(mesh, prm)
+    g.set_state(w0)
+    _, log = g.run(k, want_log=True)
+    g.close()
+    t_k = float(log[-1, 2])
+    print(f"bench: timing the reference binary on {k} steps (t = {t_k!r})", file=sys.stderr,
+          flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        write_part_file(mesh, d)
+        text = param_text("riemann33", t_k, 10 ** 8).replace(
+            "   restart               yes", "   restart               no")
+        with open(os.path.join(d, "param.in"), "w") as f:
+            f.write(text)
+        c0 = time.perf_counter()
+        r = subprocess.run([exe, "-i", "param.in", "-npart", "1", "-L", "."], cwd=d,
+                           capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - c0
+        sd = os.path.join(d, "SAMPLE_0")
+        try:
+            steps = len(open(os.path.join(sd, "residual.dat")).read().split("\n")) - 1
+            solve = None
+            for line in open(os.path.join(sd, "time_log.log")):
+                if line.startswith("SAMPLE_SOLVE"):
+                    solve = 60.0 * float(line.split()[3])  # T_AVG, minutes
+        except OSError:
+            steps, solve = 0, None
+    if r.returncode != 0 or not solve or steps < 1:
+        print(f"bench: reference run failed ({r.returncode}); timing the oracle instead",
+              file=sys.stderr, flush=True)
+        return None
+    return {"value": mesh.n_vertex * 3 * steps / solve, "unit": "vertex-stage updates/s",
+            "cores": 1, "kind": "reference",
+            "sample": f"{steps} SSP-RK3 steps from t = 0 of the same {args.n}x{args.n} Riemann "
+                      f"workload, the reference built from its sources (one MPI rank), timed by "
+                      f"its SAMPLE_SOLVE loop timer ({solve:.1f} s; {wall:.1f} s with start-up "
+                      f"and preprocessing)"}
 
 
 MC_DEFAULTS = {  # fine, coarse, t_final, train, pilot, pair, ref, fine batch
@@ -315,7 +396,7 @@ def mlmc_bench(args):
     samples = tuple(int(x) for x in args.mlmc_samples.split(","))
     say = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
     r = uq.mlmc_experiment(p, levels, samples, args.mc_ref if args.mc_ref is not None else 4096,
-                           log=say)
+                           log=say, batch=args.mc_fine_batch)
     out = {"metric": "MLMC moment error at the GPU-seconds of plain MC (uncertain vortex)",
            "value": r["plain_mc"]["mean_error"] / max(r["mlmc"]["mean_error"], 1e-300),
            "unit": "plain-MC mean error / MLMC mean error at equal GPU-seconds",
