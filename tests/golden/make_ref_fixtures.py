@@ -85,8 +85,49 @@ def run_case(case):
           f"{int(out['clock'][0])}", flush=True)
 
 
+# The output files themselves, for the writer's format: the 33^2 vortex to
+# t = 0.2 with two solution stamps, error norms, global integrals and a restart.
+OUT_CASE, OUT_TIME = "vortex33", 0.2
+OUT_FILES = ("residual.dat", "error_norms.dat", "global.dat", "master_file.visit",
+             "LAST_RESTART_HEAD.txt", "restart_0_0.dat", "sol0000_0.vtk", "sol0001_0.vtk",
+             "sol0002_0.vtk")
+
+
+def output_param_text():
+    """param.in of the output fixture (also used by the test that runs the
+    driver on the same case)."""
+    t = param_text(OUT_CASE, OUT_TIME, 10 ** 8)
+    for a, b in (("   soln_output           no", "   soln_output           yes"),
+                 ("   time_stamps           1", "   time_stamps           2"),
+                 ("   save_global           no", "   save_global           yes")):
+        assert a in t
+        t = t.replace(a, b)
+    return t
+
+
+def run_output_case():
+    mesh = CASES[OUT_CASE]["mesh"]()
+    with tempfile.TemporaryDirectory() as d:
+        write_part_file(mesh, d, periodic_tags=periodic_tags(OUT_CASE))
+        with open(os.path.join(d, "param.in"), "w") as f:
+            f.write(output_param_text())
+        r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=d,
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"output case: reference run failed\n{r.stdout[-2000:]}")
+        files = {}
+        for name in OUT_FILES:
+            with open(os.path.join(d, "SAMPLE_0", name), "rb") as f:
+                files[name.replace(".", "_")] = np.frombuffer(f.read(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, f"ref_out_{OUT_CASE}.npz"), **files)
+    print(f"output files of {OUT_CASE} to t = {OUT_TIME}: {', '.join(OUT_FILES)}", flush=True)
+
+
 if __name__ == "__main__":
     if not os.path.exists(EXE):
         sys.exit("build the reference first: make -C oracle -f ref.mk")
-    for case in (sys.argv[1:] or list(CASES)):
-        run_case(case)
+    for case in (sys.argv[1:] or list(CASES) + ["outputs"]):
+        if case == "outputs":
+            run_output_case()
+        else:
+            run_case(case)

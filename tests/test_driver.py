@@ -709,3 +709,52 @@ def test_driver_boundary_values_of_time_match_oracle(tmp_path):
     t, W = T.read_raw(str(sd / "sol0001_0.raw"))
     assert t == pytest.approx(0.05, rel=1e-14)
     assert rel_l2(W, o.get_state()) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_driver_output_files_match_the_reference_files(tmp_path):
+    """The writer against the reference's own output files
+    (tests/golden/ref_out_vortex33.npz, written by the reference built from its
+    sources, tests/golden/make_ref_fixtures.py): the 33^2 periodic vortex to
+    t = 0.2 with two solution stamps, error norms, global integrals and the
+    restart file.  Every line has the reference's layout (widths, precisions,
+    separators, headers); numbers agree to their last printed digits up to
+    the run's 1e-10-level differences."""
+    import sys
+    sys.path.insert(0, GOLD)
+    from make_ref_fixtures import OUT_CASE, OUT_FILES, output_param_text
+    from ref_cases import CASES, periodic_tags
+    from output_compare import compare
+    mesh = CASES[OUT_CASE]["mesh"]()
+    write_part_file(mesh, str(tmp_path), periodic_tags=periodic_tags(OUT_CASE))
+    (tmp_path / "param.in").write_text(output_param_text())
+    r = subprocess.run([EXE, "-i", "param.in", "-npart", "1", "-L", "."], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    ref = np.load(os.path.join(GOLD, f"ref_out_{OUT_CASE}.npz"))
+    sd = tmp_path / "SAMPLE_0"
+    report = {}
+    for name in OUT_FILES:
+        want = ref[name.replace(".", "_")].tobytes().decode()
+        got = (sd / name).read_text()
+        # error norms are printed to 9 digits of a quantity the oracle bar holds
+        # to 2e-8; everything else to its own printed precision
+        tol = dict(rtol=1e-6, atol=1e-12) if name == "error_norms.dat" else {}
+        report[name] = compare(want, got, name=name, **tol)
+    print("lines differing within tolerance:", report, flush=True)
+
+
+def test_output_comparison_pins_layout_and_values():
+    """The comparison the reference-file test uses: last-digit differences
+    pass, a different precision, width, label or value does not."""
+    from output_compare import compare
+    ref = "         1  2.190622656166e-02  1.000000000000e+00\nN = 1089\n1  0.0219062  100.558\n"
+    assert compare(ref, ref) == 0
+    assert compare(ref, ref.replace("2.190622656166e-02", "2.190622656167e-02")) == 1
+    for bad in (ref.replace("2.190622656166e-02", "2.19062265617e-02"),    # precision
+                ref.replace("         1", "        1"),                      # width
+                ref.replace("N = 1089", "n = 1089"),                        # label
+                ref.replace("100.558", "100.559"),                          # value
+                ref + "extra\n"):                                           # lines
+        with pytest.raises(AssertionError):
+            compare(ref, bad)
