@@ -75,8 +75,10 @@ typedef struct tsm_params {
                               instead of in a pass of their own */
   int graph_steps;    /* tiled path: capture this many steps in one HIP graph and
                          replay it (0: launch every kernel from the host) */
-  int host_preprocess; /* 1: build the dual-mesh geometry on the host; 0: on the
-                          device for median duals (the same arrays bit for bit) */
+  int host_preprocess; /* 1: build the dual-mesh geometry and the tiling on the
+                          host; 0: on the device for median duals (the tiling on
+                          meshes without periodic sets and without the second
+                          halo ring; the same arrays bit for bit) */
   int strong_bc;      /* bc_scheme (src/parameter.h:31-34): 0 weak; 1 strong: after
                          every stage update each bface, in bface order, imposes its
                          boundary condition on its end points' state (src/fv.cc:
@@ -336,6 +338,15 @@ int tsm_host_geometry(const tsm_mesh* mesh, int cell_type, int* face_v, double* 
 int tsm_device_geometry(const tsm_mesh* mesh, int cell_type, int* face_v, double* face_n,
                         int* bface_v, double* bface_n, double* dcarea, double* mcarea,
                         int* n_face, int* n_bface);
+
+/* The vertex tiling built on the device (median duals without periodic sets,
+   one halo ring: what tsm_create builds there unless host_preprocess) against
+   the host's build of the same mesh, array by array: *n_diff = differing
+   entries over all arrays, which = name of the first array that differs (""
+   if none; a buffer of at least 16 bytes).  TSM_E_ARG for meshes the device
+   leaves to the host. */
+int tsm_device_tiling_check(const tsm_mesh* mesh, int tile_vertices, long long* n_diff,
+                            char* which);
 
 /* Host-only: builds and bounds-checks the vertex tiling of the tiled kernels
    (two_rings: with the second halo ring of the fused stage kernel, on meshes

@@ -373,6 +373,17 @@ def _geometry(fn, mesh, cell):
                 bface_n=bn.reshape(-1, 2), dcarea=dc, mcarea=mc)
 
 
+def device_tiling_check(mesh: Mesh, tile_vertices=128):
+    """The vertex tiling built on the device against the host's build (median
+    duals without periodic sets): (differing entries over all arrays, name of
+    the first array that differs or "")."""
+    m, keep = mesh.to_c()
+    n = C.c_longlong(0)
+    which = C.create_string_buffer(32)
+    _check(lib().tsm_device_tiling_check(C.byref(m), int(tile_vertices), C.byref(n), which))
+    return n.value, which.value.decode()
+
+
 def host_tiling(mesh: Mesh, tile_vertices=128, cell="median", two_rings=False):
     """Builds and bounds-checks the tiled kernels' vertex tiling on the host."""
     m, keep = mesh.to_c()

@@ -3,8 +3,8 @@
 // The device works on tiles: a tile owns up to `tb` vertices (consecutive new
 // vertex ids) and sees a halo of every vertex adjacent to them.  Per tile the
 // host prepares
-//   * the halo list (new ids), so slot s < n_own is owned vertex vbeg+s and
-//     slot s >= n_own is halo[hbeg + s - n_own];
+//   * the halo list (new ids, ascending), so slot s < n_own is owned vertex
+//     vbeg+s and slot s >= n_own is halo[hbeg + s - n_own];
 //   * the local faces: every dual face with an owned endpoint, in reference
 //     face order, as (left slot, right slot) + normal;
 //   * per owned vertex, 16-bit gather entries into the local faces, in the
@@ -138,6 +138,10 @@ bool build_tiles(const Geometry& g, int tb, Tiling& T, std::string& msg, bool ri
         for (int j = 0; j < 3; ++j) add_slot(T.perm[g.cell[3 * (size_t)c + j]]);
     for (int v = b0; v < b1; ++v)
       for (int f : vfaces[v]) { add_slot(T.perm[g.face[f].v[0]]); add_slot(T.perm[g.face[f].v[1]]); }
+    // halo slots in ascending vertex id (the order the device tiling forms)
+    std::sort(slots.begin() + nown, slots.end());
+    std::sort(T.halo.begin() + T.hbeg.back(), T.halo.end());
+    for (int k = nown; k < (int)slots.size(); ++k) slot[slots[k]] = k;
     T.hbeg.push_back((int)T.halo.size());
     const int n1 = (int)slots.size();
     if (T.fused) {

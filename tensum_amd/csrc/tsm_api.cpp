@@ -192,6 +192,8 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
   // the Liou fix; strong boundary conditions keep the CSR kernels
   c->tiled = params->kernel_path == 0 && !params->strong_bc;
   std::vector<int> perm(nv), inv(nv);
+  tsm::TileGeo dev_tg{};   // device-built tile arrays (dev_tiles)
+  bool dev_tiles = false;
   if (c->tiled) {
     int tb = params->tile_vertices > 0 ? params->tile_vertices : 128;
     if (tb > 256) { delete c; return fail(TSM_E_ARG, "tile_vertices must be <= 256"); }
@@ -202,7 +204,22 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     const bool grad = (params->flux == 0 || params->flux == 1) && params->reconstruct != 0;
     const bool ring2 = grad && params->tile_fused_gradient && params->model == 0 &&
                        params->time_scheme == 0;
-    if (!tsm::build_tiles(g, tb, c->tiling, msg, ring2) || !tsm::validate_tiles(g, c->tiling, msg)) {
+    // on the device where the mesh allows it (no periodic sets, one halo
+    // ring), else on the host; both give the same arrays
+    int trc = 1;
+    if (!params->host_preprocess) {
+      auto dalloc = [c](void** p, size_t bytes) -> int {
+        char* q = nullptr;
+        const int r = c->alloc(&q, bytes);
+        *p = q;
+        return r;
+      };
+      trc = tsm::build_tiles_device(g, tb, ring2, c->tiling, dev_tg, dalloc, msg);
+      if (trc < 0) { delete c; return fail(trc, "tiling: " + msg); }
+    }
+    dev_tiles = trc == 0;
+    if (!dev_tiles &&
+        (!tsm::build_tiles(g, tb, c->tiling, msg, ring2) || !tsm::validate_tiles(g, c->tiling, msg))) {
       delete c;
       return fail(TSM_E_MESH, "tiling: " + msg);
     }
@@ -407,7 +424,7 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     tsm::TileGeo& TG = c->TG;
     TG.ntiles = T.ntiles; TG.tb = T.tb; TG.max_slots = T.max_slots; TG.max_faces = T.max_faces;
     TG.n_periodic_tiles = T.n_periodic_tiles;
-    TG.n_gbv = (int)T.gbv.size();
+    TG.n_gbv = dev_tiles ? dev_tg.n_gbv : (int)T.gbv.size();
     TG.gbv = nullptr;
     TG.fused = T.fused ? 1 : 0;
     TG.max_slots1 = T.max_slots1;
@@ -425,10 +442,18 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
                                  std::to_string(bg) + "/" + std::to_string(bf) + ")");
     }
 #define UT(field, vec) do { if ((rc = c->upload(&TG.field, vec))) { delete c; return rc; } } while (0)
-    UT(vbeg, T.vbeg); UT(hbeg, T.hbeg); UT(halo, T.halo); UT(fbeg, T.fbeg); UT(bbeg, T.bbeg);
-    UT(tbf, T.tbf); UT(gptr, T.gptr); UT(cptr, T.cptr); UT(cgid, T.cgid); UT(tper, T.tper);
-    UT(pinfo, T.pinfo); UT(f_lr, T.f_lr); UT(cent, T.cent); UT(f_n, T.f_n); UT(gent, T.gent);
-    if (!T.gbv.empty()) UT(gbv, T.gbv);
+    if (dev_tiles) {
+      TG.vbeg = dev_tg.vbeg; TG.hbeg = dev_tg.hbeg; TG.halo = dev_tg.halo; TG.fbeg = dev_tg.fbeg;
+      TG.bbeg = dev_tg.bbeg; TG.tbf = dev_tg.tbf; TG.gptr = dev_tg.gptr; TG.cptr = dev_tg.cptr;
+      TG.cgid = dev_tg.cgid; TG.tper = dev_tg.tper; TG.pinfo = dev_tg.pinfo; TG.f_lr = dev_tg.f_lr;
+      TG.cent = dev_tg.cent; TG.f_n = dev_tg.f_n; TG.gent = dev_tg.gent;
+      if (TG.n_gbv > 0) TG.gbv = dev_tg.gbv;
+    } else {
+      UT(vbeg, T.vbeg); UT(hbeg, T.hbeg); UT(halo, T.halo); UT(fbeg, T.fbeg); UT(bbeg, T.bbeg);
+      UT(tbf, T.tbf); UT(gptr, T.gptr); UT(cptr, T.cptr); UT(cgid, T.cgid); UT(tper, T.tper);
+      UT(pinfo, T.pinfo); UT(f_lr, T.f_lr); UT(cent, T.cent); UT(f_n, T.f_n); UT(gent, T.gent);
+      if (!T.gbv.empty()) UT(gbv, T.gbv);
+    }
     if (T.fused) {
       UT(h2beg, T.h2beg); UT(halo2, T.halo2); UT(g1beg, T.g1beg); UT(g1ptr, T.g1ptr);
       UT(g1cgid, T.g1cgid); UT(g1ent, T.g1ent);

@@ -1,7 +1,9 @@
 // tsm_host.h -- host-side data structures of libtsm (geometry, layout).
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -83,6 +85,15 @@ struct Tiling {
 };
 
 bool build_tiles(const Geometry& g, int tb, Tiling& T, std::string& msg, bool ring2 = false);
+
+// The same tiling built on the device (tsm_tiling.hip) for meshes without
+// periodic sets and without the second ring: the arrays go straight into
+// device memory from `dalloc` (pointers in TG, with n_gbv); T receives the
+// numbering (perm, inv, vbeg) and the sizes.  0 = built, 1 = left to
+// build_tiles, TSM_E_* with msg.
+struct TileGeo;
+int build_tiles_device(const Geometry& g, int tb, bool ring2, Tiling& T, TileGeo& TG,
+                       const std::function<int(void**, size_t)>& dalloc, std::string& msg);
 bool validate_tiles(const Geometry& g, const Tiling& T, std::string& msg);
 
 // Builds faces, normals and areas exactly as Grid::preproc does for one

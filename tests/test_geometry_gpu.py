@@ -47,6 +47,23 @@ def test_device_geometry_is_the_host_geometry(kind):
         assert np.array_equal(h[k], d[k]), k
 
 
+@pytest.mark.parametrize("kind", sorted(k for k in MESHES if k not in ("vortex_periodic", "couette")))
+def test_device_tiling_is_the_host_tiling(kind):
+    """Tiles, halos (ascending), local faces and boundary faces, gather and
+    cell entries, numbering: built on the device by key sorts and binary
+    searches, equal to the host's build entry for entry."""
+    m = MESHES[kind]()
+    for tv in ((64, 128, 256) if m.n_vertex < 10 ** 5 else (128,)):
+        n, which = T.device_tiling_check(m, tv)
+        assert n == 0, (kind, tv, which)
+
+
+@pytest.mark.parametrize("kind", ["vortex_periodic", "couette"])
+def test_device_tiling_leaves_periodic_meshes_to_the_host(kind):
+    with pytest.raises(T.TsmError):
+        T.device_tiling_check(MESHES[kind]())
+
+
 def test_edges_the_host_repairs_are_left_to_it():
     """An outer edge no boundary face names is a one-cell interior face, which
     the host builder drops; the device builder hands such meshes back."""
