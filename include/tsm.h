@@ -76,9 +76,9 @@ typedef struct tsm_params {
   int graph_steps;    /* tiled path: capture this many steps in one HIP graph and
                          replay it (0: launch every kernel from the host) */
   int host_preprocess; /* 1: build the dual-mesh geometry and the tiling on the
-                          host; 0: on the device for median duals (the tiling on
-                          meshes without periodic sets and without the second
-                          halo ring; the same arrays bit for bit) */
+                          host; 0: on the device for median duals (the tiling
+                          unless the fused kernel's second halo ring is asked
+                          for; the same arrays bit for bit) */
   int strong_bc;      /* bc_scheme (src/parameter.h:31-34): 0 weak; 1 strong: after
                          every stage update each bface, in bface order, imposes its
                          boundary condition on its end points' state (src/fv.cc:
@@ -339,12 +339,11 @@ int tsm_device_geometry(const tsm_mesh* mesh, int cell_type, int* face_v, double
                         int* bface_v, double* bface_n, double* dcarea, double* mcarea,
                         int* n_face, int* n_bface);
 
-/* The vertex tiling built on the device (median duals without periodic sets,
-   one halo ring: what tsm_create builds there unless host_preprocess) against
+/* The vertex tiling built on the device (median duals, one halo ring: what
+   tsm_create builds there unless host_preprocess) against
    the host's build of the same mesh, array by array: *n_diff = differing
    entries over all arrays, which = name of the first array that differs (""
-   if none; a buffer of at least 16 bytes).  TSM_E_ARG for meshes the device
-   leaves to the host. */
+   if none; a buffer of at least 16 bytes). */
 int tsm_device_tiling_check(const tsm_mesh* mesh, int tile_vertices, long long* n_diff,
                             char* which);
 

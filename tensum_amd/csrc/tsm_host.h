@@ -87,7 +87,7 @@ struct Tiling {
 bool build_tiles(const Geometry& g, int tb, Tiling& T, std::string& msg, bool ring2 = false);
 
 // The same tiling built on the device (tsm_tiling.hip) for meshes without
-// periodic sets and without the second ring: the arrays go straight into
+// the second halo ring: the arrays go straight into
 // device memory from `dalloc` (pointers in TG, with n_gbv); T receives the
 // numbering (perm, inv, vbeg) and the sizes.  0 = built, 1 = left to
 // build_tiles, TSM_E_* with msg.

@@ -1,10 +1,11 @@
 // tsm_tiling.hip -- the vertex tiling of the LDS-staged kernels built on the
 // device (the arrays build_tiles forms on the host, tiles.cpp, bit for bit),
-// for meshes without periodic sets and without the second halo ring.
+// for every mesh but those of the fused kernel's second halo ring.
 //
 // Every per-tile list is a sort of 64-bit keys (tile << 32 | id) followed by
-// binary searches: vertices in Morton order of their quantised coordinates
-// (a stable radix sort), per-vertex cell, face and boundary-face lists in
+// binary searches: periodic-set members first (whole sets in set order, in
+// tiles of their own), then the other vertices in Morton order of their
+// quantised coordinates (a stable radix sort), per-vertex cell, face and boundary-face lists in
 // ascending id (the reference's order), the halo of each tile (every corner
 // of its vertices' cells and every end of their faces that another tile
 // owns, ascending), its local faces (every face with an owned end, ascending
@@ -49,13 +50,27 @@ __device__ __forceinline__ unsigned spread16d(unsigned v) {
 
 // Morton key of each vertex from its coordinates quantised to 16 bits, with
 // the host's expressions (this file is compiled without contraction too).
+// Periodic-set members get a key above every Morton code: they sort last and
+// the host places them first, in set order.
 __global__ void k_morton(int nv, const double* x, const double* y, double x0, double y0, double sx,
-                         double sy, unsigned* key, int* id) {
+                         double sy, const int* per, unsigned long long* key, int* id) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const unsigned qx = (unsigned)((x[i] - x0) * sx + 0.5), qy = (unsigned)((y[i] - y0) * sy + 0.5);
-  key[i] = spread16d(qx) | (spread16d(qy) << 1);
+  key[i] = ((unsigned long long)(per[i] >= 0 ? 1u : 0u) << 32) | (spread16d(qx) | (spread16d(qy) << 1));
   id[i] = i;
+}
+
+// tile of each new vertex: the last tile start at or below it
+__global__ void k_owner(int nv, int nt, const int* vbeg, int* owner) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nv) return;
+  int lo = 0, hi = nt;  // vbeg[lo] <= n < vbeg[hi]
+  while (hi - lo > 1) {
+    const int m = (lo + hi) >> 1;
+    if (vbeg[m] <= n) lo = m; else hi = m;
+  }
+  owner[n] = lo;
 }
 
 __global__ void k_scatter_perm(int nv, const int* order, int* perm) {
@@ -118,46 +133,46 @@ __global__ void k_halo_count(int nv, const int* vc_ptr, const int* vf_ptr, int* 
 
 // halo candidates of each vertex: the corners of its cells, then the ends of
 // its faces, as (tile << 32 | vertex) where another tile owns them
-__global__ void k_halo_candidates(int nv, int tb, const int* vc_ptr, const int* vc, const int* cell,
-                                  const int* vf_ptr, const int* vf, const int* fv, const int* perm,
-                                  const int* off, unsigned long long* out) {
+__global__ void k_halo_candidates(int nv, const int* owner, const int* vc_ptr, const int* vc,
+                                  const int* cell, const int* vf_ptr, const int* vf, const int* fv,
+                                  const int* perm, const int* off, unsigned long long* out) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nv) return;
-  const unsigned t = (unsigned)(n / tb);
+  const int t = owner[n];
   int o = off[n];
   for (int k = vc_ptr[n]; k < vc_ptr[n + 1]; ++k)
     for (int j = 0; j < 3; ++j) {
       const int u = perm[cell[3 * (size_t)vc[k] + j]];
-      out[o++] = ((unsigned)(u / tb) == t) ? NONE : (((unsigned long long)t << 32) | (unsigned)u);
+      out[o++] = owner[u] == t ? NONE : (((unsigned long long)(unsigned)t << 32) | (unsigned)u);
     }
   for (int k = vf_ptr[n]; k < vf_ptr[n + 1]; ++k)
     for (int j = 0; j < 2; ++j) {
       const int u = perm[fv[2 * (size_t)vf[k] + j]];
-      out[o++] = ((unsigned)(u / tb) == t) ? NONE : (((unsigned long long)t << 32) | (unsigned)u);
+      out[o++] = owner[u] == t ? NONE : (((unsigned long long)(unsigned)t << 32) | (unsigned)u);
     }
 }
 
 // the tiles an element (face or boundary face) is local to: those owning an end
-__global__ void k_local(int ne, int tb, const int* ev, const int* perm, const int* skip,
+__global__ void k_local(int ne, const int* owner, const int* ev, const int* perm, const int* skip,
                         unsigned long long* out) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   if (skip && skip[e]) { out[2 * (size_t)e] = out[2 * (size_t)e + 1] = NONE; return; }
-  const unsigned t0 = (unsigned)(perm[ev[2 * (size_t)e]] / tb);
-  const unsigned t1 = (unsigned)(perm[ev[2 * (size_t)e + 1]] / tb);
+  const unsigned t0 = (unsigned)owner[perm[ev[2 * (size_t)e]]];
+  const unsigned t1 = (unsigned)owner[perm[ev[2 * (size_t)e + 1]]];
   out[2 * (size_t)e] = ((unsigned long long)t0 << 32) | (unsigned)e;
   out[2 * (size_t)e + 1] = t1 == t0 ? NONE : (((unsigned long long)t1 << 32) | (unsigned)e);
 }
 
 struct TileArrays {
-  int nv, tb;
-  const int *perm, *hbeg, *halo;
+  int nv;
+  const int *perm, *owner, *vbeg, *hbeg, *halo;
 };
 
 // slot of new vertex u in tile t: owned slots first, then the tile's halo
 // (a vertex in neither gets a slot past every limit, which the callers flag)
 __device__ __forceinline__ int slot_of(const TileArrays& A, int t, int u) {
-  const int b0 = t * A.tb, b1 = min(b0 + A.tb, A.nv);
+  const int b0 = A.vbeg[t], b1 = A.vbeg[t + 1];
   if (u >= b0 && u < b1) return u - b0;
   const int h0 = A.hbeg[t], h1 = A.hbeg[t + 1];
   const int k = lower_bound32(A.halo, h0, h1, u);
@@ -196,11 +211,11 @@ __global__ void k_vertex_entries(TileArrays A, const int* vf_ptr, const int* vf,
                                  const int* vb_ptr, const int* vb, const int* fbeg, const int* lf,
                                  const int* bbeg, const int* tbf, const int* gptr,
                                  const int* vc_ptr, const int* vc, const int* cell,
-                                 const unsigned char* ccorr, unsigned short* gent, unsigned* cent,
-                                 int* corr, int* bad) {
+                                 const unsigned char* ccorr, int n_periodic_tiles,
+                                 unsigned short* gent, unsigned* cent, int* corr, int* bad) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= A.nv) return;
-  const int t = n / A.tb;
+  const int t = A.owner[n];
   int o = gptr[n];
   for (int k = vf_ptr[n]; k < vf_ptr[n + 1]; ++k) {
     const int f = vf[k];
@@ -227,14 +242,15 @@ __global__ void k_vertex_entries(TileArrays A, const int* vf_ptr, const int* vf,
     cent[k] = s[0] | (s[1] << 10) | (s[2] << 20) | ((unsigned)ccorr[c] << 30);
     any |= ccorr[c];
   }
-  corr[n] = any;
+  // periodic tiles form their boundary corrections themselves
+  corr[n] = t >= n_periodic_tiles ? any : 0;
 }
 
-__global__ void k_tile_sizes(int nt, int tb, int nv, const int* hbeg, const int* fbeg, int* ns,
+__global__ void k_tile_sizes(int nt, const int* vbeg, const int* hbeg, const int* fbeg, int* ns,
                              int* nfl) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
-  ns[t] = min(tb, nv - t * tb) + hbeg[t + 1] - hbeg[t];
+  ns[t] = vbeg[t + 1] - vbeg[t] + hbeg[t + 1] - hbeg[t];
   nfl[t] = fbeg[t + 1] - fbeg[t];
 }
 
@@ -253,7 +269,7 @@ __global__ void k_fill(int n, int v, int* a) {
 int build_tiles_device(const Geometry& g, int tb, bool ring2, Tiling& T, TileGeo& TG,
                        const std::function<int(void**, size_t)>& dalloc, std::string& msg) {
   const int nv = g.nv, nc = g.nc, nf = (int)g.face.size(), nb = (int)g.bface.size();
-  if (ring2 || !g.periodic.empty() || tb <= 0 || nv <= 0) return 1;
+  if (ring2 || tb <= 0 || nv <= 0) return 1;
   // the halo candidates (at most 9 per cell and 4 per face) index with int
   if (9LL * nc + 4LL * nf >= (1LL << 31)) return 1;
 #define TT(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { msg = std::string("device tiling: ") + hipGetErrorString(e_); return TSM_E_HIP; } } while (0)
@@ -261,7 +277,28 @@ int build_tiles_device(const Geometry& g, int tb, bool ring2, Tiling& T, TileGeo
   hipStream_t st;
   TT(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{st};
-  const int nt = (nv + tb - 1) / tb;
+  // ---- tiles: periodic sets first, whole and in set order, a quarter tile
+  // of members per tile (their members lie on opposite sides of the domain,
+  // so each brings its own halo), then every tb vertices of the rest ----------
+  std::vector<int> prefix, tile_start;
+  {
+    const int cap = std::max(4, tb / 4);
+    int cnt = 0;
+    for (const auto& P : g.periodic) {
+      if ((int)P.size() > cap) { msg = "periodic set larger than a tile"; return TSM_E_MESH; }
+      if (cnt == 0 || cnt + (int)P.size() > cap) { tile_start.push_back((int)prefix.size()); cnt = 0; }
+      for (int v : P) prefix.push_back(v);
+      cnt += (int)P.size();
+    }
+  }
+  const int npt = (int)tile_start.size(), np = (int)prefix.size();
+  int n_member = 0;
+  for (int i = 0; i < nv; ++i) n_member += g.periodic_of[i] >= 0;
+  if (n_member != np) { msg = "tiling lost vertices"; return TSM_E_MESH; }
+  for (int k = np; k < nv; k += tb) tile_start.push_back(k);
+  const int nt = (int)tile_start.size();
+  std::vector<int> h_vbeg(tile_start);
+  h_vbeg.push_back(nv);
   // ---- inputs -----------------------------------------------------------------
   const int nbx = std::max(nb, 1);
   std::vector<int> h_fv(2 * (size_t)nf), h_bv(2 * (size_t)nbx, 0), h_lc(nbx, 0), h_bskip(nbx, 1);
@@ -295,8 +332,9 @@ int build_tiles_device(const Geometry& g, int tb, bool ring2, Tiling& T, TileGeo
   size_t tmp_bytes = 0;
   {
     size_t a = 0;
-    TT(hipcub::DeviceRadixSort::SortPairs(nullptr, a, (unsigned*)nullptr, (unsigned*)nullptr,
-                                          (int*)nullptr, (int*)nullptr, nv, 0, 32, st));
+    TT(hipcub::DeviceRadixSort::SortPairs(nullptr, a, (unsigned long long*)nullptr,
+                                          (unsigned long long*)nullptr, (int*)nullptr, (int*)nullptr,
+                                          nv, 0, 33, st));
     tmp_bytes = std::max(tmp_bytes, a);
     TT(hipcub::DeviceRadixSort::SortKeys(nullptr, a, (unsigned long long*)nullptr,
                                          (unsigned long long*)nullptr, (int)n_cap, 0, 64, st));
@@ -322,14 +360,25 @@ int build_tiles_device(const Geometry& g, int tb, bool ring2, Tiling& T, TileGeo
     y0 = std::min(y0, g.y[i]); y1 = std::max(y1, g.y[i]);
   }
   const double sx = (x1 > x0) ? 65535.0 / (x1 - x0) : 0.0, sy = (y1 > y0) ? 65535.0 / (y1 - y0) : 0.0;
-  Scratch<unsigned> mk, mk_s;
-  Scratch<int> mid, order, perm;
-  TT(mk.alloc(nv)); TT(mk_s.alloc(nv)); TT(mid.alloc(nv)); TT(order.alloc(nv)); TT(perm.alloc(nv));
-  hipLaunchKernelGGL(k_morton, dim3(nblk(nv)), dim3(256), 0, st, nv, x.p, y.p, x0, y0, sx, sy, mk.p,
-                     mid.p);
+  Scratch<unsigned long long> mk, mk_s;
+  Scratch<int> mid, msorted, order, perm, per, owner;
+  TT(mk.alloc(nv)); TT(mk_s.alloc(nv)); TT(mid.alloc(nv)); TT(msorted.alloc(nv));
+  TT(order.alloc(nv)); TT(perm.alloc(nv)); TT(per.alloc(nv)); TT(owner.alloc(nv));
+  TT(hipMemcpyAsync(per.p, g.periodic_of.data(), nv * sizeof(int), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_morton, dim3(nblk(nv)), dim3(256), 0, st, nv, x.p, y.p, x0, y0, sx, sy, per.p,
+                     mk.p, mid.p);
   ts = tmp_bytes;
-  TT(hipcub::DeviceRadixSort::SortPairs(tmp.p, ts, mk.p, mk_s.p, mid.p, order.p, nv, 0, 32, st));
+  TT(hipcub::DeviceRadixSort::SortPairs(tmp.p, ts, mk.p, mk_s.p, mid.p, msorted.p, nv, 0, 33, st));
+  // new numbering: the periodic members in set order, then the others by key
+  if (np > 0)
+    TT(hipMemcpyAsync(order.p, prefix.data(), np * sizeof(int), hipMemcpyHostToDevice, st));
+  if (nv > np)
+    TT(hipMemcpyAsync(order.p + np, msorted.p, (nv - np) * sizeof(int), hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_scatter_perm, dim3(nblk(nv)), dim3(256), 0, st, nv, order.p, perm.p);
+  int* d_vbeg = nullptr;
+  TA(d_vbeg, nt + 1);
+  TT(hipMemcpyAsync(d_vbeg, h_vbeg.data(), (nt + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_owner, dim3(nblk(nv)), dim3(256), 0, st, nv, nt, d_vbeg, owner.p);
   // ---- vertex incidences: cells, faces, boundary faces, ascending -------------
   Scratch<unsigned long long> inc, inc_s;
   TT(inc.alloc(n_inc)); TT(inc_s.alloc(n_inc));
@@ -365,7 +414,7 @@ int build_tiles_device(const Geometry& g, int tb, bool ring2, Tiling& T, TileGeo
   if (n_cand < 0 || n_cand > n_cap) { msg = "device tiling: halo candidate count"; return TSM_E_MESH; }
   Scratch<unsigned long long> hk, hk_s, hk_u;
   TT(hk.alloc(n_cand)); TT(hk_s.alloc(n_cand)); TT(hk_u.alloc(n_cand));
-  hipLaunchKernelGGL(k_halo_candidates, dim3(nblk(nv)), dim3(256), 0, st, nv, tb, d_cptr, d_cgid,
+  hipLaunchKernelGGL(k_halo_candidates, dim3(nblk(nv)), dim3(256), 0, st, nv, owner.p, d_cptr, d_cgid,
                      cell.p, vf_ptr.p, vf.p, fv.p, perm.p, hoff.p, hk.p);
   ts = tmp_bytes;
   TT(hipcub::DeviceRadixSort::SortKeys(tmp.p, ts, hk.p, hk_s.p, n_cand, 0, 64, st));
@@ -392,7 +441,7 @@ int build_tiles_device(const Geometry& g, int tb, bool ring2, Tiling& T, TileGeo
   TT(lf_tile.alloc(2 * (size_t)nf));
   auto locals = [&](int ne, const int* ev, const int* skip, int** beg, int** list, int* tile_of,
                     int& count) -> int {
-    hipLaunchKernelGGL(k_local, dim3(nblk(ne)), dim3(256), 0, st, ne, tb, ev, perm.p, skip, lk.p);
+    hipLaunchKernelGGL(k_local, dim3(nblk(ne)), dim3(256), 0, st, ne, owner.p, ev, perm.p, skip, lk.p);
     size_t t2 = tmp_bytes;
     TT(hipcub::DeviceRadixSort::SortKeys(tmp.p, t2, lk.p, lk_s.p, 2 * ne, 0, 64, st));
     hipLaunchKernelGGL(k_count_valid, dim3(1), dim3(64), 0, st, lk_s.p, 2LL * ne, small.p + 2);
@@ -411,7 +460,7 @@ int build_tiles_device(const Geometry& g, int tb, bool ring2, Tiling& T, TileGeo
   if (int r = locals(nf, fv.p, nullptr, &d_fbeg, &d_lf, lf_tile.p, n_lf)) return r;
   if (int r = locals(nbx, bv.p, bskip.p, &d_bbeg, &d_tbf, nullptr, n_tbf)) return r;
   // ---- slots of the local faces, gather and cell entries ----------------------
-  const TileArrays A{nv, tb, perm.p, d_hbeg, d_halo};
+  const TileArrays A{nv, perm.p, owner.p, d_vbeg, d_hbeg, d_halo};
   unsigned* d_flr = nullptr;
   double* d_fn = nullptr;
   TA(d_flr, n_lf);
@@ -440,15 +489,15 @@ int build_tiles_device(const Geometry& g, int tb, bool ring2, Tiling& T, TileGeo
   TT(corr.alloc(nv)); TT(iota.alloc(nv)); TT(gsel.alloc(nv));
   hipLaunchKernelGGL(k_vertex_entries, dim3(nblk(nv)), dim3(256), 0, st, A, vf_ptr.p, vf.p, fv.p,
                      vb_ptr.p, vb.p, d_fbeg, d_lf, d_bbeg, d_tbf, d_gptr, d_cptr, d_cgid, cell.p,
-                     ccorr.p, d_gent, d_cent, corr.p, bad.p);
-  // vertices with a boundary-corrected cell, ascending
+                     ccorr.p, npt, d_gent, d_cent, corr.p, bad.p);
+  // vertices with a boundary-corrected cell (outside the periodic tiles), ascending
   hipLaunchKernelGGL(k_iota, dim3(nblk(nv)), dim3(256), 0, st, nv, iota.p);
   ts = tmp_bytes;
   TT(hipcub::DeviceSelect::Flagged(tmp.p, ts, iota.p, corr.p, gsel.p, small.p + 3, nv, st));
   // largest slot and local-face counts of a tile
   Scratch<int> tns, tnf, mx;
   TT(tns.alloc(nt)); TT(tnf.alloc(nt)); TT(mx.alloc(2));
-  hipLaunchKernelGGL(k_tile_sizes, dim3(nblk(nt)), dim3(256), 0, st, nt, tb, nv, d_hbeg, d_fbeg,
+  hipLaunchKernelGGL(k_tile_sizes, dim3(nblk(nt)), dim3(256), 0, st, nt, d_vbeg, d_hbeg, d_fbeg,
                      tns.p, tnf.p);
   ts = tmp_bytes;
   TT(hipcub::DeviceReduce::Max(tmp.p, ts, tns.p, mx.p, nt, st));
@@ -466,28 +515,35 @@ int build_tiles_device(const Geometry& g, int tb, bool ring2, Tiling& T, TileGeo
   TA(d_gbv, n_gbv);
   if (n_gbv > 0)
     TT(hipMemcpyAsync(d_gbv, gsel.p, n_gbv * sizeof(int), hipMemcpyDeviceToDevice, st));
-  // no periodic sets: every tile is a spatial one
-  int *d_tper = nullptr, *d_pinfo = nullptr, *d_vbeg = nullptr;
+  // periodic tiles and, per member, its set's first slot and size (the
+  // members are the first np new ids, in set order)
+  std::vector<int> h_tper(nt, 0), h_pinfo(nv, -1);
+  {
+    int k = 0, t = 0;
+    for (const auto& P : g.periodic) {
+      while (t + 1 < npt && h_vbeg[t + 1] <= k) ++t;
+      for (size_t m = 0; m < P.size(); ++m) h_pinfo[k + m] = ((k - h_vbeg[t]) << 8) | (int)P.size();
+      h_tper[t] = 1;
+      k += (int)P.size();
+    }
+  }
+  int *d_tper = nullptr, *d_pinfo = nullptr;
   TA(d_tper, nt);
   TA(d_pinfo, nv);
-  TA(d_vbeg, nt + 1);
-  hipLaunchKernelGGL(k_fill, dim3(nblk(nt)), dim3(256), 0, st, nt, 0, d_tper);
-  hipLaunchKernelGGL(k_fill, dim3(nblk(nv)), dim3(256), 0, st, nv, -1, d_pinfo);
+  TT(hipMemcpyAsync(d_tper, h_tper.data(), nt * sizeof(int), hipMemcpyHostToDevice, st));
+  TT(hipMemcpyAsync(d_pinfo, h_pinfo.data(), nv * sizeof(int), hipMemcpyHostToDevice, st));
   // ---- the host's part: numbering and sizes -------------------------------------
   T = Tiling{};
   T.tb = tb;
   T.ntiles = nt;
-  T.n_periodic_tiles = 0;
+  T.n_periodic_tiles = npt;
   T.max_slots = h_mx[0];
   T.max_faces = h_mx[1];
   T.inv.resize(nv);
   T.perm.resize(nv);
   TT(hipMemcpyAsync(T.inv.data(), order.p, nv * sizeof(int), hipMemcpyDeviceToHost, st));
   TT(hipMemcpyAsync(T.perm.data(), perm.p, nv * sizeof(int), hipMemcpyDeviceToHost, st));
-  T.vbeg.resize(nt + 1);
-  for (int t = 0; t < nt; ++t) T.vbeg[t] = t * tb;
-  T.vbeg[nt] = nv;
-  TT(hipMemcpyAsync(d_vbeg, T.vbeg.data(), (nt + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+  T.vbeg = h_vbeg;
   TT(hipStreamSynchronize(st));
   TG.vbeg = d_vbeg; TG.hbeg = d_hbeg; TG.halo = d_halo; TG.fbeg = d_fbeg; TG.bbeg = d_bbeg;
   TG.tbf = d_tbf; TG.gptr = d_gptr; TG.cptr = d_cptr; TG.cgid = d_cgid; TG.tper = d_tper;

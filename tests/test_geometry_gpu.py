@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import tensum_amd as T
-from cases import riemann, riemann_mesh, vortex, vortex_mesh
+from cases import riemann, riemann_mesh, tgv_mesh, vortex, vortex_mesh
 from test_gpu_parity import _fan_mesh
 from test_oracle_walls import couette_mesh
 
@@ -36,6 +36,8 @@ MESHES = {
     "unjittered": lambda: T.jittered_mesh(17, jitter=0.0),
     # the benchmark's mesh: 1,050,625 vertices, 2,097,152 cells
     "riemann_1025": lambda: riemann_mesh(1025),
+    # the periodic Taylor-Green mesh of the Navier-Stokes configuration
+    "tgv_513": lambda: tgv_mesh(513),
 }
 
 
@@ -47,21 +49,16 @@ def test_device_geometry_is_the_host_geometry(kind):
         assert np.array_equal(h[k], d[k]), k
 
 
-@pytest.mark.parametrize("kind", sorted(k for k in MESHES if k not in ("vortex_periodic", "couette")))
+@pytest.mark.parametrize("kind", sorted(MESHES))
 def test_device_tiling_is_the_host_tiling(kind):
-    """Tiles, halos (ascending), local faces and boundary faces, gather and
-    cell entries, numbering: built on the device by key sorts and binary
-    searches, equal to the host's build entry for entry."""
+    """Tiles (periodic sets first, then Morton order), halos (ascending),
+    local faces and boundary faces, gather and cell entries, periodic slot
+    tables, numbering: built on the device by key sorts and binary searches,
+    equal to the host's build entry for entry."""
     m = MESHES[kind]()
     for tv in ((64, 128, 256) if m.n_vertex < 10 ** 5 else (128,)):
         n, which = T.device_tiling_check(m, tv)
         assert n == 0, (kind, tv, which)
-
-
-@pytest.mark.parametrize("kind", ["vortex_periodic", "couette"])
-def test_device_tiling_leaves_periodic_meshes_to_the_host(kind):
-    with pytest.raises(T.TsmError):
-        T.device_tiling_check(MESHES[kind]())
 
 
 def test_edges_the_host_repairs_are_left_to_it():
