@@ -1268,9 +1268,30 @@ bool tile_needs_gradient_pass(const TileGeo& T, int flux, int rec) {
   return grad && !T.fused;
 }
 
+// A gradient pass of at most two tiles per CU cannot fill the GPU, so the
+// eight-wave occupancy of k_grad_tile<false> buys nothing there, while each
+// launch costs several microseconds: on meshes whose periodic sets would split
+// the pass into two launches (periodic tiles, then the rest) and that leave no
+// vertex to k_grad_bnd, such grids take one launch of the periodic variant
+// (its set exchange is skipped for tiles without sets; the same sums in the
+// same order).  Measured on the 129^2 periodic vortex: 63 us per step against
+// 82.  With boundary vertices the split stays: forming their corrections
+// inside the boundary tiles makes those tiles the pass's critical path (93 us
+// per step on the 129^2 Riemann mesh against 112).
+static bool gradient_in_one_launch(const TileGeo& T, int S) {
+  return T.n_gbv == 0 && T.n_periodic_tiles > 0 && (long long)T.ntiles * S <= 512;
+}
+
 void launch_tile_gradient(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
                           hipStream_t st) {
   const size_t lds = tile_grad_lds(T);
+  if (gradient_in_one_launch(T, S)) {
+    (void)hipFuncSetAttribute((const void*)k_grad_tile<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_grad_tile<true>, dim3(T.ntiles, S), dim3(tile_grad_block(T)), lds, st, G, T,
+                       A, 0);
+    return;
+  }
   const int np = T.n_periodic_tiles, nrest = T.ntiles - np;
   if (np > 0) {
     (void)hipFuncSetAttribute((const void*)k_grad_tile<true>,
