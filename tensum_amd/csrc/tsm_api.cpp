@@ -426,6 +426,8 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
     TG.n_periodic_tiles = T.n_periodic_tiles;
     TG.n_gbv = dev_tiles ? dev_tg.n_gbv : (int)T.gbv.size();
     TG.gbv = nullptr;
+    TG.gb_ptr = nullptr;
+    TG.gb_ent = nullptr;
     TG.fused = T.fused ? 1 : 0;
     TG.max_slots1 = T.max_slots1;
     TG.h2beg = TG.halo2 = TG.g1beg = TG.g1ptr = TG.g1cgid = nullptr;
@@ -459,6 +461,16 @@ extern "C" int tsm_create(const tsm_mesh* mesh, const tsm_params* params, int n_
       UT(g1cgid, T.g1cgid); UT(g1ent, T.g1ent);
     }
 #undef UT
+    auto dalloc = [c](void** p, size_t bytes) -> int {
+      char* q = nullptr;
+      const int r = c->alloc(&q, bytes);
+      *p = q;
+      return r;
+    };
+    if ((rc = tsm::build_grad_bnd_table(c->G, TG, dalloc, c->st, msg))) {
+      delete c;
+      return fail(rc, msg);
+    }
   }
   tsm::StageArgs& A = c->A;
   const int S = n_samples;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <functional>
+#include <string>
 
 #include "../../include/tsm.h"
 #include "tsm_physics.h"
@@ -57,6 +59,15 @@ struct DevGeo {
   const unsigned char* snoslip;  // nv
 };
 
+// One operand record per (vertex of k_grad_bnd, cell of that vertex), in the
+// vertex's cell order: the cell's corners (new ids), its non-periodic boundary
+// faces in bface order (-1: none) and, 2 bits each, the corner (0..2) at each
+// end of each of them (bf0 end 0, bf0 end 1, bf1 end 0, bf1 end 1).
+struct alignas(16) GbEntry {
+  int a, b, d, ends;
+  int bf0, bf1, pad0, pad1;
+};
+
 // Tile arrays of the LDS-staged kernels (device pointers, new vertex ids).
 struct TileGeo {
   int ntiles, tb, max_slots, max_faces;
@@ -64,6 +75,8 @@ struct TileGeo {
   int n_periodic_tiles;      // tiles [0, n) hold the periodic sets
   int n_gbv;                 // vertices of k_grad_bnd
   const int* gbv;            // their ids
+  const int* gb_ptr;         // n_gbv+1: their records [gb_ptr[i], gb_ptr[i+1])
+  const GbEntry* gb_ent;
   int fused, max_slots1;     // two-ring tiles: gradients formed inside the stage kernel
   const int *h2beg, *halo2, *g1beg, *g1ptr, *g1cgid;
   const unsigned* g1ent;
@@ -319,6 +332,10 @@ int tile_grad_block(const TileGeo& T);
 int tile_fu_block(const TileGeo& T);
 int tile_fu_block_max();
 bool tile_needs_gradient_pass(const TileGeo& T, int flux, int rec);
+// Builds gb_ptr / gb_ent of T (n_gbv > 0) from its cell entries and G on the
+// stream; memory from dalloc.  TSM_OK, or TSM_E_* with msg.
+int build_grad_bnd_table(const DevGeo& G, TileGeo& T, const std::function<int(void**, size_t)>& dalloc,
+                         hipStream_t st, std::string& msg);
 void launch_tile_gradient_bnd(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
                               hipStream_t st);
 size_t tile_fu_lds(const TileGeo& T, int flux, int rec);

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <string>
+#include <vector>
 
 #include "tsm_device.h"
 #include "tsm_physics.h"
@@ -329,15 +331,15 @@ __global__ __launch_bounds__(512, PER ? 4 : 8) void k_grad_tile_packed(const Pac
 // boundary correction, outside periodic tiles): the same cell order and
 // arithmetic from global memory.  8 lanes per vertex, one cell per lane, then
 // the lane group's first lane adds the cell gradients in cell order (the
-// order of the tile kernel), so the long per-cell load chains run side by side.
+// order of the tile kernel), so the per-cell loads run side by side; each
+// lane reaches its cell's operands through one GbEntry record.
 constexpr int GB_LANES = 8;
 __device__ __forceinline__ void grad_bnd_body(const DevGeo& G, const TileGeo& TG,
                                               const StageArgs& A, int nbv, int bx, int s) {
   const int i = (bx * blockDim.x + threadIdx.x) / GB_LANES, j = threadIdx.x % GB_LANES;
   if (A.ctl[s].done) return;
   const bool live = i < nbv;
-  const int v = live ? TG.gbv[i] : 0;
-  const int k0 = live ? TG.cptr[v] : 0, nc = live ? TG.cptr[v + 1] - k0 : 0;
+  const int k0 = live ? TG.gb_ptr[i] : 0, nc = live ? TG.gb_ptr[i + 1] - k0 : 0;
   const size_t off = (size_t)s * G.nv;
   const Gas gas{A.gamma[s], A.gasR[s]};
   const int base = (threadIdx.x & 63) & ~(GB_LANES - 1);
@@ -349,15 +351,43 @@ __device__ __forceinline__ void grad_bnd_body(const DevGeo& G, const TileGeo& TG
   for (int c0 = 0; c0 < ncm; c0 += GB_LANES) {  // chunks of GB_LANES cells, in cell order
     double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (c0 + j < nc) {
-      const int k = k0 + c0 + j, c = TG.cgid[k];
-      const int va = G.cv[3 * c], vb = G.cv[3 * c + 1], vd = G.cv[3 * c + 2];
-      const Ent Ea = prim2ent(gas, ld_prim(A.W_in, off + va));
-      const Ent Eb = prim2ent(gas, ld_prim(A.W_in, off + vb));
-      const Ent Ed = prim2ent(gas, ld_prim(A.W_in, off + vd));
+      const GbEntry E = TG.gb_ent[k0 + c0 + j];
+      const Prim wa = ld_prim(A.W_in, off + E.a), wb = ld_prim(A.W_in, off + E.b),
+                 wd = ld_prim(A.W_in, off + E.d);
+      const Ent Ea = prim2ent(gas, wa), Eb = prim2ent(gas, wb), Ed = prim2ent(gas, wd);
       const double ea[4] = {Ea.f, Ea.vx, Ea.vy, Ea.l}, eb[4] = {Eb.f, Eb.vx, Eb.vy, Eb.l},
                    ed[4] = {Ed.f, Ed.vx, Ed.vy, Ed.l};
-      cell_grad(G.x[va], G.y[va], G.x[vb], G.y[vb], G.x[vd], G.y[vd], ea, eb, ed, g);
-      if (TG.cent[k] >> 30) grad_bnd_correction(G, A, gas, s, off, c, g);
+      cell_grad(G.x[E.a], G.y[E.a], G.x[E.b], G.y[E.b], G.x[E.d], G.y[E.d], ea, eb, ed, g);
+      // boundary correction (src/fv.cc:926-966), as grad_bnd_correction with
+      // the end states taken from the corners (the same loads and prim2ent)
+      for (int z = 0; z < 2; ++z) {
+        const int bf = z ? E.bf1 : E.bf0;
+        if (bf < 0) break;
+        const int i0 = (E.ends >> (4 * z)) & 3, i1 = (E.ends >> (4 * z + 2)) & 3;
+        const Prim w0 = i0 == 0 ? wa : (i0 == 1 ? wb : wd);
+        const Prim w1 = i1 == 0 ? wa : (i1 == 1 ? wb : wd);
+        const Ent E0 = i0 == 0 ? Ea : (i0 == 1 ? Eb : Ed);
+        const Ent E1 = i1 == 0 ? Ea : (i1 == 1 ? Eb : Ed);
+        const double nx = G.bn[2 * bf], ny = G.bn[2 * bf + 1];
+        const int type = G.btype[bf];
+        Ent q0 = E0, q1 = E1;
+        if (bc_sets_state(type)) {
+          Prim s0 = w0, s1 = w1;
+          bc_apply1(type, bc_values(G, s, bf, 0), nx, ny, s0);
+          bc_apply1(type, bc_values(G, s, bf, 1), nx, ny, s1);
+          q0 = prim2ent(gas, s0);
+          q1 = prim2ent(gas, s1);
+        }
+        const double dq[4] = {(q0.f + q1.f - E0.f - E1.f) / 2.0,
+                              (q0.vx + q1.vx - E0.vx - E1.vx) / 2.0,
+                              (q0.vy + q1.vy - E0.vy - E1.vy) / 2.0,
+                              (q0.l + q1.l - E0.l - E1.l) / 2.0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          g[2 * q] += nx * dq[q];
+          g[2 * q + 1] += ny * dq[q];
+        }
+      }
     }
     for (int m = 0; m < GB_LANES; ++m) {  // every lane takes part in the shuffles
 #pragma unroll
@@ -368,6 +398,7 @@ __device__ __forceinline__ void grad_bnd_body(const DevGeo& G, const TileGeo& TG
     }
   }
   if (live && j == 0) {
+    const int v = TG.gbv[i];
     const double fact = G.gfact[v];
     double* dEv = A.dE + 8 * (off + v);
 #pragma unroll
@@ -1305,6 +1336,84 @@ void launch_tile_gradient(const DevGeo& G, const TileGeo& T, const StageArgs& A,
                        A, np);
   }
 
+}
+
+namespace {
+__global__ void k_gb_count(int n, const int* gbv, const int* cptr, int* cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cnt[i] = cptr[gbv[i] + 1] - cptr[gbv[i]];
+}
+__global__ void k_gb_fill(int n, const int* gbv, const int* cptr, const int* cgid, const int* cv,
+                          const int* cbf, const int* bv, const int* ptr, GbEntry* out, int* bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int v = gbv[i];
+  int o = ptr[i];
+  for (int k = cptr[v]; k < cptr[v + 1]; ++k) {
+    const int c = cgid[k];
+    GbEntry E;
+    E.a = cv[3 * c]; E.b = cv[3 * c + 1]; E.d = cv[3 * c + 2];
+    E.bf0 = cbf[2 * c];
+    E.bf1 = E.bf0 >= 0 ? cbf[2 * c + 1] : -1;
+    E.ends = 0; E.pad0 = E.pad1 = 0;
+    for (int z = 0; z < 2; ++z) {
+      const int bf = z ? E.bf1 : E.bf0;
+      if (bf < 0) break;
+      for (int e = 0; e < 2; ++e) {
+        const int u = bv[2 * bf + e];
+        const int q = u == E.a ? 0 : (u == E.b ? 1 : (u == E.d ? 2 : 3));
+        if (q == 3) atomicOr(bad, 1);
+        E.ends |= (q & 3) << (2 * (2 * z + e));
+      }
+    }
+    out[o++] = E;
+  }
+}
+}  // namespace
+
+int build_grad_bnd_table(const DevGeo& G, TileGeo& T, const std::function<int(void**, size_t)>& dalloc,
+                         hipStream_t st, std::string& msg) {
+  const int n = T.n_gbv;
+  if (n <= 0) return TSM_OK;
+  auto chk = [&](hipError_t e) {
+    if (e != hipSuccess) msg = std::string("boundary gradient table: ") + hipGetErrorString(e);
+    return e == hipSuccess;
+  };
+  int* cnt = nullptr;
+  int* ptr = nullptr;
+  int* bad = nullptr;
+  GbEntry* ent = nullptr;
+  if (dalloc((void**)&cnt, (size_t)n * sizeof(int)) || dalloc((void**)&ptr, (size_t)(n + 1) * sizeof(int)) ||
+      dalloc((void**)&bad, sizeof(int))) {
+    msg = "boundary gradient table: allocation";
+    return TSM_E_HIP;
+  }
+  hipLaunchKernelGGL(k_gb_count, dim3((n + 255) / 256), dim3(256), 0, st, n, T.gbv, T.cptr, cnt);
+  std::vector<int> h(n), hp(n + 1, 0);
+  if (!chk(hipMemcpyAsync(h.data(), cnt, n * sizeof(int), hipMemcpyDeviceToHost, st)) ||
+      !chk(hipStreamSynchronize(st)))
+    return TSM_E_HIP;
+  for (int i = 0; i < n; ++i) hp[i + 1] = hp[i] + h[i];
+  if (dalloc((void**)&ent, std::max<size_t>(hp[n], 1) * sizeof(GbEntry))) {
+    msg = "boundary gradient table: allocation";
+    return TSM_E_HIP;
+  }
+  int hbad = 0;
+  if (!chk(hipMemcpyAsync(ptr, hp.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, st)) ||
+      !chk(hipMemsetAsync(bad, 0, sizeof(int), st)))
+    return TSM_E_HIP;
+  hipLaunchKernelGGL(k_gb_fill, dim3((n + 255) / 256), dim3(256), 0, st, n, T.gbv, T.cptr, T.cgid,
+                     G.cv, G.cbf, G.bv, ptr, ent, bad);
+  if (!chk(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st)) ||
+      !chk(hipStreamSynchronize(st)))
+    return TSM_E_HIP;
+  if (hbad) {
+    msg = "boundary gradient table: a boundary face end is not a corner of its cell";
+    return TSM_E_MESH;
+  }
+  T.gb_ptr = ptr;
+  T.gb_ent = ent;
+  return TSM_OK;
 }
 
 void launch_tile_gradient_bnd(const DevGeo& G, const TileGeo& T, const StageArgs& A, int S,
