@@ -232,7 +232,7 @@ __device__ __forceinline__ void cell_grad(double xa, double ya, double xb, doubl
 // Tiles [t0, t0 + gridDim.x).  PER: periodic tiles, with boundary corrections
 // and the in-tile periodic set sums.  Otherwise vertices with a
 // boundary-corrected cell are left to k_grad_bnd, which keeps this kernel's
-// register count low enough for eight waves per SIMD.
+// register count low enough for seven waves per SIMD.
 template <bool PER>
 __device__ __forceinline__ void grad_tile_body(const DevGeo& G, const TileGeo& TG,
                                                const StageArgs& A, int t, int s) {
@@ -264,12 +264,24 @@ __device__ __forceinline__ void grad_tile_body(const DevGeo& G, const TileGeo& T
     SE[i] = e.f; SE[MS + i] = e.vx; SE[2 * MS + i] = e.vy; SE[3 * MS + i] = e.l;
     SX[i] = G.x[gid]; SX[MS + i] = G.y[gid];
   }
+  // the vertex's first CEP cell entries, requested before the barrier so that
+  // the cell loop does not wait on them one by one
+  constexpr int CEP = 8;
+  unsigned ce[CEP];
+#pragma unroll
+  for (int u = 0; u < CEP; ++u) ce[u] = (kc0 + u < kc1) ? TG.cent[kc0 + u] : 0u;
   __syncthreads();
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   bool skip = false;
   if (own) {
     for (int k = kc0; k < kc1; ++k) {
-      const unsigned e = TG.cent[k];
+      unsigned e = 0;
+      if (k - kc0 < CEP) {
+#pragma unroll
+        for (int u = 0; u < CEP; ++u) if (u == k - kc0) e = ce[u];
+      } else {
+        e = TG.cent[k];
+      }
       if (!PER && (e >> 30)) { skip = true; break; }
       const int a = e & 1023, b = (e >> 10) & 1023, d = (e >> 20) & 1023;
       const double ea[4] = {SE[a], SE[MS + a], SE[2 * MS + a], SE[3 * MS + a]};
@@ -307,8 +319,15 @@ __device__ __forceinline__ void grad_tile_body(const DevGeo& G, const TileGeo& T
     for (int q = 0; q < 8; ++q) dEv[q] = acc[q] * fact;
   }
 }
+// Seven waves per SIMD for the tiles without periodic sets: the LDS of a
+// 128-vertex tile (six slot rows) allows 14 tiles of two waves per CU, so the
+// eighth wave is never resident and its registers go to the prefetched cell
+// entries (measured at 1025^2: 62.1 us per stage against 65.8).
+#ifndef TSM_GRAD_WAVES
+#define TSM_GRAD_WAVES 7
+#endif
 template <bool PER>
-__global__ __launch_bounds__(512, PER ? 4 : 8) void k_grad_tile(DevGeo G, TileGeo TG, StageArgs A,
+__global__ __launch_bounds__(512, PER ? 4 : TSM_GRAD_WAVES) void k_grad_tile(DevGeo G, TileGeo TG, StageArgs A,
                                                                 int t0) {
   grad_tile_body<PER>(G, TG, A, xcd_tile(blockIdx.x, gridDim.x) + t0, blockIdx.y);
 }
@@ -316,7 +335,7 @@ __global__ __launch_bounds__(512, PER ? 4 : 8) void k_grad_tile(DevGeo G, TileGe
 // of every level): PER selects the periodic tiles [0, np) of each level, or
 // the others.
 template <bool PER>
-__global__ __launch_bounds__(512, PER ? 4 : 8) void k_grad_tile_packed(const PackLevel* __restrict__ pk,
+__global__ __launch_bounds__(512, PER ? 4 : TSM_GRAD_WAVES) void k_grad_tile_packed(const PackLevel* __restrict__ pk,
                                                                        int nl) {
   const int b = blockIdx.x;
   int l = 0;
@@ -1300,7 +1319,7 @@ bool tile_needs_gradient_pass(const TileGeo& T, int flux, int rec) {
 }
 
 // A gradient pass of at most two tiles per CU cannot fill the GPU, so the
-// eight-wave occupancy of k_grad_tile<false> buys nothing there, while each
+// seven-wave occupancy of k_grad_tile<false> buys nothing there, while each
 // launch costs several microseconds: on meshes whose periodic sets would split
 // the pass into two launches (periodic tiles, then the rest) and that leave no
 // vertex to k_grad_bnd, such grids take one launch of the periodic variant
