@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="threads per tile of the stage kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-ranks", type=int, default=min(16, os.cpu_count() or 1),
+                    help="MPI ranks of the reference's multi-rank CPU baseline (1: one rank only)")
     ap.add_argument("--mc", action="store_true",
                     help="second metric: surrogate control-variate MC against plain MC on the "
                          "uncertain isentropic vortex (prints its own JSON line)")
@@ -280,60 +282,108 @@ def valu_issue_frac(kernel, dur_us, args):
     return round(n / (dur_us * 1e-6) / VALU_PEAK_WAVE_INSTR_PER_S, 4) if n else None
 
 
-def reference_cpu_baseline(args, mesh, prm, w0):
-    """The reference itself (oracle/_ref/tensum, built from its sources with
-    one MPI rank) on the first steps of the same Riemann workload: the part
-    file of this mesh, the parameter file of the reference's own grammar with
-    final_time at the time of step cpu_steps (from this library's dt, which is
-    the reference's), rate = V x 3 x steps / SAMPLE_SOLVE (the reference's own
-    loop timer).  None when the binary is absent or fails."""
+def _reference_run(mesh, t_k, ranks, d, say):
+    """One run of the reference binary on `ranks` MPI ranks in directory d up
+    to time t_k: (steps, SAMPLE_SOLVE seconds, wall seconds) or None.  More
+    than one rank: the mesh goes through the reference's own partitioner
+    (oracle/_ref/grid_part) as a gmsh file split into x-strips."""
     import subprocess
-    import tempfile
 
-    import tensum_amd as T
-    from cases import write_part_file
+    from cases import write_msh_strips, write_part_file
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     from ref_cases import param_text
     exe = os.path.join(ROOT, "oracle", "_ref", "tensum")
+    os.makedirs(d, exist_ok=True)
+    cmd = [exe, "-i", "param.in", "-npart", str(ranks), "-L", "."]
+    if ranks == 1:
+        write_part_file(mesh, d)
+    else:
+        part = os.path.join(ROOT, "oracle", "_ref", "grid_part")
+        mpirun = "/opt/conda/bin/mpirun"
+        if not (os.access(part, os.X_OK) and os.access(mpirun, os.X_OK)):
+            return None
+        say(f"partitioning the mesh into {ranks} strips (the reference's grid_part)")
+        write_msh_strips(mesh, os.path.join(d, "mesh.msh"), ranks)
+        r = subprocess.run([part, "-D", "2", "-I", os.path.join(d, "mesh.msh"), "-P", str(ranks),
+                            "-L", d], capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            return None
+        os.remove(os.path.join(d, "mesh.msh"))
+        cmd = [mpirun, "-np", str(ranks)] + cmd
+    text = param_text("riemann33", t_k, 10 ** 8).replace(
+        "   restart               yes", "   restart               no")
+    with open(os.path.join(d, "param.in"), "w") as f:
+        f.write(text)
+    c0 = time.perf_counter()
+    r = subprocess.run(cmd, cwd=d, capture_output=True, text=True, timeout=900)
+    wall = time.perf_counter() - c0
+    sd = os.path.join(d, "SAMPLE_0")
+    try:
+        steps = len(open(os.path.join(sd, "residual.dat")).read().split("\n")) - 1
+        solve = None
+        for line in open(os.path.join(sd, "time_log.log")):
+            if line.startswith("SAMPLE_SOLVE"):
+                solve = 60.0 * float(line.split()[2])  # T_MAX over the ranks, minutes
+    except OSError:
+        return None
+    if r.returncode != 0 or not solve or steps < 1:
+        return None
+    return steps, solve, wall
+
+
+def reference_cpu_baseline(args, mesh, prm, w0):
+    """The reference itself (oracle/_ref/tensum, built from its sources) on the
+    first steps of the same Riemann workload, on the host's cores: the mesh
+    partitioned into cpu_ranks x-strips by the reference's own partitioner and
+    run under MPICH's mpirun, and on one rank for comparison.  The parameter
+    file is the reference's own grammar with final_time at the time of the
+    sample's last step (from this library's dt, which is the reference's);
+    rate = V x 3 x steps / SAMPLE_SOLVE (the reference's own loop timer, the
+    slowest rank's).  None when the binary is absent or fails."""
+    import tempfile
+
+    import tensum_amd as T
+    say = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
+    exe = os.path.join(ROOT, "oracle", "_ref", "tensum")
     if not os.access(exe, os.X_OK):
         return None
-    k = max(1, args.cpu_steps)
+    k1 = max(1, args.cpu_steps)
+    ranks = max(1, args.cpu_ranks)
+    kn = k1 * max(1, ranks // 2)  # a sample of similar length on many ranks
     g = T.Solver(mesh, prm)
     g.set_state(w0)
-    _, log = g.run(k, want_log=True)
+    _, log = g.run(max(k1, kn), want_log=True)
     g.close()
-    t_k = float(log[-1, 2])
-    print(f"bench: timing the reference binary on {k} steps (t = {t_k!r})", file=sys.stderr,
-          flush=True)
+    rate = lambda st, sv: mesh.n_vertex * 3 * st / sv
     with tempfile.TemporaryDirectory() as d:
-        write_part_file(mesh, d)
-        text = param_text("riemann33", t_k, 10 ** 8).replace(
-            "   restart               yes", "   restart               no")
-        with open(os.path.join(d, "param.in"), "w") as f:
-            f.write(text)
-        c0 = time.perf_counter()
-        r = subprocess.run([exe, "-i", "param.in", "-npart", "1", "-L", "."], cwd=d,
-                           capture_output=True, text=True, timeout=600)
-        wall = time.perf_counter() - c0
-        sd = os.path.join(d, "SAMPLE_0")
-        try:
-            steps = len(open(os.path.join(sd, "residual.dat")).read().split("\n")) - 1
-            solve = None
-            for line in open(os.path.join(sd, "time_log.log")):
-                if line.startswith("SAMPLE_SOLVE"):
-                    solve = 60.0 * float(line.split()[3])  # T_AVG, minutes
-        except OSError:
-            steps, solve = 0, None
-    if r.returncode != 0 or not solve or steps < 1:
-        print(f"bench: reference run failed ({r.returncode}); timing the oracle instead",
-              file=sys.stderr, flush=True)
+        say(f"timing the reference binary on one rank, {k1} steps")
+        one = _reference_run(mesh, float(log[k1 - 1, 2]), 1, os.path.join(d, "one"), say)
+        many = None
+        if ranks > 1:
+            say(f"timing the reference binary on {ranks} ranks, {kn} steps")
+            many = _reference_run(mesh, float(log[kn - 1, 2]), ranks, os.path.join(d, "many"), say)
+    if one is None:
+        say("reference run failed; timing the oracle instead")
         return None
-    return {"value": mesh.n_vertex * 3 * steps / solve, "unit": "vertex-stage updates/s",
-            "cores": 1, "kind": "reference",
-            "sample": f"{steps} SSP-RK3 steps from t = 0 of the same {args.n}x{args.n} Riemann "
-                      f"workload, the reference built from its sources (one MPI rank), timed by "
-                      f"its SAMPLE_SOLVE loop timer ({solve:.1f} s; {wall:.1f} s with start-up "
-                      f"and preprocessing)"}
+    s1, v1, w1 = one
+    single = {"value": rate(s1, v1), "cores": 1,
+              "sample": f"{s1} steps, SAMPLE_SOLVE {v1:.1f} s ({w1:.1f} s with start-up)"}
+    if many is None:
+        return {"value": single["value"], "unit": "vertex-stage updates/s", "cores": 1,
+                "kind": "reference",
+                "sample": f"{s1} SSP-RK3 steps from t = 0 of the same {args.n}x{args.n} Riemann "
+                          f"workload, the reference built from its sources (one MPI rank), timed by "
+                          f"its SAMPLE_SOLVE loop timer ({v1:.1f} s; {w1:.1f} s with start-up "
+                          f"and preprocessing)"}
+    sn, vn, wn = many
+    return {"value": rate(sn, vn), "unit": "vertex-stage updates/s", "cores": ranks,
+            "kind": "reference",
+            "sample": f"{sn} SSP-RK3 steps from t = 0 of the same {args.n}x{args.n} Riemann workload, "
+                      f"the reference built from its sources on {ranks} MPI ranks (MPICH, one per "
+                      f"core; the mesh in {ranks} x-strips through the reference's own partitioner), "
+                      f"timed by its SAMPLE_SOLVE loop timer, slowest rank ({vn:.1f} s; {wn:.1f} s "
+                      f"with start-up and preprocessing)",
+            "one_rank": single}
 
 
 MC_DEFAULTS = {  # fine, coarse, t_final, train, pilot, pair, ref, fine batch

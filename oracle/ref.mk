@@ -19,7 +19,17 @@ CXXFLAGS := -std=c++11 -O3 -w -DMPICH_SKIP_MPICXX -I$(CONDA)/include -I$(REF)/ex
 SRCS := $(wildcard $(REF)/src/*.cc)
 OBJS := $(patsubst $(REF)/src/%.cc,$(OUT)/obj/%.o,$(SRCS)) $(OUT)/obj/fparser.o
 
-all: $(OUT)/tensum
+# The reference's partitioner (grid_gen/, no MPI): gmsh-2.2 mesh with
+# partition and ghost tags -> PARTITION/part_file_<p>.dat for multi-rank runs
+# of the reference (the CPU baseline on several cores).  Built without
+# optimisation: grid_gen/gmsh_split.cc:84-92 reads each element's tags into
+# tags[1..ntags] of an ntags-long stack array, one past its end (an
+# AddressSanitizer build reports the overflow there); at -O2 that write
+# corrupts the stack and the tool crashes, at -O0 it runs.
+GP_SRCS := $(wildcard $(REF)/grid_gen/*.cc)
+GP_OBJS := $(patsubst $(REF)/grid_gen/%.cc,$(OUT)/gp_obj/%.o,$(GP_SRCS))
+
+all: $(OUT)/tensum $(OUT)/grid_part
 
 $(LIB)/.links:
 	mkdir -p $(LIB) $(OUT)/obj
@@ -34,6 +44,13 @@ $(OUT)/obj/%.o: $(REF)/src/%.cc | $(LIB)/.links
 
 $(OUT)/obj/fparser.o: $(REF)/extern/fparser/fparser.cc | $(LIB)/.links
 	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/gp_obj/%.o: $(REF)/grid_gen/%.cc
+	mkdir -p $(OUT)/gp_obj
+	$(CXX) -std=c++11 -O0 -w -I$(REF)/grid_gen -c $< -o $@
+
+$(OUT)/grid_part: $(GP_OBJS)
+	$(CXX) -o $@ $(GP_OBJS)
 
 $(OUT)/tensum: $(OBJS)
 	$(CXX) -o $@ $(OBJS) -L$(abspath $(LIB)) -lmpi -ldl -Wl,-rpath,$(abspath $(LIB))

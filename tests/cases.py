@@ -119,6 +119,64 @@ def write_part_file(mesh, directory, periodic_tags=()):
         f.write("\n".join(lines) + "\n")
 
 
+def write_msh_strips(mesh, path, nparts):
+    """Writes `mesh` as a gmsh-2.2 ASCII file for the reference's partitioner
+    (grid_gen/gmsh_split.cc:15-192, grid_gen/read_gmsh.cc:16-206), split into
+    `nparts` partitions by x-strips of the cell centroids: each triangle's tags
+    are (1, 1, number of partitions, owner + 1, -(ghost + 1)...), its ghosts
+    being every other partition that owns a cell sharing one of its vertices
+    (readme/README.md:153); each boundary line's tags are (its boundary tag,
+    the same as elementary tag, 1, owner of its cell + 1)."""
+    n = mesh.n_vertex
+    cv = mesh.cell_v.reshape(-1, 3).astype(np.int64)
+    nc = cv.shape[0]
+    cx = (mesh.x[cv[:, 0]] + mesh.x[cv[:, 1]] + mesh.x[cv[:, 2]]) / 3.0
+    x0, x1 = float(mesh.x.min()), float(mesh.x.max())
+    owner = np.minimum((nparts * (cx - x0) / (x1 - x0)).astype(np.int64), nparts - 1)
+    # the smallest and largest owner among each vertex's cells
+    vmin = np.full(n, nparts, dtype=np.int64)
+    vmax = np.full(n, -1, dtype=np.int64)
+    for j in range(3):
+        np.minimum.at(vmin, cv[:, j], owner)
+        np.maximum.at(vmax, cv[:, j], owner)
+    # owners of a vertex's cells are contiguous strips, so a cell's ghosts are
+    # the strips in [min, max] over its corners other than its own
+    lo = np.minimum.reduce([vmin[cv[:, j]] for j in range(3)])
+    hi = np.maximum.reduce([vmax[cv[:, j]] for j in range(3)])
+    # boundary lines: the owner of the cell holding the edge
+    bv = mesh.bface_v.reshape(-1, 2).astype(np.int64)
+    key = lambda a, b: np.minimum(a, b) * n + np.maximum(a, b)
+    bkey = key(bv[:, 0], bv[:, 1])
+    bown = np.full(bv.shape[0], -1, dtype=np.int64)
+    order = np.argsort(bkey)
+    for j in range(3):
+        k = key(cv[:, j], cv[:, (j + 1) % 3])
+        pos = np.searchsorted(bkey[order], k)
+        pos = np.minimum(pos, bkey.size - 1)
+        hit = bkey[order][pos] == k
+        bown[order[pos[hit]]] = owner[hit]
+    assert (bown >= 0).all(), "boundary edge without a cell"
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(n)]
+    lines += [f"{i + 1} {mesh.x[i]:.17g} {mesh.y[i]:.17g} 0" for i in range(n)]
+    lines += ["$EndNodes", "$Elements", str(bv.shape[0] + nc)]
+    e = 0
+    for k in range(bv.shape[0]):
+        e += 1
+        t = int(mesh.bface_tag[k])
+        lines.append(f"{e} 1 4 {t} {t} 1 {bown[k] + 1} {bv[k, 0] + 1} {bv[k, 1] + 1}")
+    for c in range(nc):
+        e += 1
+        o = int(owner[c])
+        gh = [p for p in range(int(lo[c]), int(hi[c]) + 1) if p != o]
+        tags = [1, 1, 1 + len(gh), o + 1] + [-(p + 1) for p in gh]
+        lines.append(f"{e} 2 {len(tags)} " + " ".join(map(str, tags)) +
+                     f" {cv[c, 0] + 1} {cv[c, 1] + 1} {cv[c, 2] + 1}")
+    lines.append("$EndElements")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return owner
+
+
 VORTEX_PARAM = """grid
 {{
    cell      median
