@@ -121,6 +121,13 @@ def main():
                     help="plain-MC reference samples on the fine mesh (4096 / 512)")
     ap.add_argument("--mc-fine-batch", type=int, default=None,
                     help="fine samples per solver context (riemann: 32)")
+    ap.add_argument("--mc-ref-chunk", default=None, metavar="START:COUNT",
+                    help="solve only samples [START, START+COUNT) of the plain-MC reference and "
+                         "save their QoIs to --mc-ref-out (a reference longer than one run)")
+    ap.add_argument("--mc-ref-out", default=None, help=".npz file of --mc-ref-chunk")
+    ap.add_argument("--mc-ref-files", default=None,
+                    help="comma-separated --mc-ref-chunk files that together hold samples "
+                         "[0, --mc-ref) of the reference, used instead of solving it")
     ap.add_argument("--mlmc", action="store_true",
                     help="multilevel MC of the uncertain vortex against plain MC at equal "
                          "GPU-seconds (prints its own JSON line)")
@@ -417,8 +424,29 @@ def mc_bench(args):
     p = (uq.riemann_problem(final_time=d["time"], reconstruct="first") if riemann
          else uq.vortex_problem(final_time=d["time"]))
     say = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
+    if args.mc_ref_chunk:
+        import numpy as np
+        a, b = (int(x) for x in args.mc_ref_chunk.split(":"))
+        Q, sec = uq.reference_chunk(p, d["fine"], a, b, log=say, fine_batch=d["fine_batch"])
+        np.savez(args.mc_ref_out, start=a, count=b, seconds=sec, qoi=Q)
+        print(json.dumps({"reference_chunk": [a, b], "seconds": sec, "file": args.mc_ref_out}),
+              flush=True)
+        return
+    ref = None
+    if args.mc_ref_files:
+        import numpy as np
+        parts = sorted((np.load(f) for f in args.mc_ref_files.split(",")), key=lambda z: int(z["start"]))
+        nxt = 0
+        for z in parts:
+            if int(z["start"]) != nxt:
+                raise SystemExit(f"reference chunks leave a gap at sample {nxt}")
+            nxt += int(z["count"])
+        if nxt != d["ref"]:
+            raise SystemExit(f"reference chunks hold {nxt} samples, --mc-ref is {d['ref']}")
+        ref = (np.vstack([z["qoi"] for z in parts]), sum(float(z["seconds"]) for z in parts))
+        say(f"reference: {nxt} samples from {len(parts)} files")
     r = uq.cv_experiment(p, d["fine"], d["coarse"], d["train"], d["pilot"], d["pair"], 65536,
-                         d["ref"], log=say, fine_batch=d["fine_batch"])
+                         d["ref"], log=say, fine_batch=d["fine_batch"], ref=ref)
     nin = len(p.bounds)
     out = {
         "metric": "MC moment error per GPU-second (control variate vs plain MC, equal error)",

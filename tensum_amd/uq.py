@@ -292,8 +292,18 @@ def _sync_time(t0):
     return time.perf_counter() - t0
 
 
+def reference_chunk(problem: Problem, n_fine, start, count, seed=0, log=None, fine_batch=None):
+    """Samples [start, start + count) of cv_experiment's plain-MC reference
+    (the same scrambled Sobol points, seed + 1000): (QoI per sample, seconds),
+    so that a reference longer than one sitting can be formed in parts."""
+    t0 = time.perf_counter()
+    _, Q = plain_mc(problem, n_fine, count, seed=seed + 1000, skip=start, batch=fine_batch, log=log)
+    return Q, _sync_time(t0)
+
+
 def cv_experiment(problem: Problem, n_fine, n_coarse, n_train, n_pilot, n_pair, n_surr, n_ref,
-                  epochs=300, batch=64, seed=0, log=None, fine_batch=None, coarse_batch=None):
+                  epochs=300, batch=64, seed=0, log=None, fine_batch=None, coarse_batch=None,
+                  ref=None):
     """Surrogate control variate against plain MC on the same fine mesh.
 
     Reference: plain MC with n_ref fine samples (its own cost is not charged).
@@ -302,16 +312,22 @@ def cv_experiment(problem: Problem, n_fine, n_coarse, n_train, n_pilot, n_pair, 
     samples, coarse solver against surrogate; either way independent of the
     paired samples, so the estimate stays unbiased), then n_pair fine samples
     and n_surr surrogate evaluations.  fine_batch / coarse_batch: samples per
-    solver context on each mesh (all at once by default).  Returns the
+    solver context on each mesh (all at once by default).  ref: (QoI of the
+    reference's n_ref samples in order, their seconds) formed beforehand by
+    reference_chunk, instead of solving them here.  Returns the
     realised mean and variance errors of both estimators against the reference
     (RMS over the QoIs with non-zero reference variance, each scaled by the
     reference's standard deviation), their GPU-seconds, and the GPU-seconds
     plain MC would need to reach the control variate's standard error."""
     from .surrogate import Surrogate, _alpha, control_variate
     say = log or (lambda *_: None)
-    t0 = time.perf_counter()
-    _, Q_ref = plain_mc(problem, n_fine, n_ref, seed=seed + 1000, batch=fine_batch, log=say)
-    t_ref = _sync_time(t0)
+    if ref is not None:
+        Q_ref, t_ref = np.asarray(ref[0], dtype=np.float64), float(ref[1])
+        if Q_ref.shape != (n_ref, problem.n_qoi):
+            raise ValueError(f"reference of shape {Q_ref.shape}, expected ({n_ref}, {problem.n_qoi})")
+    else:
+        Q_ref, t_ref = reference_chunk(problem, n_fine, 0, n_ref, seed=seed, log=say,
+                                       fine_batch=fine_batch)
     m_ref, v_ref = moments(Q_ref)
     keep = v_ref > 1e-24 * np.maximum(1.0, m_ref ** 2)
     sd = np.sqrt(v_ref[keep])
