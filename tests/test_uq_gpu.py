@@ -40,6 +40,20 @@ def test_reused_context_gives_the_results_of_fresh_ones():
         s.close()
 
 
+def test_reference_in_chunks_is_the_reference_in_one_run():
+    """reference_chunk: consecutive chunks of the plain-MC reference's Sobol
+    sequence give the QoIs of the whole reference solved in one run, bit for
+    bit, and cv_experiment takes them in place of its own reference."""
+    p = uq.vortex_problem(final_time=0.4)
+    Q_all, _ = uq.reference_chunk(p, 17, 0, 10, seed=2, fine_batch=4)
+    _, Q_one = uq.plain_mc(p, 17, 10, seed=2 + 1000, batch=4)
+    assert np.array_equal(Q_all, Q_one)
+    parts = [uq.reference_chunk(p, 17, a, n, seed=2, fine_batch=4)[0] for a, n in ((0, 3), (3, 7))]
+    assert np.array_equal(np.vstack(parts), Q_all)
+    with pytest.raises(ValueError):  # a reference of the wrong size is refused
+        uq.cv_experiment(p, 17, 17, 8, 0, 4, 64, 12, epochs=1, seed=2, ref=(Q_all, 1.0))
+
+
 def test_batched_samples_match_one_at_a_time_with_material():
     p = uq.vortex_problem(final_time=0.2)
     inp = uq.sample_inputs(4, p.bounds, seed=3)
