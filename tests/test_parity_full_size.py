@@ -14,7 +14,14 @@
 * Navier-Stokes Taylor-Green vortex on the 513^2 mesh (seed 2, Mach 0.1,
   Re 1600, Pr 0.72; KEPEC + viscous fluxes): 20 steps against the oracle,
   kinetic energy and entropy integrals included.
+* The benchmark's Riemann configuration asked for t_final 0.8 against the
+  reference solver's own run (tests/golden/ref_riemann1025_stop.npz, from
+  tests/golden/make_ref_riemann1025_stop.py): the scheme is not positivity
+  preserving, the residual norm grows without bound and turns NaN, and both
+  end their time loop at the same step with the same dt and t.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -24,6 +31,10 @@ from cases import (conserved, rel_l2, riemann, riemann_mesh, taylor_green, tgv_m
                    vortex_mesh)
 
 pytestmark = pytest.mark.gpu
+
+# dt over the last steps before the blow-up against the reference's 12 printed
+# digits (measured: 5.8e-13 for dt, 6.8e-13 for t)
+TOL_DT_STOP = 1e-10
 
 
 def _run_both(mesh, prm, w0, steps):
@@ -75,3 +86,27 @@ def test_taylor_green_513_navier_stokes():
     k2, e2 = o.global_integrals()
     assert abs(k1 - k2) <= 1e-12 * abs(k2)
     assert abs(e1 - e2) <= 1e-12 * abs(e2)
+
+
+def test_riemann_1025_stops_where_the_reference_stops():
+    """The reference, run to t_final 0.8 on this mesh (its partitioner
+    re-prints the coordinates at 15-16 digits, and several MPI ranks sum the
+    shared vertices in another order), ends after the step whose residual norm
+    is NaN; this run ends at the same step, with the reference's dt and t over
+    its last 64 steps (12 printed digits)."""
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_riemann1025_stop.npz"))
+    n_ref = int(ref["steps"])
+    last = ref["iter_dt_t"]
+    m = riemann_mesh(1025)
+    prm = T.Params(flux="kepes_tecno_rusanov", reconstruct="minmod", cfl=0.5, final_time=0.8)
+    s = T.Solver(m, prm)
+    s.set_state(riemann(m.x, m.y))
+    done, log = s.run(n_ref + 20, want_log=True)
+    assert int(done[0]) == n_ref
+    assert s.status()["done"]
+    k = last[:, 0].astype(int) - 1
+    np.testing.assert_array_equal(log[k, 0], last[:, 0])
+    np.testing.assert_allclose(log[k, 2], last[:, 2], rtol=1e-11)
+    np.testing.assert_allclose(log[k, 1], last[:, 1], rtol=TOL_DT_STOP)
+    assert not np.isfinite(log[n_ref - 1, 3])
+
